@@ -1,0 +1,288 @@
+"""DA-cycle benchmark of the LBM-LETKF assimilation cycle on one MI355X.
+
+Workload: the 256^2 x M=64 OSSE (decaying D2Q9 BGK+Smagorinsky turbulence,
+tau=0.503, Cs=0.1; (u,v) observed every 8th node with sigma=0.01; Gaussian
+localization L=12 with a 43.8-cell cutoff; inflation 1.1).  One DA cycle = 100
+LBM steps of all 64 members + one LETKF analysis at all 65,536 gridpoints.
+
+The nature run and its observations for every cycle are generated first and kept
+in HBM; the timed region covers K cycles of the ensemble only (the reference's
+`forecast` + `DA` timers, src/main.cu:95-115).  Both variants are measured:
+plain (per-member LBM launches, FP32 Gram, FP64 Jacobi) and mixed (batched).
+`value` is the mixed variant's DA cycles/s.
+
+Prints ONE JSON line.  --gpus above 1 is not supported (one GPU per box).
+"""
+import argparse
+import ctypes as C
+import importlib.util
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
+LBM_BYTES_PER_NODE_UPDATE = 72  # 9 x 4 B read + 9 x 4 B written
+
+
+def load_binding():
+    spec = importlib.util.spec_from_file_location("lbmletkf", os.path.join(ROOT, "lbm2d-letkf_amd", "lbmletkf.py"))
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["lbmletkf"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+class Hip:
+    """The few HIP runtime calls the harness needs (device buffers, events)."""
+
+    def __init__(self):
+        self.l = C.CDLL("libamdhip64.so")
+        self.l.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+        self.l.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        self.l.hipEventSynchronize.argtypes = [C.c_void_p]
+        self.l.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        self.l.hipFree.argtypes = [C.c_void_p]
+
+    def ck(self, st, what):
+        if st != 0:
+            raise RuntimeError(f"{what} failed with HIP status {st}")
+
+    def malloc(self, nbytes):
+        p = C.c_void_p()
+        self.ck(self.l.hipMalloc(C.byref(p), C.c_size_t(nbytes)), "hipMalloc")
+        return p.value
+
+    def free(self, p):
+        self.l.hipFree(C.c_void_p(p))
+
+    def d2h(self, dst: np.ndarray, src: int):
+        self.ck(self.l.hipMemcpy(C.c_void_p(dst.ctypes.data), C.c_void_p(src), C.c_size_t(dst.nbytes), 2),
+                "hipMemcpy")
+
+    def sync(self):
+        self.ck(self.l.hipDeviceSynchronize(), "hipDeviceSynchronize")
+
+    def event(self):
+        e = C.c_void_p()
+        self.ck(self.l.hipEventCreate(C.byref(e)), "hipEventCreate")
+        return e
+
+    def record(self, e, stream):
+        self.ck(self.l.hipEventRecord(e, C.c_void_p(stream)), "hipEventRecord")
+
+    def elapsed_ms(self, a, b):
+        self.ck(self.l.hipEventSynchronize(b), "hipEventSynchronize")
+        ms = C.c_float()
+        self.ck(self.l.hipEventElapsedTime(C.byref(ms), a, b), "hipEventElapsedTime")
+        return ms.value
+
+    def device_name(self):
+        props = C.create_string_buffer(4096)
+        try:
+            self.l.hipDeviceGetName(props, 256, 0)
+            return props.value.decode(errors="replace")
+        except Exception:
+            return "unknown"
+
+
+def bench_variant(lb, hip, cfg, variant, M, K, W, obs, truth, n_obs, log):
+    cfg = cfg.copy()
+    cfg.variant = variant
+    ens = lb.Context(cfg, M)
+    for k in range(M):
+        ens.init_member(k, 1000 + k)
+    steps = cfg.da_interval
+    nodes = cfg.nx * cfg.ny
+    for c in range(W):
+        ens.forecast(steps)
+        ens.analysis_device(obs + 4 * n_obs * c)
+    ens.synchronize()
+    ens.profile_enable(True)
+    e0, e1 = hip.event(), hip.event()
+    hip.sync()
+    t0 = time.perf_counter()
+    hip.record(e0, ens.stream)
+    for c in range(W, W + K):
+        ens.forecast(steps)
+        ens.analysis_device(obs + 4 * n_obs * c)
+    hip.record(e1, ens.stream)
+    ens.synchronize()
+    hip.sync()
+    wall = time.perf_counter() - t0
+    ev_ms = hip.elapsed_ms(e0, e1)
+    phases = {ph: ens.profile_read(ph) for ph in ("forecast", "obs", "gram", "solve")}
+    last = W + K - 1
+    tp = truth + 4 * 3 * nodes * last
+    stats = ens.ensemble_stats_device(tp, tp + 4 * nodes, tp + 8 * nodes)
+    final_state = ens.get_all_states()
+    ens.close()
+    name = "plain" if variant == lb.VARIANT_PLAIN else "mixed"
+    log(f"[{name}] {K} cycles in {wall:.3f} s wall ({ev_ms / 1e3:.3f} s events): "
+        f"{K / wall:.2f} cycles/s; analysis rmse {stats[0]:.4e} spread {stats[1]:.4e}")
+    for ph, (sec, n, w) in phases.items():
+        log(f"    {ph:9s} {sec * 1e3 / K:9.3f} ms/cycle  launches {n}  work {w}")
+    return dict(wall=wall, event_s=ev_ms / 1e3, phases=phases, stats=stats, steps=steps, state=final_state)
+
+
+def kernel_table(r, M, nodes, K):
+    """Average per-launch duration and algorithmic rate of the main kernels."""
+    out = {}
+    sec, n, work = r["phases"]["forecast"]
+    n_steps = r["steps"] * K
+    lbm_launches_per_step = max(1, (n - 1) // n_steps) if n_steps else 1
+    avg = sec / n_steps  # one forecast step (all launches of the step)
+    out["lbm_step"] = dict(avg_ms=avg * 1e3, launches_per_step=lbm_launches_per_step,
+                           achieved=LBM_BYTES_PER_NODE_UPDATE * M * nodes / avg / 1e9, unit="GB/s",
+                           peak=HBM_PEAK_GBS, bound="hbm", share=sec)
+    sec, n, sweeps = r["phases"]["solve"]
+    flops = 9.0 * M**3 * sweeps  # two-sided rotation update (12M) + V update (6M) per rotation
+    out["letkf_solve"] = dict(avg_ms=sec / max(n, 1) * 1e3, achieved=flops / sec / 1e12 if sec else 0.0,
+                              unit="TFLOP/s", peak=FP64_PEAK_TFLOPS, bound="mfma",
+                              engine="fp64 vector ALU (MI355X fp64 vector peak = fp64 matrix peak)",
+                              sweeps_per_gridpoint=sweeps / (nodes * K), share=sec)
+    sec, n, pobs = r["phases"]["gram"]
+    gflops = 2.0 * M * M * pobs
+    out["letkf_gram"] = dict(avg_ms=sec / max(n, 1) * 1e3, achieved=gflops / sec / 1e12 if sec else 0.0,
+                             unit="TFLOP/s", peak=FP32_PEAK_TFLOPS, bound="mfma",
+                             engine="fp32 vector ALU", local_obs_per_gridpoint=pobs / (nodes * K), share=sec)
+    sec, n, _ = r["phases"]["obs"]
+    out["obs_prepare"] = dict(avg_ms=sec / max(n, 1) * 1e3, share=sec)
+    total = sum(v["share"] for v in out.values())
+    for v in out.values():
+        v["share"] = v["share"] / total if total else 0.0
+    return out
+
+
+def cpu_baseline(cfg, f_state, yo, log):
+    """The FP64 oracle on the host cores over a bounded sample of one cycle,
+    starting from the GPU run's final ensemble and its last observations."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_ref as orc
+
+    threads = min(16, os.cpu_count() or 1)
+    orc.set_threads(threads)
+    n = cfg.nx
+    M = f_state.shape[0]
+    f = f_state.astype(np.float64)
+    steps = 2
+    t0 = time.perf_counter()
+    for s in range(steps):
+        f = orc.lbm_step(cfg, f, s)
+    t_step = (time.perf_counter() - t0) / steps
+    stride = 64
+    t0 = time.perf_counter()
+    orc.letkf_analysis(cfg, f, yo, g_stride=stride)
+    t_ana = (time.perf_counter() - t0) * stride
+    cycle = cfg.da_interval * t_step + t_ana
+    sample = (f"{steps} LBM steps of all {M} members at {n}^2 (x{cfg.da_interval} for one cycle) + "
+              f"LETKF analysis at every {stride}th gridpoint ({n * n // stride} local analyses, x{stride}); "
+              f"FP64, OpenMP")
+    log(f"[cpu] step {t_step * 1e3:.1f} ms, analysis {t_ana:.2f} s (extrapolated), cycle {cycle:.2f} s, "
+        f"{threads} threads")
+    return dict(value=1.0 / cycle, unit="DA cycles/s", cores=threads, kind="port", sample=sample,
+                lbm_step_s=t_step, analysis_s=t_ana)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20, help="timed DA cycles")
+    ap.add_argument("--warmup", type=int, default=2, help="untimed DA cycles")
+    ap.add_argument("--members", type=int, default=64)
+    ap.add_argument("--variants", default="plain,mixed")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: --gpus {args.gpus} is not supported: this build targets one MI355X; "
+              "the multi-GPU layout is described in DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    lb = load_binding()
+    hip = Hip()
+    cfg = lb.preset("baseline")
+    M, K, W = args.members, args.steps, args.warmup
+    nodes = cfg.nx * cfg.ny
+    log(f"device: {hip.device_name()}  {lb.version()}  workload 256^2 x M={M}, K={K}, W={W}")
+
+    # nature run and observations of every cycle, resident in HBM
+    nat = lb.Context(cfg, 1)
+    nat.init_member(0, 0)
+    _, n_obs = nat.obs_count()
+    obs = hip.malloc(4 * n_obs * (K + W))
+    truth = hip.malloc(4 * 3 * nodes * (K + W))
+    for c in range(K + W):
+        nat.forecast(cfg.da_interval)
+        nat.observe_device(0, 2024, c, obs + 4 * n_obs * c, noise=True)
+        t = truth + 4 * 3 * nodes * c
+        nat.get_macro_device(0, t, t + 4 * nodes, t + 8 * nodes)
+    nat.synchronize()
+    nat.close()
+    log(f"nature run + {K + W} observation vectors ready ({n_obs} obs each)")
+
+    res = {}
+    for v in args.variants.split(","):
+        var = lb.VARIANT_PLAIN if v == "plain" else lb.VARIANT_MIXED
+        res[v] = bench_variant(lb, hip, cfg, var, M, K, W, obs, truth, n_obs, log)
+    head = res.get("mixed") or res[list(res)[0]]
+    kern = kernel_table(head, M, nodes, K)
+    dom_name = max(kern, key=lambda k: kern[k]["share"])
+    dom = kern[dom_name]
+    roof = dict(kernel=dom_name, bound=dom.get("bound", "hbm"), achieved=dom.get("achieved"),
+                peak=dom.get("peak"), unit=dom.get("unit"),
+                frac=(dom["achieved"] / dom["peak"]) if dom.get("peak") else None, traffic=None,
+                avg_launch_ms=dom["avg_ms"], share_of_cycle=dom["share"])
+    if "engine" in dom:
+        roof["engine"] = dom["engine"]
+    lbm = kern["lbm_step"]
+    out = {
+        "metric": "DA cycles/s at 256²×M=64 (plain vs mixed+batched) at equal analysis RMSE",
+        "value": K / head["wall"],
+        "unit": "DA cycles/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": head["wall"] / K * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 (LBM, Gram) + fp64 (Jacobi, transform)",
+        "data": "synthetic",
+        "config": {"workload": "LETKF OSSE 256x256, M=64 members, 100 LBM steps per DA cycle, "
+                               "(u,v) every 8th node, Gaussian L=12 cutoff 43.8, beta=1.1",
+                   "nx": cfg.nx, "ny": cfg.ny, "members": M, "steps_per_cycle": cfg.da_interval,
+                   "n_obs": n_obs, "variant_timed": "mixed" if "mixed" in res else list(res)[0]},
+        "variants": {k: {"cycles_per_s": K / r["wall"], "ms_per_cycle": r["wall"] / K * 1e3,
+                         "analysis_rmse": r["stats"][0], "analysis_spread": r["stats"][1],
+                         "phase_ms_per_cycle": {p: r["phases"][p][0] / K * 1e3 for p in r["phases"]}}
+                     for k, r in res.items()},
+        "roofline": roof,
+        "roofline_lbm": dict(bound="hbm", achieved=lbm["achieved"], peak=HBM_PEAK_GBS, unit="GB/s",
+                             frac=lbm["achieved"] / HBM_PEAK_GBS, traffic=None,
+                             avg_launch_ms=lbm["avg_ms"] / lbm["launches_per_step"]),
+        "kernels": kern,
+    }
+    if "plain" in res and "mixed" in res:
+        out["speedup_mixed_over_plain"] = res["plain"]["wall"] / res["mixed"]["wall"]
+        a, b = res["mixed"]["stats"][0], res["plain"]["stats"][0]
+        out["rmse_rel_diff_mixed_vs_plain"] = abs(a - b) / b
+    if not args.no_cpu:
+        yo_last = np.empty(n_obs, np.float32)
+        hip.d2h(yo_last, obs + 4 * n_obs * (K + W - 1))
+        out["cpu_baseline"] = cpu_baseline(cfg, head["state"], yo_last.astype(np.float64), log)
+    hip.free(obs)
+    hip.free(truth)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,214 @@
+/*
+ * lbmletkf.h -- C-ABI of the MI355X-native LBM-LETKF assimilation cycle.
+ *
+ * One context holds a whole ensemble of M members as one structure-of-arrays
+ * batch in a single GPU's HBM:  f[member][q][j][i]  (FP32, double-buffered),
+ * with the D2Q9 population order of the reference, q = (ex+1) + 3*(ey+1)
+ * (src/config.h:256-258) and node order ij = i + nx*j (src/config.h:210).
+ *
+ * Every entry point returns an int status (LBMLETKF_OK == 0, negative on error);
+ * lbmletkf_last_error() returns a message.  No C++ exception crosses this ABI.
+ * Calls on one context are ordered on that context's HIP stream; entry points
+ * that return data to the host synchronise that stream.  A context is not
+ * thread-safe; distinct contexts are independent.
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   update(data&, int)                         src/lbm.h:16-29
+ *   data(int) / stdata::init_lbm / init_force  src/data.h:45-64, src/data.cu:25-237
+ *   DataAssimilator(const util::mpi&)          src/assimilate.h:31-40
+ *   DataAssimilator::assimilate(data&, mpi, t) src/assimilate.h:54-68 -> src/letkf.cu:120-360
+ *   util::letkf_solver<T>                      src/util/letkf_solver.h:25-400
+ *   ensemble_stat (RMSE / spread)              ensemble_stat/main.cpp:108-177
+ */
+#ifndef LBMLETKF_H_
+#define LBMLETKF_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBMLETKF_OK              0
+#define LBMLETKF_ERR_ARG        -1  /* invalid argument / unsupported size        */
+#define LBMLETKF_ERR_HIP        -2  /* HIP runtime error (allocation, launch ...)  */
+#define LBMLETKF_ERR_NUMERIC    -3  /* NaN / non-finite value detected on device   */
+#define LBMLETKF_ERR_STATE      -4  /* call out of order (e.g. no obs network set) */
+
+/* forcing kinds */
+#define LBMLETKF_FORCE_NONE          0
+#define LBMLETKF_FORCE_WATANABE97    1   /* src/data.cu:227-350 */
+/* initial-condition kinds */
+#define LBMLETKF_IC_REFERENCE        0   /* src/data.cu:25-141: mt19937 phases, quadrant modes, max-speed scaling */
+#define LBMLETKF_IC_K4_SPECTRUM      1   /* E(k) ~ k^4 exp(-2 (k/k0)^2), half-plane modes, rms-speed scaling */
+/* observed variables (bit mask) -- src/letkf.cu:125, DA_OBS_RUV src/config.h:44 */
+#define LBMLETKF_OBS_U   1
+#define LBMLETKF_OBS_V   2
+#define LBMLETKF_OBS_RHO 4
+/* localization kinds */
+#define LBMLETKF_LOC_GC_WINDOW       0   /* reference window + Gaspari-Cohn R-localization, src/letkf.cu:44-56,97-113 */
+#define LBMLETKF_LOC_GAUSS           1   /* exp(-d^2 / 2L^2) on periodic distance, hard cutoff */
+/* analysis variants */
+#define LBMLETKF_VARIANT_PLAIN       0   /* per-member LBM launches, FP32 Gram on VALU, FP64 Jacobi */
+#define LBMLETKF_VARIANT_MIXED       1   /* batched LBM + fused LETKF, mixed precision */
+
+/* All physical / numerical parameters of a run.  The reference fixes these at
+ * compile time in src/config.h and the makefile macros
+ * (config/makefile.macro.in:1-69); here they are a runtime struct. */
+typedef struct lbmletkf_config {
+    int    nx, ny;              /* grid (config.h:133-134)                                  */
+    double c;                   /* lattice speed c = u_ref/cfl (config.h:248)               */
+    double dx, dt;              /* config.h:250-251                                         */
+    double nu;                  /* molecular viscosity (config.h:157)                       */
+    double rho_ref;             /* config.h:156                                             */
+    double cs_smag;             /* Smagorinsky constant (literal 0.2 at lbm.cu:59)          */
+    double u_ref;               /* velocity scale multiplying e_q in the force term (lbm.h:47) */
+    double friction_rate;       /* linear friction per step (config.h:180, lbm.h:52)        */
+    /* stochastic forcing, Watanabe (1997) -- data.cu:227-350 */
+    int    force_kind;
+    double force_kf;            /* injected wavenumber (config.h:161)                       */
+    double force_amp_total;     /* fkf * u_ref / t0 (config.h:171, data.cu:284-285)         */
+    double force_sigma_k;       /* Gaussian width in |k| (data.cu:236)                      */
+    double force_dk;            /* clipping half-width in |k| (data.cu:237)                 */
+    double force_rand_stdev;    /* N(0, s^2) draws, s = 2^-1/4 (data.cu:309)                */
+    unsigned long long force_seed;
+    /* initial condition -- data.cu:25-141 */
+    int    ic_kind;
+    double ic_amp;              /* max speed (REFERENCE) or rms speed (K4_SPECTRUM)         */
+    double ic_k0, ic_dk, ic_sigma;
+    /* observation network and LETKF -- letkf.cu:29-56, letkf_solver.h:235-349 */
+    int    obs_stride;          /* DA_XYPRUNE (config.h:59-61)                              */
+    int    obs_offset;          /* lattice offset of the observed sites                     */
+    int    obs_vars;            /* LBMLETKF_OBS_* mask                                      */
+    double obs_sigma_u;         /* OBS_ERROR_U  (config.h:190-202)                          */
+    double obs_sigma_rho;       /* OBS_ERROR_RHO                                            */
+    int    loc_kind;            /* LBMLETKF_LOC_*                                           */
+    int    loc_rloc_len;        /* LETKF_RLOC_LEN (0 = not defined)  letkf.cu:29-39         */
+    double loc_scale;           /* Gaussian L (cells)                                       */
+    double loc_cutoff;          /* Gaussian hard cutoff (cells)                             */
+    double inflation;           /* LETKF_COVINF beta (config.h:48-50)                       */
+    int    da_interval;         /* LBM steps per DA cycle (iiter*DAPRUNE, config.h:136,142) */
+    int    variant;             /* LBMLETKF_VARIANT_*                                       */
+    double jacobi_tol;          /* relative off-diagonal tolerance of the eigensolver       */
+    int    jacobi_max_sweeps;   /* cusolver syevj uses tol 1e-6, 100 sweeps (eigensolver_wrapper.hpp:192-193) */
+} lbmletkf_config;
+
+typedef struct lbmletkf_ctx lbmletkf_ctx;
+
+/* Library version string; never fails, needs no GPU. */
+const char* lbmletkf_version(void);
+
+/* Fill *cfg with a named preset: "ref" (src/config.h constants at SCALER=4,
+ * Cs=0.2, Watanabe97 forcing, GC window) or "baseline" (lattice units, tau=0.503,
+ * Cs=0.1, decaying turbulence, (u,v) every 8th node, Gaussian L=12).  Needs no GPU. */
+int lbmletkf_config_preset(const char* name, lbmletkf_config* cfg);
+
+/* Validate a configuration without creating a context.  Needs no GPU. */
+int lbmletkf_config_check(const lbmletkf_config* cfg, int n_members);
+
+/* Create a context for n_members ensemble members on the current HIP device.
+ * Replaces data(int) (src/data.h:55) for all members at once and
+ * DataAssimilator(mpi) (src/assimilate.h:31).  All state starts at rest (rho=rho_ref). */
+int lbmletkf_create(const lbmletkf_config* cfg, int n_members, lbmletkf_ctx** out);
+int lbmletkf_destroy(lbmletkf_ctx* ctx);
+
+/* Last error message of ctx (or of the calling thread when ctx is NULL). */
+const char* lbmletkf_last_error(const lbmletkf_ctx* ctx);
+
+/* Initial condition of one member from cfg.ic_* with the given seed; replaces
+ * stdata::init_lbm (src/data.cu:25-141; the reference passes ensemble_id[+334]). */
+int lbmletkf_init_member(lbmletkf_ctx* ctx, int member, unsigned long long seed);
+
+/* Host copies of one member's populations, layout [q][j][i] (src/config.h:212). */
+int lbmletkf_set_state(lbmletkf_ctx* ctx, int member, const float* f_host);
+int lbmletkf_get_state(lbmletkf_ctx* ctx, int member, float* f_host);
+
+/* f := feq(rho,u,v) for one member (src/lbm.h:31-39); host arrays [j][i]. */
+int lbmletkf_set_macro(lbmletkf_ctx* ctx, int member, const float* rho, const float* u, const float* v);
+
+/* rho,u,v of one member from its populations (src/lbm.h:72-109).  Host arrays [j][i];
+ * any pointer may be NULL.  lbmletkf_get_macro_device writes device arrays, stream-ordered. */
+int lbmletkf_get_macro(lbmletkf_ctx* ctx, int member, float* rho, float* u, float* v);
+int lbmletkf_get_macro_device(lbmletkf_ctx* ctx, int member, float* rho, float* u, float* v);
+
+/* Advance all members by n_steps LBM steps; replaces n calls of
+ * update(data&, t) (src/lbm.h:16-29): forcing -> pull-stream + moments ->
+ * Smagorinsky nu_t -> BGK collide + force -> swap.  Stream-ordered, non-blocking. */
+int lbmletkf_forecast(lbmletkf_ctx* ctx, int n_steps);
+
+/* Step counter that indexes the forcing random stream (the reference consumes
+ * its cuRAND buffer once per update(), src/data.cu:305-313). */
+int lbmletkf_set_step(lbmletkf_ctx* ctx, long long step);
+int lbmletkf_get_step(lbmletkf_ctx* ctx, long long* step);
+
+/* Observation network and localization from cfg.obs_* / cfg.loc_*; builds the
+ * per-gridpoint local observation sets on the device.  Replaces init_letkf /
+ * set_rR (src/letkf.cu:59-118).  Called implicitly by the first analysis. */
+int lbmletkf_setup_obs(lbmletkf_ctx* ctx);
+
+/* Number of observed sites and of scalar observations (sites x observed vars).
+ * Observation vectors are ordered site-major: o = site*n_vars + var, with site
+ * s = sx + sy*(nx/stride) and vars in the order u, v, rho (those in the mask). */
+int lbmletkf_obs_count(lbmletkf_ctx* ctx, int* n_sites, int* n_obs);
+
+/* Synthetic observations of one member (normally the nature run, member 0 of
+ * a one-member context): y = H(x) + sigma*N(0,1), noise from Philox4x32-10 keyed
+ * by (seed, counter); noise=0 gives H(x).  Output has n_obs floats; device or host.
+ * Replaces the OBSERVE branch of output() (src/io.cu:160-246). */
+int lbmletkf_observe(lbmletkf_ctx* ctx, int member, unsigned long long seed,
+                     unsigned long long counter, int noise, float* yo, int yo_on_device);
+
+/* Gather observations from full fields laid out like the reference's
+ * io/observed files (float [ny][nx], NaN where unobserved; src/letkf.cu:197-221,
+ * 237-269).  Any field may be NULL if its variable is not observed. Host arrays. */
+int lbmletkf_obs_from_fields(lbmletkf_ctx* ctx, const float* rho, const float* u,
+                             const float* v, float* yo_host);
+
+/* One LETKF analysis of all members at all gridpoints with observation vector yo
+ * (n_obs floats, device or host pointer).  Replaces DataAssimilator::assimilate
+ * (src/assimilate.h:54-68 -> src/letkf.cu:120-360 and letkf_solver::solve_evd /
+ * solve_xa, src/util/letkf_solver.h:235-349).  Stream-ordered; a NaN in the
+ * analysis is reported by the next synchronising call (LBMLETKF_ERR_NUMERIC). */
+int lbmletkf_analysis(lbmletkf_ctx* ctx, const float* yo, int yo_on_device);
+
+/* One full DA cycle: analysis(yo) then forecast(cfg.da_interval) -- the order of
+ * src/main.cu:95-113.  In the MIXED variant the cycle is replayed as one hipGraph. */
+int lbmletkf_cycle(lbmletkf_ctx* ctx, const float* yo_device);
+
+/* Ensemble statistics against a truth (ensemble_stat/main.cpp:108-177, with the
+ * actual grid size rather than its hard-coded 128*128).  truth arrays [j][i], device
+ * or host; out = {rmse_uv, spread_uv, rmse_rho, spread_rho}.  Spread uses the biased
+ * (1/M) standard deviation as the reference does. */
+int lbmletkf_ensemble_stats(lbmletkf_ctx* ctx, const float* truth_rho, const float* truth_u,
+                            const float* truth_v, int truth_on_device, double out[4]);
+
+/* Local observation set of one gridpoint (parity diagnostics): site indices and
+ * localization weights in gather order.  *n receives the count; at most cap are written. */
+int lbmletkf_local_obs(lbmletkf_ctx* ctx, int gridpoint, int* n, int* sites, float* weights, int cap);
+
+/* Record the ensemble transform of gridpoints [g0, g0+count) at every following
+ * analysis (count=0 disables); lbmletkf_get_dump returns, per gridpoint, W (MxM
+ * row-major), w (M) and the ascending eigenvalues of C (M), in FP64.
+ * Diagnostics counterpart of the reference's HDF5 dumps (src/util/letkf_solver.h:244-345). */
+int lbmletkf_set_dump(lbmletkf_ctx* ctx, int g0, int count);
+int lbmletkf_get_dump(lbmletkf_ctx* ctx, double* W, double* w, double* evals);
+
+/* Phase timers on the context's stream (HIP events around each phase), the
+ * counterpart of the reference's util::timer tags (src/util/timer.hpp:17-85,
+ * src/main.cu:95-115, src/letkf.cu:217-403).  enable(1) clears and starts them.
+ * Phases: "forecast" (all LBM kernels incl. forcing), "obs" (H(x), innovations),
+ * "gram" (C and b), "solve" (eigensolve, transform, state update).
+ * read() returns the accumulated seconds, the number of kernel launches and a
+ * work count: node updates (forecast), observations (obs), local scalar
+ * observations summed over gridpoints (gram), Jacobi sweeps summed over
+ * gridpoints (solve).  read() synchronises the stream. */
+int lbmletkf_profile_enable(lbmletkf_ctx* ctx, int on);
+int lbmletkf_profile_read(lbmletkf_ctx* ctx, const char* phase, double* seconds,
+                          long long* launches, long long* work);
+
+/* Stream access for callers that time or order work around the context. */
+void* lbmletkf_get_stream(lbmletkf_ctx* ctx);
+int lbmletkf_synchronize(lbmletkf_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LBMLETKF_H_ */

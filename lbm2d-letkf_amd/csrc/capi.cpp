@@ -1,0 +1,919 @@
+// C-ABI of the LBM-LETKF cycle: configuration presets, context lifetime, the
+// initial conditions (host, once per member) and the orchestration of the
+// forecast and analysis kernels on the context's stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <new>
+#include <random>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "context.hpp"
+#include "philox.hpp"
+
+using namespace lbmletkf;
+
+namespace {
+
+thread_local std::string g_thread_error;
+constexpr double PI = 3.14159265358979323846;
+
+int fail(Ctx* c, int code, const std::string& msg) {
+    if (c) c->last_error = msg;
+    g_thread_error = msg;
+    return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                              \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return fail((ctx), LBMLETKF_ERR_HIP,                                        \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));             \
+    } while (0)
+
+template <class T>
+hipError_t dalloc(T** p, size_t n) {
+    *p = nullptr;
+    if (n == 0) n = 1;
+    return hipMalloc(reinterpret_cast<void**>(p), n * sizeof(T));
+}
+
+
+// ---------------------------------------------------------------------------
+// forcing modes, src/data.cu:227-290 (kx, ky in [0, kf+dk), kf-dk <= |k| <= kf+dk,
+// amp ~ exp(-(|k|-kf)^2/sigma) normalised so that sum(amp) = fkf*u_ref/t0)
+// ---------------------------------------------------------------------------
+void build_force_table(const lbmletkf_config& cfg, ForceTable& t) {
+    t.n = 0;
+    if (cfg.force_kind != LBMLETKF_FORCE_WATANABE97) return;
+    const int kmax_search = int(cfg.force_kf + cfg.force_dk);
+    long double asum = 0;
+    std::vector<float> kx, ky, amp;
+    for (int y = 0; y < kmax_search; ++y) {
+        for (int x = 0; x < kmax_search; ++x) {
+            if (x == 0 && y == 0) continue;
+            const double k = std::sqrt(double(y * y + x * x));
+            if (cfg.force_kf - cfg.force_dk <= k && k <= cfg.force_kf + cfg.force_dk) {
+                const float a = float(std::exp(-(k - cfg.force_kf) * (k - cfg.force_kf) / cfg.force_sigma_k));
+                asum += a;
+                kx.push_back(float(x));
+                ky.push_back(float(y));
+                amp.push_back(a);
+            }
+        }
+    }
+    const int n = std::min<int>(int(kx.size()), MAX_FORCE_MODES);
+    t.n = n;
+    for (int m = 0; m < n; ++m) {
+        t.kx[m] = kx[m];
+        t.ky[m] = ky[m];
+        t.amp[m] = float((long double)amp[m] * (long double)cfg.force_amp_total / asum);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// initial velocity fields.  Both kinds are the velocity of a random-phase
+// streamfunction psi = sum_k p(k) sin(kx i + ky j + theta_k):
+//   u = sum p ky cos(kx i + ky j + theta),  v = -sum p kx cos(...)
+// (src/data.cu:142-176), evaluated here by two separable passes in FP64.
+// ---------------------------------------------------------------------------
+struct Mode { int ki, kj; double p, theta; };
+
+void modes_reference(const lbmletkf_config& cfg, unsigned long long seed, int nx,
+                     std::vector<Mode>& modes) {
+    // src/data.cu:107-139: std::mt19937(seed), uniform_real_distribution<float>(-1,1),
+    // p(k) = exp(-(k-k0)^2/sigma) on k0-dk <= k <= k0+dk, quadrant 0 <= ki,kj < nx/2
+    const int nk = nx / 2;
+    std::mt19937 eng(static_cast<std::mt19937::result_type>(seed));
+    std::uniform_real_distribution<float> dist(-1, 1);
+    const float k0 = float(cfg.ic_k0), dk = float(cfg.ic_dk), sigma = float(cfg.ic_sigma);
+    for (int kj = 0; kj < nk; ++kj) {
+        for (int ki = 0; ki < nk; ++ki) {
+            const float k = std::sqrt(float(ki * ki + kj * kj));
+            const float pk = (k0 - dk <= k && k <= k0 + dk) ? std::exp(-(k - k0) * (k - k0) / sigma) : 0.f;
+            const float theta = float(dist(eng) * PI);
+            if (pk != 0.f) modes.push_back({ki, kj, double(pk), double(theta)});
+        }
+    }
+}
+
+void modes_k4(const lbmletkf_config& cfg, unsigned long long seed, int nx, int ny,
+              std::vector<Mode>& modes) {
+    // E(k) ~ k^4 exp(-2 (k/k0)^2) over the half plane kj >= 0 (one of each +-k
+    // pair); streamfunction amplitude p = sqrt(E(k)/k^3) = sqrt(k) exp(-(k/k0)^2).
+    const int nk = std::min(nx, ny) / 2;
+    for (int kj = 0; kj < nk; ++kj) {
+        for (int ki = -nk + 1; ki < nk; ++ki) {
+            if (kj == 0 && ki <= 0) continue;
+            const double k = std::sqrt(double(ki * ki + kj * kj));
+            if (k >= nk) continue;
+            u32x4 ctr{{uint32_t(ki + nk), uint32_t(kj), 0u, STREAM_IC}};
+            const u32x4 r = philox4x32_10(ctr, uint32_t(seed), uint32_t(seed >> 32));
+            const double theta = PI * (2.0 * u32_to_open01(r.v[0]) - 1.0);
+            const double p = std::sqrt(k) * std::exp(-(k / cfg.ic_k0) * (k / cfg.ic_k0));
+            modes.push_back({ki, kj, p, theta});
+        }
+    }
+}
+
+template <class F>
+void parallel_for(int n, F fn) {
+    int nt = int(std::thread::hardware_concurrency());
+    nt = std::max(1, std::min(nt, 16));
+    nt = std::min(nt, n);
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; ++t)
+        th.emplace_back([=, &fn] {
+            for (int a = t; a < n; a += nt) fn(a);
+        });
+    for (auto& x : th) x.join();
+}
+
+void velocity_from_modes(const std::vector<Mode>& modes, int nx, int ny, std::vector<double>& u,
+                         std::vector<double>& v) {
+    using cd = std::complex<double>;
+    int kj_max = 0;
+    for (const auto& m : modes) kj_max = std::max(kj_max, m.kj);
+    const int nkj = kj_max + 1;
+    std::vector<cd> ex(nx), ey(ny);
+    for (int a = 0; a < nx; ++a) ex[a] = std::polar(1.0, 2 * PI * a / nx);
+    for (int a = 0; a < ny; ++a) ey[a] = std::polar(1.0, 2 * PI * a / ny);
+    // group modes by kj
+    std::vector<std::vector<int>> by_kj(nkj);
+    for (int n = 0; n < int(modes.size()); ++n) by_kj[modes[n].kj].push_back(n);
+    std::vector<cd> Au(size_t(nkj) * nx), Av(size_t(nkj) * nx);
+    parallel_for(nkj, [&](int kj) {
+        for (int i = 0; i < nx; ++i) {
+            cd su(0, 0), sv(0, 0);
+            for (int n : by_kj[kj]) {
+                const Mode& m = modes[n];
+                const int idx = ((m.ki * i) % nx + nx) % nx;
+                const cd e = ex[idx] * std::polar(1.0, m.theta);
+                const double kx = 2 * PI * m.ki / nx;
+                su += m.p * e;
+                sv += -m.p * kx * e;
+            }
+            Au[size_t(kj) * nx + i] = su;
+            Av[size_t(kj) * nx + i] = sv;
+        }
+    });
+    u.assign(size_t(nx) * ny, 0.0);
+    v.assign(size_t(nx) * ny, 0.0);
+    parallel_for(ny, [&](int j) {
+        for (int i = 0; i < nx; ++i) {
+            double su = 0, sv = 0;
+            for (int kj = 0; kj < nkj; ++kj) {
+                if (by_kj[kj].empty()) continue;
+                const cd e = ey[(size_t(kj) * j) % ny];
+                const double ky = 2 * PI * kj / ny;
+                su += ky * (Au[size_t(kj) * nx + i] * e).real();
+                sv += (Av[size_t(kj) * nx + i] * e).real();
+            }
+            u[size_t(j) * nx + i] = su;
+            v[size_t(j) * nx + i] = sv;
+        }
+    });
+}
+
+int check_config(const lbmletkf_config* cfg, int n_members, std::string& why) {
+    auto bad = [&](const std::string& m) { why = m; return LBMLETKF_ERR_ARG; };
+    if (!cfg) return bad("config is NULL");
+    if (n_members < 1 || n_members > MAX_MEMBERS) return bad("n_members must be in [1, 64]");
+    if (cfg->nx < 4 || cfg->ny < 4) return bad("nx and ny must be >= 4");
+    if ((long long)cfg->nx * cfg->ny > (1LL << 30)) return bad("grid too large");
+    if (!(cfg->c > 0 && cfg->dx > 0 && cfg->dt > 0)) return bad("c, dx and dt must be positive");
+    if (!(cfg->nu >= 0) || !(cfg->cs_smag >= 0)) return bad("nu and cs_smag must be >= 0");
+    if (!(cfg->rho_ref > 0)) return bad("rho_ref must be positive");
+    if (cfg->force_kind != LBMLETKF_FORCE_NONE && cfg->force_kind != LBMLETKF_FORCE_WATANABE97)
+        return bad("unknown force_kind");
+    if (cfg->ic_kind != LBMLETKF_IC_REFERENCE && cfg->ic_kind != LBMLETKF_IC_K4_SPECTRUM)
+        return bad("unknown ic_kind");
+    if (cfg->obs_stride < 1 || cfg->nx % cfg->obs_stride || cfg->ny % cfg->obs_stride)
+        return bad("obs_stride must divide nx and ny (src/letkf.cu:114)");
+    if (cfg->obs_offset < 0 || cfg->obs_offset >= cfg->obs_stride) return bad("obs_offset must be in [0, obs_stride)");
+    if (cfg->obs_vars <= 0 || cfg->obs_vars > 7) return bad("obs_vars must be a non-empty mask of u|v|rho");
+    if (!(cfg->obs_sigma_u > 0) || !(cfg->obs_sigma_rho > 0)) return bad("observation errors must be positive");
+    if (cfg->loc_kind == LBMLETKF_LOC_GC_WINDOW) {
+        if (cfg->obs_offset != 0) return bad("the GC window localization needs obs_offset == 0");
+        if (cfg->loc_rloc_len < 0) return bad("loc_rloc_len must be >= 0");
+    } else if (cfg->loc_kind == LBMLETKF_LOC_GAUSS) {
+        if (!(cfg->loc_scale > 0) || !(cfg->loc_cutoff >= 0)) return bad("Gaussian localization needs scale > 0, cutoff >= 0");
+        const int R = int(std::floor(cfg->loc_cutoff));
+        if (2 * R + 1 > std::min(cfg->nx, cfg->ny)) return bad("localization cutoff must be below half the domain");
+    } else {
+        return bad("unknown loc_kind");
+    }
+    if (!(cfg->inflation > 0)) return bad("inflation must be positive");
+    if (cfg->da_interval < 1) return bad("da_interval must be >= 1");
+    if (cfg->variant != LBMLETKF_VARIANT_PLAIN && cfg->variant != LBMLETKF_VARIANT_MIXED)
+        return bad("unknown variant");
+    if (!(cfg->jacobi_tol > 0) || cfg->jacobi_max_sweeps < 1) return bad("jacobi_tol > 0 and jacobi_max_sweeps >= 1 required");
+    return LBMLETKF_OK;
+}
+
+LbmParams make_lbm_params(const lbmletkf_config& cfg, int M) {
+    LbmParams p{};
+    p.nx = cfg.nx;
+    p.ny = cfg.ny;
+    p.n_members = M;
+    p.nodes = (long long)cfg.nx * cfg.ny;
+    p.member_stride = Q * p.nodes;
+    p.c = float(cfg.c);
+    p.inv_c = float(1.0 / cfg.c);
+    p.inv_c2 = float(1.0 / (cfg.c * cfg.c));
+    p.k_tau = float(3.0 / (cfg.c * cfg.c * cfg.dt));
+    p.nu = float(cfg.nu);
+    p.les_coef = float(cfg.cs_smag * cfg.cs_smag * cfg.dx * cfg.dx);
+    p.inv_2dx = float(1.0 / (2.0 * cfg.dx));
+    p.force_scale = float(cfg.u_ref * 3.0 / (cfg.c * cfg.c));
+    p.fric_over_dt = float(cfg.friction_rate / cfg.dt);
+    p.dt = float(cfg.dt);
+    p.has_force = cfg.force_kind != LBMLETKF_FORCE_NONE;
+    return p;
+}
+
+int sync_and_check(Ctx* c) {
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    int st = 0;
+    HIP_TRY(c, hipMemcpy(&st, c->d_status, sizeof(int), hipMemcpyDeviceToHost));
+    if (st) {
+        HIP_TRY(c, hipMemset(c->d_status, 0, sizeof(int)));
+        std::string m;
+        if (st & 1) m += "non-finite analysis state (the reference traps the same condition, src/util/letkf_solver.h:352-364); ";
+        if (st & 2) m += "non-finite observation at a used site; ";
+        return fail(c, LBMLETKF_ERR_NUMERIC, m);
+    }
+    return LBMLETKF_OK;
+}
+
+int setup_obs_impl(Ctx* c) {
+    if (c->obs_ready) return LBMLETKF_OK;
+    const auto& cfg = c->cfg;
+    ObsNet& o = c->obs;
+    o.stride = cfg.obs_stride;
+    o.offset = cfg.obs_offset;
+    o.n_sx = c->nx / o.stride;
+    o.n_sy = c->ny / o.stride;
+    o.n_sites = o.n_sx * o.n_sy;
+    o.n_vars = 0;
+    const double isu = 1.0 / (cfg.obs_sigma_u * cfg.obs_sigma_u);
+    const double isr = 1.0 / (cfg.obs_sigma_rho * cfg.obs_sigma_rho);
+    if (cfg.obs_vars & LBMLETKF_OBS_U) { o.var_id[o.n_vars] = 0; o.inv_sigma2[o.n_vars++] = float(isu); }
+    if (cfg.obs_vars & LBMLETKF_OBS_V) { o.var_id[o.n_vars] = 1; o.inv_sigma2[o.n_vars++] = float(isu); }
+    if (cfg.obs_vars & LBMLETKF_OBS_RHO) { o.var_id[o.n_vars] = 2; o.inv_sigma2[o.n_vars++] = float(isr); }
+    const int n_obs = o.n_sites * o.n_vars;
+
+    std::vector<int> site_node(o.n_sites);
+    for (int sy = 0; sy < o.n_sy; ++sy)
+        for (int sx = 0; sx < o.n_sx; ++sx)
+            site_node[sx + sy * o.n_sx] = (sx * o.stride + o.offset) + (sy * o.stride + o.offset) * c->nx;
+    HIP_TRY(c, dalloc(&c->d_site_node, o.n_sites));
+    HIP_TRY(c, hipMemcpy(c->d_site_node, site_node.data(), sizeof(int) * o.n_sites, hipMemcpyHostToDevice));
+
+    // local sets: count on device, scan on host, fill on device
+    int* d_count = nullptr;
+    HIP_TRY(c, dalloc(&d_count, c->nodes));
+    HIP_TRY(c, launch_local_count(*c, d_count));
+    std::vector<int> count(c->nodes);
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipMemcpy(count.data(), d_count, sizeof(int) * c->nodes, hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipFree(d_count));
+    std::vector<int> ptr(c->nodes + 1);
+    long long acc = 0;
+    int mx = 0;
+    for (long long g = 0; g < c->nodes; ++g) {
+        ptr[g] = int(acc);
+        acc += count[g];
+        mx = std::max(mx, count[g]);
+        if (acc > (1LL << 31) - 1) return fail(c, LBMLETKF_ERR_ARG, "local observation sets exceed 2^31 entries");
+    }
+    ptr[c->nodes] = int(acc);
+    o.max_local = mx;
+    o.n_entries = acc;
+    HIP_TRY(c, dalloc(&c->d_loc_ptr, c->nodes + 1));
+    HIP_TRY(c, hipMemcpy(c->d_loc_ptr, ptr.data(), sizeof(int) * (c->nodes + 1), hipMemcpyHostToDevice));
+    HIP_TRY(c, dalloc(&c->d_loc_site, acc));
+    HIP_TRY(c, dalloc(&c->d_loc_w, acc));
+    HIP_TRY(c, launch_local_fill(*c));
+
+    float coef[3] = {o.inv_sigma2[0], o.inv_sigma2[1], o.inv_sigma2[2]};
+    HIP_TRY(c, dalloc(&c->d_obs_coef, 3));
+    HIP_TRY(c, hipMemcpy(c->d_obs_coef, coef, sizeof(coef), hipMemcpyHostToDevice));
+    HIP_TRY(c, dalloc(&c->d_yo, n_obs));
+    HIP_TRY(c, dalloc(&c->d_hx, (size_t)n_obs * c->M));
+    HIP_TRY(c, dalloc(&c->d_dy, (size_t)n_obs * c->M));
+    HIP_TRY(c, dalloc(&c->d_dyo, n_obs));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->obs_ready = true;
+    return LBMLETKF_OK;
+}
+
+int ensure_plain_buffers(Ctx* c) {
+    if (c->d_gram) return LBMLETKF_OK;
+    HIP_TRY(c, dalloc(&c->d_gram, (size_t)c->nodes * c->M * c->M));
+    HIP_TRY(c, dalloc(&c->d_bvec, (size_t)c->nodes * c->M));
+    return LBMLETKF_OK;
+}
+
+int ensure_scratch(Ctx* c) {
+    if (c->d_truth) return LBMLETKF_OK;
+    HIP_TRY(c, dalloc(&c->d_truth, 3 * c->nodes));
+    c->red_cap = ensemble_stats_blocks(*c) * 4;
+    HIP_TRY(c, dalloc(&c->d_red, c->red_cap));
+    return LBMLETKF_OK;
+}
+
+// ---- phase timers: an event pair per phase occurrence, summed lazily -----------
+hipEvent_t prof_event(Ctx* c) {
+    if (!c->event_pool.empty()) {
+        hipEvent_t e = c->event_pool.back();
+        c->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+hipEvent_t prof_begin(Ctx* c) {
+    if (!c->prof) return nullptr;
+    hipEvent_t a = prof_event(c);
+    if (a) (void)hipEventRecord(a, c->stream);
+    return a;
+}
+
+void prof_end(Ctx* c, int phase, hipEvent_t a, long long launches, long long work) {
+    if (!c->prof || !a) return;
+    hipEvent_t b = prof_event(c);
+    if (!b) return;
+    (void)hipEventRecord(b, c->stream);
+    c->prof_pending.push_back({phase, a, b, launches, work});
+}
+
+int prof_collect(Ctx* c) {
+    for (const ProfRec& r : c->prof_pending) {
+        HIP_TRY(c, hipEventSynchronize(r.b));
+        float ms = 0.f;
+        HIP_TRY(c, hipEventElapsedTime(&ms, r.a, r.b));
+        c->prof_sec[r.phase] += 1e-3 * ms;
+        c->prof_launch[r.phase] += r.launches;
+        c->prof_work[r.phase] += r.work;
+        c->event_pool.push_back(r.a);
+        c->event_pool.push_back(r.b);
+    }
+    c->prof_pending.clear();
+    return LBMLETKF_OK;
+}
+
+#define CHECK_CTX(c) \
+    do { if (!(c)) return fail(nullptr, LBMLETKF_ERR_ARG, "context is NULL"); } while (0)
+#define CHECK_MEMBER(c, m) \
+    do { if ((m) < 0 || (m) >= (c)->M) return fail((c), LBMLETKF_ERR_ARG, "member index out of range"); } while (0)
+
+}  // namespace
+
+namespace lbmletkf {
+Ctx::~Ctx() {
+    void* ptrs[] = {f[0], f[1], plain_ruv, plain_nus, force_xy, d_force, d_step, d_status,
+                    d_site_node, d_loc_ptr, d_loc_site, d_loc_w, d_obs_coef, d_yo, d_hx, d_dy,
+                    d_dyo, d_gram, d_bvec, d_dump, d_red, d_truth, d_sweeps};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    for (const ProfRec& r : prof_pending) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    for (hipEvent_t e : event_pool) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+}  // namespace lbmletkf
+
+extern "C" {
+
+const char* lbmletkf_version(void) { return "lbmletkf 0.1 (gfx950)"; }
+
+int lbmletkf_config_preset(const char* name, lbmletkf_config* cfg) {
+    if (!name || !cfg) return fail(nullptr, LBMLETKF_ERR_ARG, "NULL argument");
+    lbmletkf_config c{};
+    const std::string n(name);
+    if (n == "ref") {
+        // src/config.h at SCALER=4, src/lbm.cu:170 (Cs=0.2), src/data.cu:227-350
+        const double u_ref = 1.0, cfl = 0.05, h_ref = 2 * PI;
+        c.nx = c.ny = 64 * 4;
+        c.c = u_ref / cfl;
+        c.dx = h_ref / (c.ny - 1);
+        c.dt = c.dx / c.c;
+        c.nu = 1e-4;
+        c.rho_ref = 1.0;
+        c.cs_smag = 0.2;
+        c.u_ref = u_ref;
+        c.friction_rate = 5e-4;
+        c.force_kind = LBMLETKF_FORCE_WATANABE97;
+        c.force_kf = 4;
+        c.force_amp_total = 5.6 * u_ref / 1.0;
+        c.force_sigma_k = 1;
+        c.force_dk = 2;
+        c.force_rand_stdev = 0.84089641525;
+        c.force_seed = 1991;
+        c.ic_kind = LBMLETKF_IC_REFERENCE;
+        c.ic_amp = 0.01 * u_ref;
+        c.ic_k0 = 4;
+        c.ic_dk = 10;
+        c.ic_sigma = 5;
+        c.obs_stride = 1;
+        c.obs_offset = 0;
+        c.obs_vars = LBMLETKF_OBS_U | LBMLETKF_OBS_V | LBMLETKF_OBS_RHO;
+        c.obs_sigma_u = 0.1;     // run/jserr:146-147
+        c.obs_sigma_rho = 0.01;
+        c.loc_kind = LBMLETKF_LOC_GC_WINDOW;
+        c.loc_rloc_len = 0;
+        c.loc_scale = 0;
+        c.loc_cutoff = 0;
+        c.inflation = 1.0;
+        c.da_interval = 5 * 4 * 10;  // iiter * DAPRUNE
+        c.variant = LBMLETKF_VARIANT_PLAIN;
+        c.jacobi_tol = 1e-10;
+        c.jacobi_max_sweeps = 100;
+    } else if (n == "baseline") {
+        c.nx = c.ny = 256;
+        c.c = 1.0;
+        c.dx = 1.0;
+        c.dt = 1.0;
+        c.nu = (0.503 - 0.5) / 3.0;
+        c.rho_ref = 1.0;
+        c.cs_smag = 0.1;
+        c.u_ref = 1.0;
+        c.friction_rate = 0.0;
+        c.force_kind = LBMLETKF_FORCE_NONE;
+        c.force_kf = 4;
+        c.force_amp_total = 0;
+        c.force_sigma_k = 1;
+        c.force_dk = 2;
+        c.force_rand_stdev = 0.84089641525;
+        c.force_seed = 1991;
+        c.ic_kind = LBMLETKF_IC_K4_SPECTRUM;
+        c.ic_amp = 0.05;
+        c.ic_k0 = 8;
+        c.ic_dk = 0;
+        c.ic_sigma = 0;
+        c.obs_stride = 8;
+        c.obs_offset = 0;
+        c.obs_vars = LBMLETKF_OBS_U | LBMLETKF_OBS_V;
+        c.obs_sigma_u = 0.2 * 0.05;
+        c.obs_sigma_rho = 0.01;
+        c.loc_kind = LBMLETKF_LOC_GAUSS;
+        c.loc_rloc_len = 0;
+        c.loc_scale = 12.0;
+        c.loc_cutoff = 43.8;
+        c.inflation = 1.1;
+        c.da_interval = 100;
+        c.variant = LBMLETKF_VARIANT_MIXED;
+        c.jacobi_tol = 1e-10;
+        c.jacobi_max_sweeps = 100;
+    } else {
+        return fail(nullptr, LBMLETKF_ERR_ARG, "unknown preset: " + n);
+    }
+    *cfg = c;
+    return LBMLETKF_OK;
+}
+
+int lbmletkf_config_check(const lbmletkf_config* cfg, int n_members) {
+    std::string why;
+    const int st = check_config(cfg, n_members, why);
+    if (st) return fail(nullptr, st, why);
+    return LBMLETKF_OK;
+}
+
+const char* lbmletkf_last_error(const lbmletkf_ctx* ctx) {
+    if (ctx) return reinterpret_cast<const Ctx*>(ctx)->last_error.c_str();
+    return g_thread_error.c_str();
+}
+
+int lbmletkf_create(const lbmletkf_config* cfg, int n_members, lbmletkf_ctx** out) {
+    if (!out) return fail(nullptr, LBMLETKF_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    std::string why;
+    if (int st = check_config(cfg, n_members, why)) return fail(nullptr, st, why);
+    Ctx* c = new (std::nothrow) Ctx();
+    if (!c) return fail(nullptr, LBMLETKF_ERR_ARG, "out of host memory");
+    c->cfg = *cfg;
+    c->M = n_members;
+    c->nx = cfg->nx;
+    c->ny = cfg->ny;
+    c->nodes = (long long)cfg->nx * cfg->ny;
+    c->lp = make_lbm_params(*cfg, n_members);
+    build_force_table(*cfg, c->force);
+    auto bail = [&](int st) { g_thread_error = c->last_error; delete c; return st; };
+#define CREATE_TRY(expr)                                                                      \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess) {                                                               \
+            fail(c, LBMLETKF_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));     \
+            return bail(LBMLETKF_ERR_HIP);                                                    \
+        }                                                                                     \
+    } while (0)
+    CREATE_TRY(hipGetDevice(&c->device));
+    CREATE_TRY(hipStreamCreate(&c->stream));
+    const size_t fsz = size_t(c->M) * Q * c->nodes;
+    CREATE_TRY(dalloc(&c->f[0], fsz));
+    CREATE_TRY(dalloc(&c->f[1], fsz));
+    CREATE_TRY(dalloc(&c->plain_ruv, 3 * c->nodes));
+    CREATE_TRY(dalloc(&c->plain_nus, c->nodes));
+    CREATE_TRY(dalloc(&c->force_xy, 2 * c->nodes));
+    CREATE_TRY(hipMemset(c->force_xy, 0, sizeof(float) * 2 * c->nodes));
+    CREATE_TRY(dalloc(&c->d_force, 1));
+    CREATE_TRY(hipMemcpy(c->d_force, &c->force, sizeof(ForceTable), hipMemcpyHostToDevice));
+    CREATE_TRY(dalloc(&c->d_step, 1));
+    CREATE_TRY(hipMemset(c->d_step, 0, sizeof(long long)));
+    CREATE_TRY(dalloc(&c->d_status, 1));
+    CREATE_TRY(hipMemset(c->d_status, 0, sizeof(int)));
+    // every member at rest: f = feq(rho_ref, 0, 0)
+    {
+        float *r = nullptr, *z = nullptr;
+        CREATE_TRY(dalloc(&r, c->nodes));
+        CREATE_TRY(dalloc(&z, c->nodes));
+        std::vector<float> ones(c->nodes, float(cfg->rho_ref));
+        CREATE_TRY(hipMemcpy(r, ones.data(), sizeof(float) * c->nodes, hipMemcpyHostToDevice));
+        CREATE_TRY(hipMemset(z, 0, sizeof(float) * c->nodes));
+        for (int m = 0; m < c->M; ++m) CREATE_TRY(launch_set_feq(*c, m, r, z, z));
+        CREATE_TRY(hipStreamSynchronize(c->stream));
+        CREATE_TRY(hipFree(r));
+        CREATE_TRY(hipFree(z));
+    }
+#undef CREATE_TRY
+    *out = reinterpret_cast<lbmletkf_ctx*>(c);
+    return LBMLETKF_OK;
+}
+
+int lbmletkf_destroy(lbmletkf_ctx* ctx) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c) return LBMLETKF_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    delete c;
+    return LBMLETKF_OK;
+}
+
+int lbmletkf_set_macro(lbmletkf_ctx* ctx, int member, const float* rho, const float* u,
+                       const float* v) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    CHECK_CTX(c);
+    CHECK_MEMBER(c, member);
+    if (!rho || !u || !v) return fail(c, LBMLETKF_ERR_ARG, "rho, u and v are required");
+    if (int st = ensure_scratch(c)) return st;
+    const size_t n = size_t(c->nodes);
+    HIP_TRY(c, hipMemcpyAsync(c->d_truth, rho, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->d_truth + n, u, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->d_truth + 2 * n, v, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, launch_set_feq(*c, member, c->d_truth, c->d_truth + n, c->d_truth + 2 * n));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return LBMLETKF_OK;
+}
+
+int lbmletkf_init_member(lbmletkf_ctx* ctx, int member, unsigned long long seed) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    CHECK_CTX(c);
+    CHECK_MEMBER(c, member);
+    const int nx = c->nx, ny = c->ny;
+    std::vector<Mode> modes;
+    if (c->cfg.ic_kind == LBMLETKF_IC_REFERENCE) modes_reference(c->cfg, seed, nx, modes);
+    else modes_k4(c->cfg, seed, nx, ny, modes);
+    std::vector<double> u, v;
+    velocity_from_modes(modes, nx, ny, u, v);
+    const size_t n = size_t(nx) * ny;
+    double scale = 0.0;
+    if (c->cfg.ic_kind == LBMLETKF_IC_REFERENCE) {
+        // src/data.cu:178-186: scale to max speed p_amp*u_ref
+        double vmax = 1e-30 * c->cfg.u_ref;
+        for (size_t a = 0; a < n; ++a) vmax = std::max(vmax, std::sqrt(u[a] * u[a] + v[a] * v[a]));
+        scale = c->cfg.ic_amp / vmax;
+    } else {
+        double e = 0.0;
+        for (size_t a = 0; a < n; ++a) e += u[a] * u[a] + v[a] * v[a];
+        const double rms = std::sqrt(e / double(n));
+        scale = rms > 0 ? c->cfg.ic_amp / rms : 0.0;
+    }
+    std::vector<float> rr(n, float(c->cfg.rho_ref)), uf(n), vf(n);
+    for (size_t a = 0; a < n; ++a) {
+        uf[a] = float(u[a] * scale);
+        vf[a] = float(v[a] * scale);
+    }
+    return lbmletkf_set_macro(ctx, member, rr.data(), uf.data(), vf.data());
+}
+
+int lbmletkf_set_state(lbmletkf_ctx* ctx, int member, const float* f_host) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    CHECK_CTX(c);
+    CHECK_MEMBER(c, member);
+    if (!f_host) return fail(c, LBMLETKF_ERR_ARG, "f is NULL");
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipMemcpy(c->f[c->cur] + (size_t)member * c->lp.member_stride, f_host,
+                         sizeof(float) * c->lp.member_stride, hipMemcpyHostToDevice));
+    return LBMLETKF_OK;
+}
+
+int lbmletkf_get_state(lbmletkf_ctx* ctx, int member, float* f_host) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    CHECK_CTX(c);
+    CHECK_MEMBER(c, member);
+    if (!f_host) return fail(c, LBMLETKF_ERR_ARG, "f is NULL");
+    if (int st = sync_and_check(c)) return st;
+    HIP_TRY(c, hipMemcpy(f_host, c->f[c->cur] + (size_t)member * c->lp.member_stride,
+                         sizeof(float) * c->lp.member_stride, hipMemcpyDeviceToHost));
+    return LBMLETKF_OK;
+}
+
+int lbmletkf_get_macro_device(lbmletkf_ctx* ctx, int member, float* rho, float* u, float* v) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    CHECK_CTX(c);
+    CHECK_MEMBER(c, member);
+    HIP_TRY(c, launch_macro(*c, c->f[c->cur] + (size_t)member * c->lp.member_stride, rho, u, v));
+    return LBMLETKF_OK;
+}
+
+int lbmletkf_get_macro(lbmletkf_ctx* ctx, int member, float* rho, float* u, float* v) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    CHECK_CTX(c);
+    CHECK_MEMBER(c, member);
+    if (int st = ensure_scratch(c)) return st;
+    const size_t n = size_t(c->nodes);
+    HIP_TRY(c, launch_macro(*c, c->f[c->cur] + (size_t)member * c->lp.member_stride, c->d_truth,
+                            c->d_truth + n, c->d_truth + 2 * n));
+    if (int st = sync_and_check(c)) return st;
+    if (rho) HIP_TRY(c, hipMemcpy(rho, c->d_truth, sizeof(float) * n, hipMemcpyDeviceToHost));
+    if (u) HIP_TRY(c, hipMemcpy(u, c->d_truth + n, sizeof(float) * n, hipMemcpyDeviceToHost));
+    if (v) HIP_TRY(c, hipMemcpy(v, c->d_truth + 2 * n, sizeof(float) * n, hipMemcpyDeviceToHost));
+    return LBMLETKF_OK;
+}
+
+int lbmletkf_forecast(lbmletkf_ctx* ctx, int n_steps) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    CHECK_CTX(c);
+    if (n_steps < 0) return fail(c, LBMLETKF_ERR_ARG, "n_steps must be >= 0");
+    const bool fused = c->cfg.variant == LBMLETKF_VARIANT_MIXED && lbm_fused_supported(c->nx, c->ny);
+    hipEvent_t ev = prof_begin(c);
+    for (int s = 0; s < n_steps; ++s) {
+        if (c->lp.has_force) HIP_TRY(c, launch_force_field(*c, s));
+        const float* fin = c->f[c->cur];
+        float* fout = c->f[1 - c->cur];
+        if (fused) {
+            HIP_TRY(c, launch_lbm_fused(*c, fin, fout));
+        } else {
+            for (int m = 0; m < c->M; ++m) HIP_TRY(c, launch_lbm_plain_member(*c, fin, fout, m));
+        }
+        c->cur = 1 - c->cur;
+    }
+    if (n_steps > 0) HIP_TRY(c, launch_step_advance(*c, n_steps));
+    const long long per_step = (fused ? 1 : 3LL * c->M) + (c->lp.has_force ? 1 : 0);
+    prof_end(c, PH_FORECAST, ev, per_step * n_steps + (n_steps > 0), (long long)n_steps * c->M * c->nodes);
+    c->step += n_steps;
+    return LBMLETKF_OK;
+}
+
+int lbmletkf_set_step(lbmletkf_ctx* ctx, long long step) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    CHECK_CTX(c);
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipMemcpy(c->d_step, &step, sizeof(long long), hipMemcpyHostToDevice));
+    c->step = step;
+    return LBMLETKF_OK;
+}
+
+int lbmletkf_get_step(lbmletkf_ctx* ctx, long long* step) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    CHECK_CTX(c);
+    if (!step) return fail(c, LBMLETKF_ERR_ARG, "step is NULL");
+    *step = c->step;
+    return LBMLETKF_OK;
+}
+
+int lbmletkf_setup_obs(lbmletkf_ctx* ctx) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    CHECK_CTX(c);
+    return setup_obs_impl(c);
+}
+
+int lbmletkf_obs_count(lbmletkf_ctx* ctx, int* n_sites, int* n_obs) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    CHECK_CTX(c);
+    if (int st = setup_obs_impl(c)) return st;
+    if (n_sites) *n_sites = c->obs.n_sites;
+    if (n_obs) *n_obs = c->obs.n_sites * c->obs.n_vars;
+    return LBMLETKF_OK;
+}
+
+int lbmletkf_observe(lbmletkf_ctx* ctx, int member, unsigned long long seed,
+                     unsigned long long counter, int noise, float* yo, int yo_on_device) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    CHECK_CTX(c);
+    CHECK_MEMBER(c, member);
+    if (!yo) return fail(c, LBMLETKF_ERR_ARG, "yo is NULL");
+    if (int st = setup_obs_impl(c)) return st;
+    const int n_obs = c->obs.n_sites * c->obs.n_vars;
+    // H(x) of the member into d_dyo (free outside an analysis), then noise into the target
+    HIP_TRY(c, launch_obs_moments(*c, c->f[c->cur], c->d_dyo, member, 1));
+    if (yo_on_device) {
+        HIP_TRY(c, launch_observe(*c, c->d_dyo, yo, seed, counter, noise));
+        return LBMLETKF_OK;
+    }
+    HIP_TRY(c, launch_observe(*c, c->d_dyo, c->d_yo, seed, counter, noise));
+    if (int st = sync_and_check(c)) return st;
+    HIP_TRY(c, hipMemcpy(yo, c->d_yo, sizeof(float) * n_obs, hipMemcpyDeviceToHost));
+    return LBMLETKF_OK;
+}
+
+int lbmletkf_obs_from_fields(lbmletkf_ctx* ctx, const float* rho, const float* u, const float* v,
+                             float* yo_host) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    CHECK_CTX(c);
+    if (!yo_host) return fail(c, LBMLETKF_ERR_ARG, "yo is NULL");
+    if (int st = setup_obs_impl(c)) return st;
+    const float* fields[3] = {u, v, rho};
+    for (int a = 0; a < c->obs.n_vars; ++a)
+        if (!fields[c->obs.var_id[a]]) return fail(c, LBMLETKF_ERR_ARG, "a field of an observed variable is NULL");
+    const ObsNet& o = c->obs;
+    for (int s = 0; s < o.n_sites; ++s) {
+        const int sx = s % o.n_sx, sy = s / o.n_sx;
+        const long long node = (sx * o.stride + o.offset) + (long long)(sy * o.stride + o.offset) * c->nx;
+        for (int a = 0; a < o.n_vars; ++a) yo_host[s * o.n_vars + a] = fields[o.var_id[a]][node];
+    }
+    return LBMLETKF_OK;
+}
+
+int lbmletkf_analysis(lbmletkf_ctx* ctx, const float* yo, int yo_on_device) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    CHECK_CTX(c);
+    if (!yo) return fail(c, LBMLETKF_ERR_ARG, "yo is NULL");
+    if (c->M < 2) return fail(c, LBMLETKF_ERR_ARG, "the LETKF needs at least two members ((M-1) I is singular for M=1)");
+    if (int st = setup_obs_impl(c)) return st;
+    if (int st = ensure_plain_buffers(c)) return st;
+    const int n_obs = c->obs.n_sites * c->obs.n_vars;
+    HIP_TRY(c, hipMemcpyAsync(c->d_yo, yo, sizeof(float) * n_obs,
+                              yo_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                              c->stream));
+    hipEvent_t e0 = prof_begin(c);
+    HIP_TRY(c, launch_obs_moments(*c, c->f[c->cur], c->d_hx, 0, c->M));
+    HIP_TRY(c, launch_obs_prepare(*c));
+    prof_end(c, PH_OBS, e0, 2, n_obs);
+    hipEvent_t e1 = prof_begin(c);
+    HIP_TRY(c, launch_gram_fp32(*c));
+    prof_end(c, PH_GRAM, e1, 1, c->obs.n_entries * c->obs.n_vars);
+    hipEvent_t e2 = prof_begin(c);
+    HIP_TRY(c, launch_solve_fp64(*c, c->f[c->cur], c->prof ? c->d_sweeps : nullptr));
+    prof_end(c, PH_SOLVE, e2, 1, 0);
+    return LBMLETKF_OK;
+}
+
+int lbmletkf_cycle(lbmletkf_ctx* ctx, const float* yo_device) {
+    if (int st = lbmletkf_analysis(ctx, yo_device, 1)) return st;
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    return lbmletkf_forecast(ctx, c->cfg.da_interval);
+}
+
+int lbmletkf_ensemble_stats(lbmletkf_ctx* ctx, const float* truth_rho, const float* truth_u,
+                            const float* truth_v, int truth_on_device, double out[4]) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    CHECK_CTX(c);
+    if (!truth_rho || !truth_u || !truth_v || !out) return fail(c, LBMLETKF_ERR_ARG, "NULL argument");
+    if (int st = ensure_scratch(c)) return st;
+    const size_t n = size_t(c->nodes);
+    const float* tr = truth_rho;
+    const float* tu = truth_u;
+    const float* tv = truth_v;
+    if (!truth_on_device) {
+        HIP_TRY(c, hipMemcpyAsync(c->d_truth, truth_rho, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(c->d_truth + n, truth_u, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(c->d_truth + 2 * n, truth_v, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+        tr = c->d_truth;
+        tu = c->d_truth + n;
+        tv = c->d_truth + 2 * n;
+    }
+    const int nb = ensemble_stats_blocks(*c);
+    HIP_TRY(c, launch_ensemble_stats(*c, tr, tu, tv, c->d_red, nb));
+    if (int st = sync_and_check(c)) return st;
+    std::vector<double> part(size_t(nb) * 4);
+    HIP_TRY(c, hipMemcpy(part.data(), c->d_red, sizeof(double) * part.size(), hipMemcpyDeviceToHost));
+    long double s[4] = {0, 0, 0, 0};
+    for (int b = 0; b < nb; ++b)
+        for (int a = 0; a < 4; ++a) s[a] += part[size_t(b) * 4 + a];
+    out[0] = double(std::sqrt(s[0] / (2.0L * n)));
+    out[1] = double(std::sqrt(s[1] / (2.0L * n)));
+    out[2] = double(std::sqrt(s[2] / (long double)n));
+    out[3] = double(std::sqrt(s[3] / (long double)n));
+    return LBMLETKF_OK;
+}
+
+int lbmletkf_local_obs(lbmletkf_ctx* ctx, int gridpoint, int* n, int* sites, float* weights, int cap) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    CHECK_CTX(c);
+    if (!n) return fail(c, LBMLETKF_ERR_ARG, "n is NULL");
+    if (gridpoint < 0 || gridpoint >= c->nodes) return fail(c, LBMLETKF_ERR_ARG, "gridpoint out of range");
+    if (int st = setup_obs_impl(c)) return st;
+    int pr[2];
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipMemcpy(pr, c->d_loc_ptr + gridpoint, sizeof(int) * 2, hipMemcpyDeviceToHost));
+    *n = pr[1] - pr[0];
+    const int k = std::min(*n, std::max(cap, 0));
+    if (k > 0 && sites) HIP_TRY(c, hipMemcpy(sites, c->d_loc_site + pr[0], sizeof(int) * k, hipMemcpyDeviceToHost));
+    if (k > 0 && weights) HIP_TRY(c, hipMemcpy(weights, c->d_loc_w + pr[0], sizeof(float) * k, hipMemcpyDeviceToHost));
+    return LBMLETKF_OK;
+}
+
+int lbmletkf_set_dump(lbmletkf_ctx* ctx, int g0, int count) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    CHECK_CTX(c);
+    if (count < 0 || g0 < 0 || (long long)g0 + count > c->nodes) return fail(c, LBMLETKF_ERR_ARG, "dump range out of bounds");
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (c->d_dump) {
+        HIP_TRY(c, hipFree(c->d_dump));
+        c->d_dump = nullptr;
+    }
+    c->dump_g0 = g0;
+    c->dump_n = count;
+    if (count > 0) {
+        const size_t per = size_t(c->M) * c->M + 2 * size_t(c->M);
+        HIP_TRY(c, dalloc(&c->d_dump, per * count));
+        HIP_TRY(c, hipMemset(c->d_dump, 0, sizeof(double) * per * count));
+    }
+    return LBMLETKF_OK;
+}
+
+int lbmletkf_get_dump(lbmletkf_ctx* ctx, double* W, double* w, double* evals) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    CHECK_CTX(c);
+    if (c->dump_n == 0) return fail(c, LBMLETKF_ERR_STATE, "no dump range set");
+    if (int st = sync_and_check(c)) return st;
+    const int M = c->M;
+    const size_t per = size_t(M) * M + 2 * size_t(M);
+    std::vector<double> h(per * c->dump_n);
+    HIP_TRY(c, hipMemcpy(h.data(), c->d_dump, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+    for (int g = 0; g < c->dump_n; ++g) {
+        const double* src = h.data() + per * g;
+        if (W) std::copy(src, src + size_t(M) * M, W + size_t(M) * M * g);
+        if (w) std::copy(src + size_t(M) * M, src + size_t(M) * M + M, w + size_t(M) * g);
+        if (evals) {
+            std::vector<double> d(src + size_t(M) * M + M, src + size_t(M) * M + 2 * M);
+            std::sort(d.begin(), d.end());
+            std::copy(d.begin(), d.end(), evals + size_t(M) * g);
+        }
+    }
+    return LBMLETKF_OK;
+}
+
+int lbmletkf_profile_enable(lbmletkf_ctx* ctx, int on) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    CHECK_CTX(c);
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (int st = prof_collect(c)) return st;
+    for (int p = 0; p < PH_COUNT; ++p) {
+        c->prof_sec[p] = 0;
+        c->prof_launch[p] = 0;
+        c->prof_work[p] = 0;
+    }
+    if (!c->d_sweeps) HIP_TRY(c, dalloc(&c->d_sweeps, 1));
+    HIP_TRY(c, hipMemset(c->d_sweeps, 0, sizeof(unsigned long long)));
+    c->prof = on != 0;
+    return LBMLETKF_OK;
+}
+
+int lbmletkf_profile_read(lbmletkf_ctx* ctx, const char* phase, double* seconds,
+                          long long* launches, long long* work) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    CHECK_CTX(c);
+    if (!phase) return fail(c, LBMLETKF_ERR_ARG, "phase is NULL");
+    static const char* names[PH_COUNT] = {"forecast", "obs", "gram", "solve"};
+    int ph = -1;
+    for (int p = 0; p < PH_COUNT; ++p)
+        if (std::strcmp(phase, names[p]) == 0) ph = p;
+    if (ph < 0) return fail(c, LBMLETKF_ERR_ARG, std::string("unknown phase: ") + phase);
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (int st = prof_collect(c)) return st;
+    long long wk = c->prof_work[ph];
+    if (ph == PH_SOLVE && c->d_sweeps) {
+        unsigned long long sw = 0;
+        HIP_TRY(c, hipMemcpy(&sw, c->d_sweeps, sizeof(sw), hipMemcpyDeviceToHost));
+        wk = (long long)sw;
+    }
+    if (seconds) *seconds = c->prof_sec[ph];
+    if (launches) *launches = c->prof_launch[ph];
+    if (work) *work = wk;
+    return LBMLETKF_OK;
+}
+
+void* lbmletkf_get_stream(lbmletkf_ctx* ctx) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    return c ? reinterpret_cast<void*>(c->stream) : nullptr;
+}
+
+int lbmletkf_synchronize(lbmletkf_ctx* ctx) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    CHECK_CTX(c);
+    return sync_and_check(c);
+}
+
+}  // extern "C"

@@ -1,0 +1,147 @@
+// Internal state of a context and the launchers shared between translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/lbmletkf.h"
+
+namespace lbmletkf {
+
+constexpr int Q = 9;
+constexpr int MAX_MEMBERS = 64;      // one wavefront of lanes per ensemble matrix
+constexpr int MAX_FORCE_MODES = 100; // src/data.cu:258 prunes to 100 modes
+
+// Per-step LBM constants, in the arithmetic type of the kernels (FP32).
+struct LbmParams {
+    int nx, ny, n_members;
+    long long nodes;          // nx*ny
+    long long member_stride;  // Q*nx*ny floats
+    float c, inv_c, inv_c2;   // c, 1/c, 1/c^2
+    float k_tau;              // tau = 0.5 + k_tau*(nu + nu_t), k_tau = 3/(c^2 dt)   (lbm.cu:144-148)
+    float nu;
+    float les_coef;           // Cs^2 dx^2                                           (lbm.cu:170-171)
+    float inv_2dx;            // central difference 1/(2 dx)                         (lbm.cu:155-158)
+    float force_scale;        // u_ref * 3/c^2: e_q*u_ref/cs2 of force_acc            (lbm.h:47-54)
+    float fric_over_dt;       // friction_rate/dt                                   (lbm.h:52-53)
+    float dt;
+    int has_force;
+};
+
+// Forcing modes of Watanabe (1997) as built by init_forces (src/data.cu:227-290).
+struct ForceTable {
+    int n = 0;
+    float kx[MAX_FORCE_MODES], ky[MAX_FORCE_MODES], amp[MAX_FORCE_MODES];
+};
+
+// Observed lattice and per-variable error weights.
+struct ObsNet {
+    int stride = 0, offset = 0;
+    int n_sx = 0, n_sy = 0, n_sites = 0;
+    int n_vars = 0;
+    int var_id[3] = {0, 0, 0};          // 0=u 1=v 2=rho, in observation order
+    float inv_sigma2[3] = {0, 0, 0};    // R^-1 diagonal per variable (letkf.cu:174-178)
+    int max_local = 0;                  // largest local site count over gridpoints
+    long long n_entries = 0;            // total local (site, weight) pairs
+};
+
+enum Phase { PH_FORECAST = 0, PH_OBS, PH_GRAM, PH_SOLVE, PH_COUNT };
+
+struct ProfRec {
+    int phase;
+    hipEvent_t a, b;
+    long long launches, work;
+};
+
+struct Ctx {
+    lbmletkf_config cfg{};
+    int M = 0;
+    int nx = 0, ny = 0;
+    long long nodes = 0;
+    LbmParams lp{};
+    ForceTable force{};
+    hipStream_t stream = nullptr;
+    int device = 0;
+    std::string last_error;
+
+    // state, double-buffered SoA f[member][q][j][i]
+    float* f[2] = {nullptr, nullptr};
+    int cur = 0;
+    // plain-variant intermediates, one member at a time: rho,u,v after streaming, nu_t
+    float* plain_ruv = nullptr;   // [3][nodes]
+    float* plain_nus = nullptr;   // [nodes]
+    // forcing field of the current step, shared by all members: [2][nodes]
+    float* force_xy = nullptr;
+    ForceTable* d_force = nullptr;
+    long long* d_step = nullptr;  // device copy of the step counter (graph-replay safe)
+    long long step = 0;
+
+    // device status word: bit 0 = NaN in analysis
+    int* d_status = nullptr;
+    int* h_status = nullptr;      // pinned
+
+    // observation network and local sets (CSR over gridpoints)
+    bool obs_ready = false;
+    ObsNet obs{};
+    int* d_site_node = nullptr;   // [n_sites] node index of each site
+    int* d_loc_ptr = nullptr;     // [nodes+1]
+    int* d_loc_site = nullptr;    // [n_entries]
+    float* d_loc_w = nullptr;     // [n_entries] localization weight (without R^-1)
+    float* d_obs_coef = nullptr;  // [3] R^-1 per variable, device copy
+
+    // analysis buffers
+    float* d_yo = nullptr;        // [n_obs]
+    float* d_hx = nullptr;        // [n_obs][M] H(x) of every member
+    float* d_dy = nullptr;        // [n_obs][M] perturbations dY = Y - ybar
+    float* d_dyo = nullptr;       // [n_obs] innovation yo - ybar
+    float* d_gram = nullptr;      // plain: [nodes][M][M] C matrices
+    float* d_bvec = nullptr;      // plain: [nodes][M] dY^T R^-1 dyo
+
+    // transform dump
+    int dump_g0 = 0, dump_n = 0;
+    double* d_dump = nullptr;     // [dump_n][M*M + 2M]
+
+    // scratch for reductions
+    double* d_red = nullptr;      // [red_cap]
+    int red_cap = 0;
+
+    // macro scratch for stats
+    float* d_truth = nullptr;     // [3][nodes]
+
+    // phase timers
+    bool prof = false;
+    std::vector<ProfRec> prof_pending;
+    std::vector<hipEvent_t> event_pool;
+    double prof_sec[PH_COUNT] = {};
+    long long prof_launch[PH_COUNT] = {};
+    long long prof_work[PH_COUNT] = {};
+    unsigned long long* d_sweeps = nullptr;  // Jacobi sweeps summed over gridpoints
+
+    ~Ctx();
+};
+
+// ---- launchers (lbm.hip) ------------------------------------------------------
+hipError_t launch_force_field(Ctx& c, int step_offset);
+hipError_t launch_lbm_fused(Ctx& c, const float* fin, float* fout);
+hipError_t launch_lbm_plain_member(Ctx& c, const float* fin, float* fout, int member);
+hipError_t launch_set_feq(Ctx& c, int member, const float* d_rho, const float* d_u, const float* d_v);
+hipError_t launch_macro(Ctx& c, const float* f_member, float* rho, float* u, float* v);
+hipError_t launch_ensemble_stats(Ctx& c, const float* truth_rho, const float* truth_u,
+                                 const float* truth_v, double* partials, int n_blocks);
+int ensemble_stats_blocks(const Ctx& c);
+hipError_t launch_step_advance(Ctx& c, int n);
+bool lbm_fused_supported(int nx, int ny);
+
+// ---- launchers (letkf.hip) ----------------------------------------------------
+hipError_t launch_local_count(Ctx& c, int* d_count);
+hipError_t launch_local_fill(Ctx& c);
+hipError_t launch_obs_moments(Ctx& c, const float* f, float* hx, int member0, int n_members);
+hipError_t launch_obs_prepare(Ctx& c);
+hipError_t launch_observe(Ctx& c, const float* hx_member, float* yo, uint64_t seed,
+                          uint64_t counter, int noise);
+hipError_t launch_gram_fp32(Ctx& c);
+hipError_t launch_solve_fp64(Ctx& c, float* f, unsigned long long* sweeps);
+
+}  // namespace lbmletkf

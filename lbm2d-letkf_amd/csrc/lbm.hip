@@ -1,0 +1,410 @@
+// D2Q9 BGK + Smagorinsky forecast kernels for gfx950.
+//
+// Physics follows the reference step update() (src/lbm.h:16-29):
+//   forcing (src/data.cu:305-350) -> pull streaming + moments (src/lbm.cu:67-103)
+//   -> Smagorinsky nu_t from central differences of the streamed velocity
+//   (src/lbm.cu:24-65) -> BGK relaxation with tau = 0.5 + 3(nu+nu_t)/(c^2 dt)
+//   and the first-order force term (src/lbm.cu:105-160, src/lbm.h:31-70).
+//
+// Two forms are built:
+//   * lbm_fused: one launch advances every member by one step.  A 64x16 tile of
+//     one member is streamed into registers, its streamed velocity (plus a one-node
+//     halo) goes to LDS for the strain stencil, and the collided populations are
+//     written once: 36 B read + 36 B written per node update.
+//   * the plain form keeps the reference's three passes (stream+macro, les,
+//     collide) and is launched member by member, as one reference rank would.
+#include "context.hpp"
+#include "philox.hpp"
+
+namespace lbmletkf {
+
+namespace {
+
+constexpr float W0 = 4.f / 9.f, W1 = 1.f / 9.f, W2 = 1.f / 36.f;
+
+__device__ __forceinline__ int wrap(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
+
+// Pull streaming of the 9 populations into node (i,j): fc[q] = f[q](i-ex_q, j-ey_q)
+// with ex_q = q%3-1, ey_q = q/3-1 (src/config.h:256-258, src/lbm.cu:193-198).
+// im/ip are the wrapped columns i-1/i+1, rm/r0/rp the row offsets of j-1/j/j+1.
+__device__ __forceinline__ void pull9(const float* __restrict__ f, long long nodes, int im, int i,
+                                      int ip, long long rm, long long r0, long long rp,
+                                      float fc[Q]) {
+    fc[0] = f[0 * nodes + rp + ip];
+    fc[1] = f[1 * nodes + rp + i];
+    fc[2] = f[2 * nodes + rp + im];
+    fc[3] = f[3 * nodes + r0 + ip];
+    fc[4] = f[4 * nodes + r0 + i];
+    fc[5] = f[5 * nodes + r0 + im];
+    fc[6] = f[6 * nodes + rm + ip];
+    fc[7] = f[7 * nodes + rm + i];
+    fc[8] = f[8 * nodes + rm + im];
+}
+
+// rho, u = c sum f ex / rho, v = c sum f ey / rho   (src/lbm.h:72-109)
+__device__ __forceinline__ void moments(const float fc[Q], float c, float& rho, float& u, float& v) {
+    float r = 0.f;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) r += fc[q];
+    const float jx = (fc[2] + fc[5] + fc[8]) - (fc[0] + fc[3] + fc[6]);
+    const float jy = (fc[6] + fc[7] + fc[8]) - (fc[0] + fc[1] + fc[2]);
+    rho = r;
+    u = c * jx / r;
+    v = c * jy / r;
+}
+
+// nu_t = Cs^2 dx^2 sqrt(2 S:S) from central differences (src/lbm.cu:147-171).
+__device__ __forceinline__ float smagorinsky(float uE, float uW, float uN, float uS, float vE,
+                                             float vW, float vN, float vS, const LbmParams& p) {
+    const float ux = (uE - uW) * p.inv_2dx;
+    const float uy = (uN - uS) * p.inv_2dx;
+    const float vx = (vE - vW) * p.inv_2dx;
+    const float vy = (vN - vS) * p.inv_2dx;
+    const float s01 = 0.5f * (uy + vx);
+    const float ss = sqrtf(2.f * (ux * ux + 2.f * s01 * s01 + vy * vy));
+    return p.les_coef * ss;
+}
+
+// BGK relaxation towards feq plus the first-order force term with linear
+// friction (src/lbm_collision_srt.h:8-16, src/lbm.h:31-70).
+__device__ __forceinline__ void collide9(float fc[Q], float rho, float u, float v, float nut,
+                                         float Fx, float Fy, const LbmParams& p) {
+    const float tau = 0.5f + p.k_tau * (p.nu + nut);
+    const float omega = 1.f / tau;
+    const float uu = (u * u + v * v) * p.inv_c2;
+    const float fxp = (Fx - p.fric_over_dt * u) * p.dt;
+    const float fyp = (Fy - p.fric_over_dt * v) * p.dt;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int ex = q % 3 - 1, ey = q / 3 - 1;
+        const float wq = (ex == 0 && ey == 0) ? W0 : ((ex == 0 || ey == 0) ? W1 : W2);
+        const float uc = (u * float(ex) + v * float(ey)) * p.inv_c;
+        const float feq = rho * wq * (1.f + 3.f * uc + 4.5f * uc * uc - 1.5f * uu);
+        const float fq = fc[q] - omega * (fc[q] - feq);
+        fc[q] = fq + wq * rho * p.force_scale * (float(ex) * fxp + float(ey) * fyp);
+    }
+}
+
+// ----------------------------------------------------------------------------
+// fused, batched step
+// ----------------------------------------------------------------------------
+constexpr int TX = 64, TY = 16, NT = 256;
+constexpr int ROWS_PER_PASS = NT / TX;        // 4
+constexpr int NODES_PER_THREAD = TY / ROWS_PER_PASS;  // 4
+constexpr int HALO = 2 * (TX + 2) + 2 * TY;   // 164
+
+__global__ __launch_bounds__(NT) void k_lbm_fused(const float* __restrict__ fin,
+                                                  float* __restrict__ fout,
+                                                  const float* __restrict__ force, LbmParams p,
+                                                  int tiles_x, int tiles_per_member) {
+    // XCD-aware tile order: workgroups are dealt round-robin to the 8 XCDs, so
+    // give each XCD a contiguous run of tiles (neighbouring tiles share halo rows
+    // in that XCD's L2).
+    const int nb = gridDim.x;
+    const int b = blockIdx.x;
+    const int t = (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
+    const int member = t / tiles_per_member;
+    const int tt = t - member * tiles_per_member;
+    const int x0 = (tt % tiles_x) * TX;
+    const int y0 = (tt / tiles_x) * TY;
+    const int nx = p.nx, ny = p.ny;
+    const long long nodes = p.nodes;
+    const float* f = fin + (long long)member * p.member_stride;
+    float* fo = fout + (long long)member * p.member_stride;
+
+    __shared__ float su[TY + 2][TX + 2];
+    __shared__ float sv[TY + 2][TX + 2];
+
+    const int lx = threadIdx.x % TX;
+    const int ly0 = threadIdx.x / TX;
+    const int i = x0 + lx;
+    const int im = wrap(i - 1, nx), ip = wrap(i + 1, nx);
+
+    float fc[NODES_PER_THREAD][Q];
+    float rr[NODES_PER_THREAD], uu[NODES_PER_THREAD], vv[NODES_PER_THREAD];
+#pragma unroll
+    for (int r = 0; r < NODES_PER_THREAD; ++r) {
+        const int ly = ly0 + r * ROWS_PER_PASS;
+        const int j = y0 + ly;
+        const long long r0 = (long long)j * nx;
+        const long long rm = (long long)wrap(j - 1, ny) * nx;
+        const long long rp = (long long)wrap(j + 1, ny) * nx;
+        pull9(f, nodes, im, i, ip, rm, r0, rp, fc[r]);
+        moments(fc[r], p.c, rr[r], uu[r], vv[r]);
+        su[ly + 1][lx + 1] = uu[r];
+        sv[ly + 1][lx + 1] = vv[r];
+    }
+    // halo ring of streamed velocities
+    if (threadIdx.x < HALO) {
+        int h = threadIdx.x, hx, hy;
+        if (h < TX + 2) { hx = h - 1; hy = -1; }
+        else if (h < 2 * (TX + 2)) { hx = h - (TX + 2) - 1; hy = TY; }
+        else if (h < 2 * (TX + 2) + TY) { hx = -1; hy = h - 2 * (TX + 2); }
+        else { hx = TX; hy = h - 2 * (TX + 2) - TY; }
+        const int gi = wrap(x0 + hx, nx), gj = wrap(y0 + hy, ny);
+        const int gim = wrap(gi - 1, nx), gip = wrap(gi + 1, nx);
+        const long long r0 = (long long)gj * nx;
+        const long long rm = (long long)wrap(gj - 1, ny) * nx;
+        const long long rp = (long long)wrap(gj + 1, ny) * nx;
+        float fh[Q], rh, uh, vh;
+        pull9(f, nodes, gim, gi, gip, rm, r0, rp, fh);
+        moments(fh, p.c, rh, uh, vh);
+        su[hy + 1][hx + 1] = uh;
+        sv[hy + 1][hx + 1] = vh;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < NODES_PER_THREAD; ++r) {
+        const int ly = ly0 + r * ROWS_PER_PASS;
+        const int j = y0 + ly;
+        const float nut = smagorinsky(su[ly + 1][lx + 2], su[ly + 1][lx], su[ly + 2][lx + 1],
+                                      su[ly][lx + 1], sv[ly + 1][lx + 2], sv[ly + 1][lx],
+                                      sv[ly + 2][lx + 1], sv[ly][lx + 1], p);
+        const long long node = (long long)j * nx + i;
+        float Fx = 0.f, Fy = 0.f;
+        if (p.has_force) {
+            Fx = force[node];
+            Fy = force[nodes + node];
+        }
+        collide9(fc[r], rr[r], uu[r], vv[r], nut, Fx, Fy, p);
+#pragma unroll
+        for (int q = 0; q < Q; ++q) fo[q * nodes + node] = fc[r][q];
+    }
+}
+
+// ----------------------------------------------------------------------------
+// plain form: the reference's three passes for one member
+// ----------------------------------------------------------------------------
+__global__ void k_stream_macro(const float* __restrict__ f, float* __restrict__ fn,
+                               float* __restrict__ ruv, LbmParams p) {
+    const long long node = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= p.nodes) return;
+    const int nx = p.nx, ny = p.ny;
+    const int i = int(node % nx), j = int(node / nx);
+    float fc[Q];
+    pull9(f, p.nodes, wrap(i - 1, nx), i, wrap(i + 1, nx), (long long)wrap(j - 1, ny) * nx,
+          (long long)j * nx, (long long)wrap(j + 1, ny) * nx, fc);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) fn[q * p.nodes + node] = fc[q];
+    float r, u, v;
+    moments(fc, p.c, r, u, v);
+    ruv[node] = r;
+    ruv[p.nodes + node] = u;
+    ruv[2 * p.nodes + node] = v;
+}
+
+__global__ void k_les(const float* __restrict__ ruv, float* __restrict__ nus, LbmParams p) {
+    const long long node = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= p.nodes) return;
+    const int nx = p.nx, ny = p.ny;
+    const int i = int(node % nx), j = int(node / nx);
+    const float* u = ruv + p.nodes;
+    const float* v = ruv + 2 * p.nodes;
+    const long long e = (long long)j * nx + wrap(i + 1, nx), w = (long long)j * nx + wrap(i - 1, nx);
+    const long long n = (long long)wrap(j + 1, ny) * nx + i, s = (long long)wrap(j - 1, ny) * nx + i;
+    nus[node] = smagorinsky(u[e], u[w], u[n], u[s], v[e], v[w], v[n], v[s], p);
+}
+
+__global__ void k_collide(float* __restrict__ fn, const float* __restrict__ ruv,
+                          const float* __restrict__ nus, const float* __restrict__ force,
+                          LbmParams p) {
+    const long long node = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= p.nodes) return;
+    float fc[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) fc[q] = fn[q * p.nodes + node];
+    float Fx = 0.f, Fy = 0.f;
+    if (p.has_force) {
+        Fx = force[node];
+        Fy = force[p.nodes + node];
+    }
+    collide9(fc, ruv[node], ruv[p.nodes + node], ruv[2 * p.nodes + node], nus[node], Fx, Fy, p);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) fn[q * p.nodes + node] = fc[q];
+}
+
+// ----------------------------------------------------------------------------
+// stochastic forcing of Watanabe (1997): one field per step, shared by all
+// members (the reference seeds every member's generator identically,
+// src/data.cu:293-294).  f_x = sum_n amp_n 2 pi ky_n [(r0 kx + r1 ky) sin(th)
+// + (r2 kx + r3 ky) cos(th)], f_y likewise with -kx   (src/data.cu:383-414)
+// ----------------------------------------------------------------------------
+__global__ void k_force_field(const ForceTable* __restrict__ tab, uint64_t seed, float stdev,
+                              const long long* __restrict__ d_step, int step_offset,
+                              float* __restrict__ force, int nx, int ny) {
+    __shared__ float rnd[4][MAX_FORCE_MODES];
+    __shared__ int kxs[MAX_FORCE_MODES], kys[MAX_FORCE_MODES];
+    __shared__ float amps[MAX_FORCE_MODES];
+    const int n = tab->n;
+    const long long step = *d_step + step_offset;
+    for (int m = threadIdx.x; m < n; m += blockDim.x) {
+        double z[4];
+        philox_normal4(seed, uint32_t(step), uint32_t(uint64_t(step) >> 32), uint32_t(m),
+                       STREAM_FORCE, z);
+        for (int a = 0; a < 4; ++a) rnd[a][m] = float(z[a] * double(stdev));
+        kxs[m] = int(tab->kx[m]);
+        kys[m] = int(tab->ky[m]);
+        amps[m] = tab->amp[m];
+    }
+    __syncthreads();
+    const long long node = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long nodes = (long long)nx * ny;
+    if (node >= nodes) return;
+    const int i = int(node % nx), j = int(node / nx);
+    // phase 2 pi (kx x + ky y), x = (i - nx/2)/nx, reduced exactly in integers
+    const long long period = (long long)nx * ny;
+    const float two_pi = 6.28318530717958647692f;
+    float fx = 0.f, fy = 0.f;
+    for (int m = 0; m < n; ++m) {
+        const int kx = kxs[m], ky = kys[m];
+        long long num = (long long)kx * (i - nx / 2) * ny + (long long)ky * (j - ny / 2) * nx;
+        num %= period;
+        if (num < 0) num += period;
+        float s, c;
+        sincospif(2.f * float(double(num) / double(period)), &s, &c);
+        const float fek = (rnd[0][m] * kx + rnd[1][m] * ky) * s + (rnd[2][m] * kx + rnd[3][m] * ky) * c;
+        fx += amps[m] * two_pi * float(ky) * fek;
+        fy += amps[m] * two_pi * float(-kx) * fek;
+    }
+    force[node] = fx;
+    force[nodes + node] = fy;
+}
+
+__global__ void k_step_advance(long long* d_step, int n) { *d_step += n; }
+
+// f := feq(rho, u, v) (src/lbm.h:31-39)
+__global__ void k_set_feq(float* __restrict__ f, const float* __restrict__ rho,
+                          const float* __restrict__ u, const float* __restrict__ v, LbmParams p) {
+    const long long node = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= p.nodes) return;
+    const float r = rho[node], uu = u[node], vv = v[node];
+    const float u2 = (uu * uu + vv * vv) * p.inv_c2;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int ex = q % 3 - 1, ey = q / 3 - 1;
+        const float wq = (ex == 0 && ey == 0) ? W0 : ((ex == 0 || ey == 0) ? W1 : W2);
+        const float uc = (uu * float(ex) + vv * float(ey)) * p.inv_c;
+        f[q * p.nodes + node] = r * wq * (1.f + 3.f * uc + 4.5f * uc * uc - 1.5f * u2);
+    }
+}
+
+// rho, u, v of the stored populations (no streaming), src/letkf.cu:336-357
+__global__ void k_macro(const float* __restrict__ f, float* __restrict__ rho,
+                        float* __restrict__ u, float* __restrict__ v, LbmParams p) {
+    const long long node = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= p.nodes) return;
+    float fc[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) fc[q] = f[q * p.nodes + node];
+    float r, uu, vv;
+    moments(fc, p.c, r, uu, vv);
+    if (rho) rho[node] = r;
+    if (u) u[node] = uu;
+    if (v) v[node] = vv;
+}
+
+// Ensemble mean / biased spread of (u,v) and rho against a truth, summed per
+// block in FP64 (ensemble_stat/main.cpp:108-177).  partials[block][4] =
+// { sum (ubar-ut)^2+(vbar-vt)^2, sum su^2+sv^2, sum (rbar-rt)^2, sum sr^2 }.
+constexpr int STATS_NT = 256;
+__global__ __launch_bounds__(STATS_NT) void k_ensemble_stats(
+    const float* __restrict__ f, const float* __restrict__ trho, const float* __restrict__ tu,
+    const float* __restrict__ tv, double* __restrict__ partials, LbmParams p) {
+    __shared__ double red[4][STATS_NT];
+    const long long node = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[4] = {0, 0, 0, 0};
+    if (node < p.nodes) {
+        double su = 0, sv = 0, sr = 0, suu = 0, svv = 0, srr = 0;
+        const int M = p.n_members;
+        for (int k = 0; k < M; ++k) {
+            const float* fk = f + (long long)k * p.member_stride;
+            float fc[Q];
+#pragma unroll
+            for (int q = 0; q < Q; ++q) fc[q] = fk[q * p.nodes + node];
+            float r, u, v;
+            moments(fc, p.c, r, u, v);
+            su += u; sv += v; sr += r;
+            suu += double(u) * u; svv += double(v) * v; srr += double(r) * r;
+        }
+        const double iM = 1.0 / M;
+        const double mu = su * iM, mv = sv * iM, mr = sr * iM;
+        const double vu = fmax(suu * iM - mu * mu, 0.0), vv = fmax(svv * iM - mv * mv, 0.0),
+                     vr = fmax(srr * iM - mr * mr, 0.0);
+        const double du = mu - tu[node], dv = mv - tv[node], dr = mr - trho[node];
+        acc[0] = du * du + dv * dv;
+        acc[1] = vu + vv;
+        acc[2] = dr * dr;
+        acc[3] = vr;
+    }
+    for (int a = 0; a < 4; ++a) red[a][threadIdx.x] = acc[a];
+    __syncthreads();
+    for (int s = STATS_NT / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int a = 0; a < 4; ++a) red[a][threadIdx.x] += red[a][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int a = 0; a < 4; ++a) partials[blockIdx.x * 4 + a] = red[a][0];
+}
+
+inline int node_blocks(long long nodes, int nt) { return int((nodes + nt - 1) / nt); }
+
+}  // namespace
+
+bool lbm_fused_supported(int nx, int ny) { return nx % TX == 0 && ny % TY == 0; }
+
+hipError_t launch_force_field(Ctx& c, int step_offset) {
+    const int nt = 256;
+    k_force_field<<<node_blocks(c.nodes, nt), nt, 0, c.stream>>>(
+        c.d_force, c.cfg.force_seed, float(c.cfg.force_rand_stdev), c.d_step, step_offset,
+        c.force_xy, c.nx, c.ny);
+    return hipGetLastError();
+}
+
+hipError_t launch_step_advance(Ctx& c, int n) {
+    k_step_advance<<<1, 1, 0, c.stream>>>(c.d_step, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_lbm_fused(Ctx& c, const float* fin, float* fout) {
+    const int tiles_x = c.nx / TX, tiles_y = c.ny / TY;
+    const int tiles = tiles_x * tiles_y;
+    const int nb = tiles * c.M;
+    k_lbm_fused<<<nb, NT, 0, c.stream>>>(fin, fout, c.force_xy, c.lp, tiles_x, tiles);
+    return hipGetLastError();
+}
+
+hipError_t launch_lbm_plain_member(Ctx& c, const float* fin, float* fout, int member) {
+    const int nt = 256;
+    const int nb = node_blocks(c.nodes, nt);
+    const long long off = (long long)member * c.lp.member_stride;
+    k_stream_macro<<<nb, nt, 0, c.stream>>>(fin + off, fout + off, c.plain_ruv, c.lp);
+    k_les<<<nb, nt, 0, c.stream>>>(c.plain_ruv, c.plain_nus, c.lp);
+    k_collide<<<nb, nt, 0, c.stream>>>(fout + off, c.plain_ruv, c.plain_nus, c.force_xy, c.lp);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_feq(Ctx& c, int member, const float* d_rho, const float* d_u,
+                          const float* d_v) {
+    const int nt = 256;
+    k_set_feq<<<node_blocks(c.nodes, nt), nt, 0, c.stream>>>(
+        c.f[c.cur] + (long long)member * c.lp.member_stride, d_rho, d_u, d_v, c.lp);
+    return hipGetLastError();
+}
+
+hipError_t launch_macro(Ctx& c, const float* f_member, float* rho, float* u, float* v) {
+    const int nt = 256;
+    k_macro<<<node_blocks(c.nodes, nt), nt, 0, c.stream>>>(f_member, rho, u, v, c.lp);
+    return hipGetLastError();
+}
+
+int ensemble_stats_blocks(const Ctx& c) { return node_blocks(c.nodes, STATS_NT); }
+
+hipError_t launch_ensemble_stats(Ctx& c, const float* truth_rho, const float* truth_u,
+                                 const float* truth_v, double* partials, int n_blocks) {
+    k_ensemble_stats<<<n_blocks, STATS_NT, 0, c.stream>>>(c.f[c.cur], truth_rho, truth_u,
+                                                          truth_v, partials, c.lp);
+    return hipGetLastError();
+}
+
+}  // namespace lbmletkf

@@ -1,0 +1,309 @@
+"""ctypes binding of liblbmletkf.so (the C-ABI in include/lbmletkf.h).
+
+The names mirror the reference's host interface so that callers read like the
+reference's driver (src/main.cu:22-155):
+
+* ``update(ctx, n)``                 -- n calls of ``update(data&, t)`` (src/lbm.h:16-29)
+* ``DataAssimilator(ctx).assimilate(yo)`` -- ``DataAssimilator::assimilate`` (src/assimilate.h:54-68)
+
+Every call raises :class:`LbmLetkfError` on a non-zero status.  There is no
+fallback: if the HIP library is missing the import fails.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblbmletkf.so")
+
+OK, ERR_ARG, ERR_HIP, ERR_NUMERIC, ERR_STATE = 0, -1, -2, -3, -4
+FORCE_NONE, FORCE_WATANABE97 = 0, 1
+IC_REFERENCE, IC_K4_SPECTRUM = 0, 1
+OBS_U, OBS_V, OBS_RHO = 1, 2, 4
+LOC_GC_WINDOW, LOC_GAUSS = 0, 1
+VARIANT_PLAIN, VARIANT_MIXED = 0, 1
+
+
+class Config(C.Structure):
+    """Field-for-field mirror of ``lbmletkf_config``."""
+
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int),
+        ("c", C.c_double), ("dx", C.c_double), ("dt", C.c_double), ("nu", C.c_double),
+        ("rho_ref", C.c_double), ("cs_smag", C.c_double), ("u_ref", C.c_double),
+        ("friction_rate", C.c_double),
+        ("force_kind", C.c_int),
+        ("force_kf", C.c_double), ("force_amp_total", C.c_double), ("force_sigma_k", C.c_double),
+        ("force_dk", C.c_double), ("force_rand_stdev", C.c_double),
+        ("force_seed", C.c_ulonglong),
+        ("ic_kind", C.c_int),
+        ("ic_amp", C.c_double), ("ic_k0", C.c_double), ("ic_dk", C.c_double), ("ic_sigma", C.c_double),
+        ("obs_stride", C.c_int), ("obs_offset", C.c_int), ("obs_vars", C.c_int),
+        ("obs_sigma_u", C.c_double), ("obs_sigma_rho", C.c_double),
+        ("loc_kind", C.c_int), ("loc_rloc_len", C.c_int),
+        ("loc_scale", C.c_double), ("loc_cutoff", C.c_double), ("inflation", C.c_double),
+        ("da_interval", C.c_int), ("variant", C.c_int),
+        ("jacobi_tol", C.c_double), ("jacobi_max_sweeps", C.c_int),
+    ]
+
+    def copy(self) -> "Config":
+        c = Config()
+        C.memmove(C.byref(c), C.byref(self), C.sizeof(Config))
+        return c
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class LbmLetkfError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"lbmletkf status {status}: {message}")
+        self.status = status
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+    lib = C.CDLL(LIB_PATH)
+    P, I, D, F = C.c_void_p, C.c_int, C.c_double, C.POINTER(C.c_float)
+    sig = {
+        "lbmletkf_version": ([], C.c_char_p),
+        "lbmletkf_config_preset": ([C.c_char_p, C.POINTER(Config)], I),
+        "lbmletkf_config_check": ([C.POINTER(Config), I], I),
+        "lbmletkf_create": ([C.POINTER(Config), I, C.POINTER(P)], I),
+        "lbmletkf_destroy": ([P], I),
+        "lbmletkf_last_error": ([P], C.c_char_p),
+        "lbmletkf_init_member": ([P, I, C.c_ulonglong], I),
+        "lbmletkf_set_state": ([P, I, P], I),
+        "lbmletkf_get_state": ([P, I, P], I),
+        "lbmletkf_set_macro": ([P, I, P, P, P], I),
+        "lbmletkf_get_macro": ([P, I, P, P, P], I),
+        "lbmletkf_get_macro_device": ([P, I, P, P, P], I),
+        "lbmletkf_forecast": ([P, I], I),
+        "lbmletkf_set_step": ([P, C.c_longlong], I),
+        "lbmletkf_get_step": ([P, C.POINTER(C.c_longlong)], I),
+        "lbmletkf_setup_obs": ([P], I),
+        "lbmletkf_obs_count": ([P, C.POINTER(I), C.POINTER(I)], I),
+        "lbmletkf_observe": ([P, I, C.c_ulonglong, C.c_ulonglong, I, P, I], I),
+        "lbmletkf_obs_from_fields": ([P, P, P, P, P], I),
+        "lbmletkf_analysis": ([P, P, I], I),
+        "lbmletkf_cycle": ([P, P], I),
+        "lbmletkf_ensemble_stats": ([P, P, P, P, I, P], I),
+        "lbmletkf_local_obs": ([P, I, C.POINTER(I), P, P, I], I),
+        "lbmletkf_set_dump": ([P, I, I], I),
+        "lbmletkf_get_dump": ([P, P, P, P], I),
+        "lbmletkf_profile_enable": ([P, I], I),
+        "lbmletkf_profile_read": ([P, C.c_char_p, C.POINTER(D), C.POINTER(C.c_longlong),
+                                   C.POINTER(C.c_longlong)], I),
+        "lbmletkf_get_stream": ([P], P),
+        "lbmletkf_synchronize": ([P], I),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    return lib
+
+
+lib = _load()
+
+
+def _ptr(a: np.ndarray) -> C.c_void_p:
+    return C.c_void_p(a.ctypes.data)
+
+
+def _check(status: int, ctx_handle=None):
+    if status != OK:
+        msg = lib.lbmletkf_last_error(ctx_handle)
+        raise LbmLetkfError(status, msg.decode() if msg else "")
+
+
+def preset(name: str) -> Config:
+    cfg = Config()
+    _check(lib.lbmletkf_config_preset(name.encode(), C.byref(cfg)))
+    return cfg
+
+
+def config_check(cfg: Config, n_members: int) -> None:
+    _check(lib.lbmletkf_config_check(C.byref(cfg), n_members))
+
+
+def version() -> str:
+    return lib.lbmletkf_version().decode()
+
+
+class Context:
+    """An ensemble of M members on the current GPU (``data`` x M + ``DataAssimilator``)."""
+
+    def __init__(self, cfg: Config, n_members: int):
+        self.cfg = cfg.copy()
+        self.M = n_members
+        self.nx, self.ny = cfg.nx, cfg.ny
+        self.nodes = cfg.nx * cfg.ny
+        h = C.c_void_p()
+        _check(lib.lbmletkf_create(C.byref(self.cfg), n_members, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.lbmletkf_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _ck(self, st):
+        _check(st, self.h)
+
+    # state ------------------------------------------------------------------
+    def init_member(self, member: int, seed: int):
+        self._ck(lib.lbmletkf_init_member(self.h, member, seed))
+
+    def set_state(self, member: int, f: np.ndarray):
+        a = np.ascontiguousarray(f, dtype=np.float32).reshape(9, self.ny, self.nx)
+        self._ck(lib.lbmletkf_set_state(self.h, member, _ptr(a)))
+
+    def get_state(self, member: int) -> np.ndarray:
+        a = np.empty((9, self.ny, self.nx), np.float32)
+        self._ck(lib.lbmletkf_get_state(self.h, member, _ptr(a)))
+        return a
+
+    def get_all_states(self) -> np.ndarray:
+        return np.stack([self.get_state(k) for k in range(self.M)])
+
+    def set_all_states(self, f: np.ndarray):
+        for k in range(self.M):
+            self.set_state(k, f[k])
+
+    def set_macro(self, member: int, rho, u, v):
+        arrs = [np.ascontiguousarray(x, dtype=np.float32).reshape(self.ny, self.nx) for x in (rho, u, v)]
+        self._ck(lib.lbmletkf_set_macro(self.h, member, *[_ptr(a) for a in arrs]))
+
+    def get_macro(self, member: int):
+        r, u, v = (np.empty((self.ny, self.nx), np.float32) for _ in range(3))
+        self._ck(lib.lbmletkf_get_macro(self.h, member, _ptr(r), _ptr(u), _ptr(v)))
+        return r, u, v
+
+    # forecast -----------------------------------------------------------------
+    def forecast(self, n_steps: int = 1):
+        self._ck(lib.lbmletkf_forecast(self.h, n_steps))
+
+    def set_step(self, step: int):
+        self._ck(lib.lbmletkf_set_step(self.h, step))
+
+    def get_step(self) -> int:
+        s = C.c_longlong()
+        self._ck(lib.lbmletkf_get_step(self.h, C.byref(s)))
+        return s.value
+
+    # observations and analysis -----------------------------------------------
+    def obs_count(self):
+        ns, no = C.c_int(), C.c_int()
+        self._ck(lib.lbmletkf_obs_count(self.h, C.byref(ns), C.byref(no)))
+        return ns.value, no.value
+
+    def observe(self, member: int, seed: int, counter: int, noise: bool = True) -> np.ndarray:
+        _, n_obs = self.obs_count()
+        yo = np.empty(n_obs, np.float32)
+        self._ck(lib.lbmletkf_observe(self.h, member, seed, counter, int(noise), _ptr(yo), 0))
+        return yo
+
+    def observe_device(self, member: int, seed: int, counter: int, yo_dev: int, noise: bool = True):
+        self._ck(lib.lbmletkf_observe(self.h, member, seed, counter, int(noise), C.c_void_p(yo_dev), 1))
+
+    def obs_from_fields(self, rho=None, u=None, v=None) -> np.ndarray:
+        _, n_obs = self.obs_count()
+        yo = np.empty(n_obs, np.float32)
+        arrs = [None if x is None else np.ascontiguousarray(x, np.float32) for x in (rho, u, v)]
+        ptrs = [None if a is None else _ptr(a) for a in arrs]
+        self._ck(lib.lbmletkf_obs_from_fields(self.h, *ptrs, _ptr(yo)))
+        return yo
+
+    def analysis(self, yo: np.ndarray):
+        a = np.ascontiguousarray(yo, dtype=np.float32)
+        self._ck(lib.lbmletkf_analysis(self.h, _ptr(a), 0))
+        self.synchronize()
+
+    def analysis_device(self, yo_dev: int):
+        self._ck(lib.lbmletkf_analysis(self.h, C.c_void_p(yo_dev), 1))
+
+    def cycle_device(self, yo_dev: int):
+        self._ck(lib.lbmletkf_cycle(self.h, C.c_void_p(yo_dev)))
+
+    def local_obs(self, gridpoint: int, cap: int = 4096):
+        n = C.c_int()
+        sites = np.empty(cap, np.int32)
+        w = np.empty(cap, np.float32)
+        self._ck(lib.lbmletkf_local_obs(self.h, gridpoint, C.byref(n), _ptr(sites), _ptr(w), cap))
+        k = min(n.value, cap)
+        return sites[:k].copy(), w[:k].copy()
+
+    def set_dump(self, g0: int, count: int):
+        self._ck(lib.lbmletkf_set_dump(self.h, g0, count))
+        self._dump = (g0, count)
+
+    def get_dump(self):
+        g0, count = self._dump
+        M = self.M
+        W = np.empty((count, M, M))
+        w = np.empty((count, M))
+        ev = np.empty((count, M))
+        self._ck(lib.lbmletkf_get_dump(self.h, _ptr(W), _ptr(w), _ptr(ev)))
+        return W, w, ev
+
+    def ensemble_stats(self, truth_rho, truth_u, truth_v):
+        arrs = [np.ascontiguousarray(x, np.float32) for x in (truth_rho, truth_u, truth_v)]
+        out = np.empty(4)
+        self._ck(lib.lbmletkf_ensemble_stats(self.h, *[_ptr(a) for a in arrs], 0, _ptr(out)))
+        return out
+
+    def ensemble_stats_device(self, d_rho: int, d_u: int, d_v: int):
+        out = np.empty(4)
+        self._ck(lib.lbmletkf_ensemble_stats(self.h, C.c_void_p(d_rho), C.c_void_p(d_u),
+                                             C.c_void_p(d_v), 1, _ptr(out)))
+        return out
+
+    def get_macro_device(self, member: int, d_rho: int, d_u: int, d_v: int):
+        self._ck(lib.lbmletkf_get_macro_device(self.h, member, C.c_void_p(d_rho), C.c_void_p(d_u),
+                                               C.c_void_p(d_v)))
+
+    def profile_enable(self, on: bool = True):
+        self._ck(lib.lbmletkf_profile_enable(self.h, int(on)))
+
+    def profile_read(self, phase: str):
+        """(seconds, kernel launches, work count) accumulated for one phase."""
+        sec, n, w = C.c_double(), C.c_longlong(), C.c_longlong()
+        self._ck(lib.lbmletkf_profile_read(self.h, phase.encode(), C.byref(sec), C.byref(n), C.byref(w)))
+        return sec.value, n.value, w.value
+
+    @property
+    def stream(self) -> int:
+        return lib.lbmletkf_get_stream(self.h) or 0
+
+    def synchronize(self):
+        self._ck(lib.lbmletkf_synchronize(self.h))
+
+
+def update(ctx: Context, n_steps: int = 1):
+    """``update(data&, t)`` applied n times to every member (src/lbm.h:16-29)."""
+    ctx.forecast(n_steps)
+
+
+class DataAssimilator:
+    """``DataAssimilator`` (src/assimilate.h:14-82) bound to one context."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        ctx.obs_count()  # init_letkf: builds the local observation sets
+
+    def assimilate(self, yo: np.ndarray):
+        self.ctx.analysis(yo)

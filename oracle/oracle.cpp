@@ -1,0 +1,607 @@
+// FP64 CPU oracle of the LBM-LETKF cycle -- TEST INFRASTRUCTURE ONLY (see oracle.h).
+// Straight loops in double precision, parallelised with OpenMP over nodes,
+// members or gridpoints.  Nothing here is shipped or called by the product.
+#include "oracle.h"
+
+#include <omp.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <vector>
+
+namespace {
+
+constexpr double PI = 3.14159265358979323846;
+constexpr int Q = 9;
+
+inline int ex_of(int q) { return q % 3 - 1; }  // src/config.h:257
+inline int ey_of(int q) { return q / 3 - 1; }  // src/config.h:258
+inline double wq_of(int q) {                    // src/config.h:260-263
+    const int ee = ex_of(q) * ex_of(q) + ey_of(q) * ey_of(q);
+    return ee == 0 ? 4.0 / 9.0 : (ee == 1 ? 1.0 / 9.0 : 1.0 / 36.0);
+}
+inline int pmod(int a, int n) { int r = a % n; return r < 0 ? r + n : r; }
+
+// src/lbm.h:31-39 (non-ICLBM branch)
+inline double feq_q(const orc_config& c, double rho, double u, double v, int q) {
+    const double uc = (u * ex_of(q) + v * ey_of(q)) / c.c;
+    const double uu = (u * u + v * v) / (c.c * c.c);
+    return rho * wq_of(q) * (1.0 + 3.0 * uc + 4.5 * uc * uc - 1.5 * uu);
+}
+
+// src/lbm.h:41-70 (first-order term only; FORCE_SECOND_ORDER and
+// VELOCITY_FORCE_CORRECTION are not defined in src/config.h:16-17)
+inline double force_q(const orc_config& c, double fx_, double fy_, int q, double rho, double u, double v) {
+    const double cx = c.u_ref * ex_of(q);
+    const double cy = c.u_ref * ey_of(q);
+    const double cs2 = c.c * c.c / 3.0;
+    const double fx = (fx_ - c.friction_rate / c.dt * u) * c.dt;
+    const double fy = (fy_ - c.friction_rate / c.dt * v) * c.dt;
+    return (cx * fx + cy * fy) / cs2 * wq_of(q) * rho;
+}
+
+void philox_block(const uint32_t in[4], uint32_t k0, uint32_t k1, uint32_t out[4]) {
+    uint32_t x[4] = {in[0], in[1], in[2], in[3]};
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = uint64_t(0xD2511F53u) * x[0];
+        const uint64_t p1 = uint64_t(0xCD9E8D57u) * x[2];
+        const uint32_t hi0 = uint32_t(p0 >> 32), lo0 = uint32_t(p0);
+        const uint32_t hi1 = uint32_t(p1 >> 32), lo1 = uint32_t(p1);
+        const uint32_t y0 = hi1 ^ x[1] ^ k0, y2 = hi0 ^ x[3] ^ k1;
+        x[0] = y0;
+        x[1] = lo1;
+        x[2] = y2;
+        x[3] = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    for (int a = 0; a < 4; ++a) out[a] = x[a];
+}
+
+void normal4(uint64_t key, const uint32_t ctr[4], double out[4]) {
+    uint32_t r[4];
+    philox_block(ctr, uint32_t(key), uint32_t(key >> 32), r);
+    for (int h = 0; h < 2; ++h) {
+        const double u1 = (double(r[2 * h]) + 0.5) / 4294967296.0;
+        const double u2 = (double(r[2 * h + 1]) + 0.5) / 4294967296.0;
+        const double rad = std::sqrt(-2.0 * std::log(u1));
+        out[2 * h] = rad * std::cos(2 * PI * u2);
+        out[2 * h + 1] = rad * std::sin(2 * PI * u2);
+    }
+}
+
+constexpr uint32_t STREAM_FORCE = 0x464f5243u, STREAM_IC = 0x49434b34u, STREAM_OBS = 0x4f425345u;
+
+struct ObsInfo {
+    int n_sx, n_sy, n_sites, n_vars;
+    int var_id[3];
+    double rinv[3];
+};
+
+ObsInfo obs_info(const orc_config& c) {
+    ObsInfo o{};
+    o.n_sx = c.nx / c.obs_stride;
+    o.n_sy = c.ny / c.obs_stride;
+    o.n_sites = o.n_sx * o.n_sy;
+    const double ru = 1.0 / (c.obs_sigma_u * c.obs_sigma_u), rr = 1.0 / (c.obs_sigma_rho * c.obs_sigma_rho);
+    if (c.obs_vars & 1) { o.var_id[o.n_vars] = 0; o.rinv[o.n_vars++] = ru; }
+    if (c.obs_vars & 2) { o.var_id[o.n_vars] = 1; o.rinv[o.n_vars++] = ru; }
+    if (c.obs_vars & 4) { o.var_id[o.n_vars] = 2; o.rinv[o.n_vars++] = rr; }
+    return o;
+}
+
+void moments(const orc_config& c, const double* f, long long N, long long node, double& r, double& u, double& v) {
+    // src/lbm.h:72-109
+    double rr = 0, jx = 0, jy = 0;
+    for (int q = 0; q < Q; ++q) {
+        const double fq = f[q * N + node];
+        rr += fq;
+        jx += fq * c.c * ex_of(q);
+        jy += fq * c.c * ey_of(q);
+    }
+    r = rr;
+    u = jx / rr;
+    v = jy / rr;
+}
+
+}  // namespace
+
+extern "C" {
+
+int orc_sizeof_config(void) { return int(sizeof(orc_config)); }
+void orc_set_threads(int n) { if (n > 0) omp_set_num_threads(n); }
+
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    philox_block(ctr, key[0], key[1], out);
+}
+void orc_philox_normal4(uint64_t key, const uint32_t ctr[4], double out[4]) { normal4(key, ctr, out); }
+
+// src/data.cu:227-290
+int orc_force_modes(const orc_config* cfg, double* kx, double* ky, double* amp, int cap) {
+    const orc_config& c = *cfg;
+    if (c.force_kind != 1) return 0;
+    const int kmax_search = int(c.force_kf + c.force_dk);
+    std::vector<double> vkx, vky, va;
+    long double asum = 0;
+    for (int y = 0; y < kmax_search; y++) {
+        for (int x = 0; x < kmax_search; x++) {
+            if (x == 0 && y == 0) continue;
+            const double k = std::sqrt(double(y * y + x * x));
+            if (c.force_kf - c.force_dk <= k && k <= c.force_kf + c.force_dk) {
+                const double a = std::exp(-(k - c.force_kf) * (k - c.force_kf) / c.force_sigma_k);
+                asum += a;
+                vkx.push_back(x);
+                vky.push_back(y);
+                va.push_back(a);
+            }
+        }
+    }
+    const int n = int(std::min<size_t>(vkx.size(), 100));
+    for (int m = 0; m < n && m < cap; ++m) {
+        kx[m] = vkx[m];
+        ky[m] = vky[m];
+        amp[m] = double((long double)va[m] * c.force_amp_total / asum);
+    }
+    return n;
+}
+
+// src/data.cu:383-414, randoms from Philox instead of cuRAND MTGP32
+void orc_force_field(const orc_config* cfg, long long step, double* fx, double* fy) {
+    const orc_config& c = *cfg;
+    double kx[100], ky[100], amp[100];
+    const int n = orc_force_modes(cfg, kx, ky, amp, 100);
+    std::vector<double> rnd(size_t(n) * 4);
+    for (int m = 0; m < n; ++m) {
+        const uint32_t ctr[4] = {uint32_t(step), uint32_t(uint64_t(step) >> 32), uint32_t(m), STREAM_FORCE};
+        double z[4];
+        normal4(c.force_seed, ctr, z);
+        for (int a = 0; a < 4; ++a) rnd[m * 4 + a] = z[a] * c.force_rand_stdev;
+    }
+    const int nx = c.nx, ny = c.ny;
+#pragma omp parallel for
+    for (int j = 0; j < ny; ++j) {
+        for (int i = 0; i < nx; ++i) {
+            const double x = double(i - nx / 2) / nx, y = double(j - ny / 2) / ny;
+            double sx = 0, sy = 0;
+            for (int m = 0; m < n; ++m) {
+                const double th = 2 * PI * (kx[m] * x + ky[m] * y);
+                const double* r = &rnd[m * 4];
+                const double fek = (r[0] * kx[m] + r[1] * ky[m]) * std::sin(th) + (r[2] * kx[m] + r[3] * ky[m]) * std::cos(th);
+                sx += amp[m] * 2 * PI * ky[m] * fek;
+                sy += amp[m] * 2 * PI * (-kx[m]) * fek;
+            }
+            fx[i + nx * j] = sx;
+            fy[i + nx * j] = sy;
+        }
+    }
+}
+
+void orc_feq(const orc_config* cfg, const double* rho, const double* u, const double* v, double* f) {
+    const long long N = (long long)cfg->nx * cfg->ny;
+#pragma omp parallel for
+    for (long long n = 0; n < N; ++n)
+        for (int q = 0; q < Q; ++q) f[q * N + n] = feq_q(*cfg, rho[n], u[n], v[n], q);
+}
+
+void orc_macro(const orc_config* cfg, const double* f, double* rho, double* u, double* v) {
+    const long long N = (long long)cfg->nx * cfg->ny;
+#pragma omp parallel for
+    for (long long n = 0; n < N; ++n) {
+        double r, uu, vv;
+        moments(*cfg, f, N, n, r, uu, vv);
+        if (rho) rho[n] = r;
+        if (u) u[n] = uu;
+        if (v) v[n] = vv;
+    }
+}
+
+// src/lbm.cu:187-200 (pull streaming)
+void orc_stream(const orc_config* cfg, const double* f, double* fn) {
+    const int nx = cfg->nx, ny = cfg->ny;
+    const long long N = (long long)nx * ny;
+#pragma omp parallel for
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i)
+            for (int q = 0; q < Q; ++q) {
+                const int iu = pmod(i - ex_of(q), nx), ju = pmod(j - ey_of(q), ny);
+                fn[q * N + i + (long long)nx * j] = f[q * N + iu + (long long)nx * ju];
+            }
+}
+
+// src/lbm.cu:135-176 (Smagorinsky, LES_CSM not defined)
+void orc_les(const orc_config* cfg, const double* u, const double* v, double* nus) {
+    const int nx = cfg->nx, ny = cfg->ny;
+    const double dx = cfg->dx, cs = cfg->cs_smag;
+#pragma omp parallel for
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            const long long e = pmod(i + 1, nx) + (long long)nx * j, w = pmod(i - 1, nx) + (long long)nx * j;
+            const long long n = i + (long long)nx * pmod(j + 1, ny), s = i + (long long)nx * pmod(j - 1, ny);
+            const double ux = (u[e] - u[w]) / (2 * dx), uy = (u[n] - u[s]) / (2 * dx);
+            const double vx = (v[e] - v[w]) / (2 * dx), vy = (v[n] - v[s]) / (2 * dx);
+            const double s00 = ux, s01 = 0.5 * (uy + vx), s10 = s01, s11 = vy;
+            const double ss = std::sqrt(2 * (s00 * s00 + s01 * s10 + s10 * s01 + s11 * s11));
+            nus[i + (long long)nx * j] = cs * cs * dx * dx * ss;
+        }
+}
+
+void orc_lbm_step(const orc_config* cfg, int M, const double* fin, double* fout, long long step) {
+    const orc_config& c = *cfg;
+    const long long N = (long long)c.nx * c.ny;
+    std::vector<double> Fx(N, 0.0), Fy(N, 0.0);
+    if (c.force_kind == 1) orc_force_field(cfg, step, Fx.data(), Fy.data());
+    std::vector<double> r(N), u(N), v(N), nus(N);
+    for (int k = 0; k < M; ++k) {
+        const double* f = fin + (long long)k * Q * N;
+        double* fn = fout + (long long)k * Q * N;
+        orc_stream(cfg, f, fn);                               // src/lbm.cu:178-214
+        orc_macro(cfg, fn, r.data(), u.data(), v.data());
+        orc_les(cfg, u.data(), v.data(), nus.data());         // src/lbm.cu:135-176
+#pragma omp parallel for
+        for (long long n = 0; n < N; ++n) {                    // src/lbm.cu:216-271
+            const double nu = c.nu + nus[n];
+            const double tau = 0.5 + 3.0 * nu / c.c / c.c / c.dt;
+            const double omega = 1.0 / tau;
+            for (int q = 0; q < Q; ++q) {
+                double fc = fn[q * N + n];
+                fc -= omega * (fc - feq_q(c, r[n], u[n], v[n], q));   // lbm_collision_srt.h:8-16
+                fc += force_q(c, Fx[n], Fy[n], q, r[n], u[n], v[n]);
+                fn[q * N + n] = fc;
+            }
+        }
+    }
+}
+
+// src/data.cu:91-186 (REFERENCE kind); the K4 kind uses the same streamfunction
+// form over the half plane with Philox phases.  Direct sums over all modes.
+void orc_initial_velocity(const orc_config* cfg, unsigned long long seed, double* u, double* v) {
+    const orc_config& c = *cfg;
+    const int nx = c.nx, ny = c.ny;
+    struct M4 { double kx, ky, p, th; };
+    std::vector<M4> modes;
+    if (c.ic_kind == 0) {
+        const int nk = nx / 2;
+        std::mt19937 eng(static_cast<std::mt19937::result_type>(seed));
+        std::uniform_real_distribution<float> dist(-1, 1);
+        const float k0 = float(c.ic_k0), dk = float(c.ic_dk), sg = float(c.ic_sigma);
+        for (int kj = 0; kj < nk; kj++)
+            for (int ki = 0; ki < nk; ki++) {
+                const float k = std::sqrt(float(ki * ki + kj * kj));
+                const float pk = (k0 - dk <= k && k <= k0 + dk) ? std::exp(-(k - k0) * (k - k0) / sg) : 0.f;
+                const float th = float(dist(eng) * PI);
+                if (pk != 0.f) modes.push_back({2 * PI * ki / nx, 2 * PI * kj / ny, double(pk), double(th)});
+            }
+    } else {
+        const int nk = std::min(nx, ny) / 2;
+        for (int kj = 0; kj < nk; ++kj)
+            for (int ki = -nk + 1; ki < nk; ++ki) {
+                if (kj == 0 && ki <= 0) continue;
+                const double k = std::sqrt(double(ki * ki + kj * kj));
+                if (k >= nk) continue;
+                const uint32_t ctr[4] = {uint32_t(ki + nk), uint32_t(kj), 0u, STREAM_IC};
+                const uint32_t key[2] = {uint32_t(seed), uint32_t(seed >> 32)};
+                uint32_t r[4];
+                orc_philox4x32_10(ctr, key, r);
+                const double th = PI * (2.0 * (double(r[0]) + 0.5) / 4294967296.0 - 1.0);
+                const double p = std::sqrt(k) * std::exp(-(k / c.ic_k0) * (k / c.ic_k0));
+                modes.push_back({2 * PI * ki / nx, 2 * PI * kj / ny, p, th});
+            }
+    }
+    const long long N = (long long)nx * ny;
+#pragma omp parallel for
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            double su = 0, sv = 0;
+            for (const M4& m : modes) {
+                const double cs = std::cos(m.kx * i + m.ky * j + m.th);
+                su += m.p * m.ky * cs;
+                sv += m.p * -m.kx * cs;
+            }
+            u[i + (long long)nx * j] = su;
+            v[i + (long long)nx * j] = sv;
+        }
+    double scale;
+    if (c.ic_kind == 0) {
+        double vmax = 1e-30 * c.u_ref;
+        for (long long n = 0; n < N; ++n) vmax = std::max(vmax, std::sqrt(u[n] * u[n] + v[n] * v[n]));
+        scale = c.ic_amp / vmax;
+    } else {
+        double e = 0;
+        for (long long n = 0; n < N; ++n) e += u[n] * u[n] + v[n] * v[n];
+        const double rms = std::sqrt(e / double(N));
+        scale = rms > 0 ? c.ic_amp / rms : 0;
+    }
+    for (long long n = 0; n < N; ++n) {
+        u[n] *= scale;
+        v[n] *= scale;
+    }
+}
+
+void orc_obs_count(const orc_config* cfg, int* n_sites, int* n_obs) {
+    const ObsInfo o = obs_info(*cfg);
+    if (n_sites) *n_sites = o.n_sites;
+    if (n_obs) *n_obs = o.n_sites * o.n_vars;
+}
+
+// src/letkf.cu:105-109
+double orc_gaspari_cohn(double d) {
+    return (d <= 0) ? 1
+         : (d <= 1) ? -d * d * d * d * d / 4. + d * d * d * d / 2. + d * d * d * 5. / 8. - d * d * 5. / 3. + 1
+         : (d <= 2) ? d * d * d * d * d / 12. - d * d * d * d / 2. + d * d * d * 5. / 8. + d * d * 5. / 3. - d * 5 + 4 - 2. / 3. / d
+         : 0;
+}
+
+int orc_local_obs(const orc_config* cfg, int gridpoint, int* sites, double* weights, int cap) {
+    const orc_config& c = *cfg;
+    const ObsInfo o = obs_info(c);
+    const int i = gridpoint % c.nx, j = gridpoint / c.nx;
+    const int p = c.obs_stride;
+    int n = 0;
+    auto put = [&](int s, double w) {
+        if (n < cap) {
+            if (sites) sites[n] = s;
+            if (weights) weights[n] = w;
+        }
+        ++n;
+    };
+    if (c.loc_kind == 0) {
+        // window geometry, src/letkf.cu:129-140
+        int n_obs_local, obs_offset, n_obs_x;
+        double d_local;
+        if (c.loc_rloc_len <= 0) {
+            d_local = p - 1;
+            n_obs_local = p == 1 ? 0 : 2;
+            obs_offset = p == 1 ? 0 : 1;
+            n_obs_x = p == 1 ? 1 : 4;
+        } else {
+            d_local = double(c.loc_rloc_len) * p;
+            obs_offset = 0;
+            n_obs_local = c.loc_rloc_len * 2;
+            n_obs_x = n_obs_local * 2 + 1;
+        }
+        for (int jr = 0; jr < n_obs_x; ++jr)
+            for (int ir = 0; ir < n_obs_x; ++ir) {
+                // src/letkf.cu:164-165 and 248-254
+                const int io = (int(i / p) + ir - (n_obs_local - obs_offset)) * p;
+                const int jo = (int(j / p) + jr - (n_obs_local - obs_offset)) * p;
+                const int di = std::abs(i - io), dj = std::abs(j - jo);
+                const int iop = (io + c.nx) % c.nx, jop = (jo + c.ny) % c.ny;
+                double w = 1.0;
+                if (p > 1) {  // src/letkf.cu:182-197
+                    const double cutoff = d_local + 1e-6;
+                    w = orc_gaspari_cohn(std::sqrt(double(di) * di + double(dj) * dj) / cutoff);
+                }
+                if (di > n_obs_local * p || dj > n_obs_local * p) w = 0.0;  // src/letkf.cu:265
+                put(iop / p + (jop / p) * o.n_sx, w);
+            }
+    } else {
+        const int R = int(std::floor(c.loc_cutoff));
+        for (int sy = (int)std::ceil(double(j - c.obs_offset - R) / p); sy * p + c.obs_offset <= j + R; ++sy)
+            for (int sx = (int)std::ceil(double(i - c.obs_offset - R) / p); sx * p + c.obs_offset <= i + R; ++sx) {
+                const int dx = i - (sx * p + c.obs_offset), dy = j - (sy * p + c.obs_offset);
+                const double d2 = double(dx * dx + dy * dy);
+                if (d2 <= c.loc_cutoff * c.loc_cutoff)
+                    put(pmod(sx, o.n_sx) + pmod(sy, o.n_sy) * o.n_sx, std::exp(-d2 / (2 * c.loc_scale * c.loc_scale)));
+            }
+    }
+    return n;
+}
+
+// H(x) at every site for every member: hx[(s*n_vars+a)*M + k]  (src/letkf.cu:150-187)
+void orc_obs_moments(const orc_config* cfg, int M, const double* f, double* hx) {
+    const orc_config& c = *cfg;
+    const ObsInfo o = obs_info(c);
+    const long long N = (long long)c.nx * c.ny;
+#pragma omp parallel for
+    for (int s = 0; s < o.n_sites; ++s) {
+        const int sx = s % o.n_sx, sy = s / o.n_sx;
+        const long long node = (sx * c.obs_stride + c.obs_offset) + (long long)c.nx * (sy * c.obs_stride + c.obs_offset);
+        for (int k = 0; k < M; ++k) {
+            double r, u, v;
+            moments(c, f + (long long)k * Q * N, N, node, r, u, v);
+            const double val[3] = {u, v, r};
+            for (int a = 0; a < o.n_vars; ++a) hx[((long long)s * o.n_vars + a) * M + k] = val[o.var_id[a]];
+        }
+    }
+}
+
+void orc_observe(const orc_config* cfg, const double* hx, unsigned long long seed,
+                 unsigned long long counter, int noise, double* yo) {
+    const ObsInfo o = obs_info(*cfg);
+    const int n_obs = o.n_sites * o.n_vars;
+    for (int a = 0; a < n_obs; ++a) {
+        double y = hx[a];
+        if (noise) {
+            const uint32_t ctr[4] = {uint32_t(counter), uint32_t(counter >> 32), uint32_t(a / 4), STREAM_OBS};
+            double z[4];
+            normal4(seed, ctr, z);
+            const double sig = o.var_id[a % o.n_vars] == 2 ? cfg->obs_sigma_rho : cfg->obs_sigma_u;
+            y += sig * z[a % 4];
+        }
+        yo[a] = y;
+    }
+}
+
+// Row-cyclic Jacobi (Golub & Van Loan, Matrix Computations, Alg. 8.5.3).
+int orc_jacobi_eig(int n, double* A, double* V, double* d, double tol, int max_sweeps) {
+    for (int a = 0; a < n * n; ++a) V[a] = 0.0;
+    for (int a = 0; a < n; ++a) V[a * n + a] = 1.0;
+    int sweep = 0;
+    for (; sweep < max_sweeps; ++sweep) {
+        double off = 0, tot = 0;
+        for (int a = 0; a < n; ++a)
+            for (int b = 0; b < n; ++b) {
+                const double x = A[a * n + b];
+                tot += x * x;
+                if (a != b) off += x * x;
+            }
+        if (off <= tol * tol * tot) break;
+        for (int p = 0; p < n - 1; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                const double apq = A[p * n + q];
+                if (apq == 0.0) continue;
+                const double theta = (A[q * n + q] - A[p * n + p]) / (2 * apq);
+                const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1));
+                const double cs = 1 / std::sqrt(t * t + 1), sn = t * cs;
+                for (int k = 0; k < n; ++k) {  // columns p, q
+                    const double akp = A[k * n + p], akq = A[k * n + q];
+                    A[k * n + p] = cs * akp - sn * akq;
+                    A[k * n + q] = sn * akp + cs * akq;
+                }
+                for (int k = 0; k < n; ++k) {  // rows p, q
+                    const double apk = A[p * n + k], aqk = A[q * n + k];
+                    A[p * n + k] = cs * apk - sn * aqk;
+                    A[q * n + k] = sn * apk + cs * aqk;
+                }
+                A[p * n + q] = 0.0;
+                A[q * n + p] = 0.0;
+                for (int k = 0; k < n; ++k) {
+                    const double vkp = V[k * n + p], vkq = V[k * n + q];
+                    V[k * n + p] = cs * vkp - sn * vkq;
+                    V[k * n + q] = sn * vkp + cs * vkq;
+                }
+            }
+    }
+    for (int a = 0; a < n; ++a) d[a] = A[a * n + a];
+    return sweep;
+}
+
+int orc_letkf_transform(int M, int p, const double* dY, const double* rinv, const double* dyo,
+                        double beta, double tol, int max_sweeps, double* W, double* w, double* evals) {
+    // src/util/letkf_solver.h:251-254: C = (M-1)/beta I + dY^T rR dY
+    std::vector<double> C(size_t(M) * M, 0.0), V(size_t(M) * M), d(M), b(M, 0.0);
+    for (int a = 0; a < M; ++a) C[a * M + a] = (M - 1) / beta;
+    for (int o = 0; o < p; ++o) {
+        const double* y = dY + (size_t)o * M;
+        const double r = rinv[o];
+        if (r == 0.0) continue;
+        for (int a = 0; a < M; ++a) {
+            const double ra = r * y[a];
+            b[a] += ra * dyo[o];
+            for (int bb = 0; bb < M; ++bb) C[a * M + bb] += ra * y[bb];
+        }
+    }
+    const int sweeps = orc_jacobi_eig(M, C.data(), V.data(), d.data(), tol, max_sweeps);
+    // letkf_solver.h:288-306: Pa = V D^-1 V^T, w = Pa dY^T rR dyo, W = sqrt(M-1) V D^-1/2 V^T
+    std::vector<double> g(M, 0.0);
+    for (int i = 0; i < M; ++i) {
+        for (int k = 0; k < M; ++k) g[i] += V[k * M + i] * b[k];
+        g[i] /= d[i];
+    }
+    for (int k = 0; k < M; ++k) {
+        double s = 0;
+        for (int i = 0; i < M; ++i) s += V[k * M + i] * g[i];
+        w[k] = s;
+    }
+    const double sq = std::sqrt(double(M - 1));
+    for (int k = 0; k < M; ++k)
+        for (int l = 0; l < M; ++l) {
+            double s = 0;
+            for (int i = 0; i < M; ++i) s += V[k * M + i] * V[l * M + i] / std::sqrt(d[i]);
+            W[k * M + l] = sq * s;
+        }
+    if (evals) {
+        std::vector<double> e(d);
+        std::sort(e.begin(), e.end());
+        std::copy(e.begin(), e.end(), evals);
+    }
+    return sweeps;
+}
+
+int orc_letkf_analysis(const orc_config* cfg, int M, double* f, const double* yo, int dump_g0,
+                       int dump_n, double* W, double* w, double* evals, int g_stride) {
+    const orc_config& c = *cfg;
+    const ObsInfo o = obs_info(c);
+    const long long N = (long long)c.nx * c.ny;
+    const int n_obs = o.n_sites * o.n_vars;
+    if (g_stride < 1) g_stride = 1;
+    std::vector<double> hx(size_t(n_obs) * M), ybar(n_obs);
+    orc_obs_moments(cfg, M, f, hx.data());
+    for (int a = 0; a < n_obs; ++a) {  // letkf_solver.h:239 column mean
+        double s = 0;
+        for (int k = 0; k < M; ++k) s += hx[(size_t)a * M + k];
+        ybar[a] = s / M;
+    }
+    int bad = 0;
+    const int cap = 4096;
+#pragma omp parallel
+    {
+        std::vector<int> sites(cap);
+        std::vector<double> wl(cap), dY, rinv, dyo, Wl(size_t(M) * M), wv(M), ev(M);
+#pragma omp for schedule(dynamic, 64)
+        for (long long g = 0; g < N; g += g_stride) {
+            const int n = std::min(orc_local_obs(cfg, int(g), sites.data(), wl.data(), cap), cap);
+            dY.assign(size_t(n) * o.n_vars * M, 0.0);
+            rinv.assign(size_t(n) * o.n_vars, 0.0);
+            dyo.assign(size_t(n) * o.n_vars, 0.0);
+            int p = 0;
+            for (int e = 0; e < n; ++e)
+                for (int a = 0; a < o.n_vars; ++a, ++p) {
+                    const long long ob = (long long)sites[e] * o.n_vars + a;
+                    for (int k = 0; k < M; ++k) dY[(size_t)p * M + k] = hx[ob * M + k] - ybar[ob];
+                    rinv[p] = wl[e] * o.rinv[a];
+                    dyo[p] = yo[ob] - ybar[ob];
+                }
+            orc_letkf_transform(M, p, dY.data(), rinv.data(), dyo.data(), c.inflation, c.jacobi_tol,
+                                c.jacobi_max_sweeps, Wl.data(), wv.data(), ev.data());
+            if (g >= dump_g0 && g < (long long)dump_g0 + dump_n) {
+                const long long t = g - dump_g0;
+                if (W) std::copy(Wl.begin(), Wl.end(), W + t * M * M);
+                if (w) std::copy(wv.begin(), wv.end(), w + t * M);
+                if (evals) std::copy(ev.begin(), ev.end(), evals + t * M);
+            }
+            // Xa = xbar + dX (W + w 1^T)   (letkf_solver.h:313-316)
+            for (int q = 0; q < Q; ++q) {
+                std::vector<double> x(M);
+                double xbar = 0;
+                for (int k = 0; k < M; ++k) {
+                    x[k] = f[(long long)k * Q * N + q * N + g];
+                    xbar += x[k];
+                }
+                xbar /= M;
+                for (int l = 0; l < M; ++l) {
+                    double s = xbar;
+                    for (int k = 0; k < M; ++k) s += (x[k] - xbar) * (Wl[k * M + l] + wv[k]);
+                    if (!std::isfinite(s)) bad = 1;
+                    f[(long long)l * Q * N + q * N + g] = s;
+                }
+            }
+        }
+    }
+    return bad ? -3 : 0;
+}
+
+void orc_ensemble_stats(const orc_config* cfg, int M, const double* f, const double* trho,
+                        const double* tu, const double* tv, double out[4]) {
+    const orc_config& c = *cfg;
+    const long long N = (long long)c.nx * c.ny;
+    long double serr_v = 0, ssp_v = 0, serr_r = 0, ssp_r = 0;
+    for (long long n = 0; n < N; ++n) {
+        long double su = 0, ssu = 0, sv = 0, ssv = 0, sr = 0, ssr = 0;
+        for (int k = 0; k < M; ++k) {
+            double r, u, v;
+            moments(c, f + (long long)k * Q * N, N, n, r, u, v);
+            su += u; ssu += (long double)u * u;
+            sv += v; ssv += (long double)v * v;
+            sr += r; ssr += (long double)r * r;
+        }
+        auto sd = [&](long double s, long double ss) {
+            const long double m = s / M, var = ss / M - m * m;
+            return var <= 0 ? 0.0L : std::sqrt(var);
+        };
+        const long double mu = su / M, mv = sv / M, mr = sr / M;
+        const long double du = sd(su, ssu), dv = sd(sv, ssv), dr = sd(sr, ssr);
+        serr_v += (mu - tu[n]) * (mu - tu[n]) + (mv - tv[n]) * (mv - tv[n]);
+        ssp_v += du * du + dv * dv;
+        serr_r += (mr - trho[n]) * (mr - trho[n]);
+        ssp_r += dr * dr;
+    }
+    out[0] = double(std::sqrt(serr_v / (2 * N)));
+    out[1] = double(std::sqrt(ssp_v / (2 * N)));
+    out[2] = double(std::sqrt(serr_r / N));
+    out[3] = double(std::sqrt(ssp_r / N));
+}
+
+}  // extern "C"

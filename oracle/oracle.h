@@ -1,0 +1,93 @@
+/*
+ * FP64 CPU oracle of the LBM-LETKF cycle -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain C++/OpenMP restatement of the reference's algorithm, used by the
+ * tests as the parity checker and by bench.py as the CPU baseline.  Product
+ * code never links, loads or calls it.
+ *
+ * Every function cites the reference file:line it restates (paths relative to
+ * the reference repository).  How each function is pinned (golden vector,
+ * closed form, library cross-check, invariant) is listed in tests/ and in the
+ * design notes; the cuRAND MTGP32 forcing stream of the reference is replaced by
+ * Philox4x32-10 and is therefore not pinned to the reference's draws.
+ *
+ * The configuration struct has the field order of lbmletkf_config.
+ */
+#ifndef LBMLETKF_ORACLE_H_
+#define LBMLETKF_ORACLE_H_
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_config {
+    int    nx, ny;
+    double c, dx, dt, nu, rho_ref, cs_smag, u_ref, friction_rate;
+    int    force_kind;
+    double force_kf, force_amp_total, force_sigma_k, force_dk, force_rand_stdev;
+    unsigned long long force_seed;
+    int    ic_kind;
+    double ic_amp, ic_k0, ic_dk, ic_sigma;
+    int    obs_stride, obs_offset, obs_vars;
+    double obs_sigma_u, obs_sigma_rho;
+    int    loc_kind, loc_rloc_len;
+    double loc_scale, loc_cutoff, inflation;
+    int    da_interval, variant;
+    double jacobi_tol;
+    int    jacobi_max_sweeps;
+} orc_config;
+
+int  orc_sizeof_config(void);
+void orc_set_threads(int n);
+
+/* Philox4x32-10 block (Salmon et al. SC'11) and the Box-Muller normals built on it */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+void orc_philox_normal4(uint64_t key, const uint32_t ctr[4], double out[4]);
+
+/* forcing: src/data.cu:227-290 (modes) and :383-414 (field) */
+int  orc_force_modes(const orc_config* cfg, double* kx, double* ky, double* amp, int cap);
+void orc_force_field(const orc_config* cfg, long long step, double* fx, double* fy);
+
+/* D2Q9 pieces: src/lbm.h:31-109, src/lbm.cu:24-160, src/lbm_collision_srt.h:8-16 */
+void orc_feq(const orc_config* cfg, const double* rho, const double* u, const double* v, double* f);
+void orc_macro(const orc_config* cfg, const double* f, double* rho, double* u, double* v);
+void orc_stream(const orc_config* cfg, const double* f, double* fn);
+void orc_les(const orc_config* cfg, const double* u, const double* v, double* nus);
+/* one full step of M members, f[M][9][ny][nx]; step indexes the forcing stream */
+void orc_lbm_step(const orc_config* cfg, int M, const double* fin, double* fout, long long step);
+
+/* initial velocity of one member, src/data.cu:91-186 (direct mode sums) */
+void orc_initial_velocity(const orc_config* cfg, unsigned long long seed, double* u, double* v);
+
+/* observation network and local sets: src/letkf.cu:29-56, 97-113, 150-187 */
+void orc_obs_count(const orc_config* cfg, int* n_sites, int* n_obs);
+int  orc_local_obs(const orc_config* cfg, int gridpoint, int* sites, double* weights, int cap);
+double orc_gaspari_cohn(double d);
+void orc_obs_moments(const orc_config* cfg, int M, const double* f, double* hx);
+void orc_observe(const orc_config* cfg, const double* hx, unsigned long long seed,
+                 unsigned long long counter, int noise, double* yo);
+
+/* symmetric eigen-decomposition by row-cyclic Jacobi; returns sweeps used.
+ * A (n x n, row-major) is destroyed; V receives eigenvectors as columns. */
+int  orc_jacobi_eig(int n, double* A, double* V, double* d, double tol, int max_sweeps);
+
+/* LETKF transform of one local problem, src/util/letkf_solver.h:235-349:
+ * dY [p][M] (rows = observations), rinv [p], dyo [p] -> W [M][M], w [M], evals [M] (ascending) */
+int  orc_letkf_transform(int M, int p, const double* dY, const double* rinv, const double* dyo,
+                         double beta, double tol, int max_sweeps, double* W, double* w, double* evals);
+
+/* full analysis of f[M][9][ny][nx] in place; optional dump of W, w, evals for
+ * gridpoints [dump_g0, dump_g0+dump_n).  g_stride > 1 analyses only every
+ * g_stride-th gridpoint (used to time a bounded sample).  Returns 0, or -3 on NaN. */
+int  orc_letkf_analysis(const orc_config* cfg, int M, double* f, const double* yo,
+                        int dump_g0, int dump_n, double* W, double* w, double* evals, int g_stride);
+
+/* ensemble_stat/main.cpp:108-177 with N = nx*ny: out = {rmse_uv, spread_uv, rmse_rho, spread_rho} */
+void orc_ensemble_stats(const orc_config* cfg, int M, const double* f, const double* trho,
+                        const double* tu, const double* tv, double out[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
