@@ -307,6 +307,7 @@ int setup_obs_impl(Ctx* c) {
     HIP_TRY(c, hipMemcpy(c->d_obs_coef, coef, sizeof(coef), hipMemcpyHostToDevice));
     HIP_TRY(c, dalloc(&c->d_yo, n_obs));
     HIP_TRY(c, dalloc(&c->d_hx, (size_t)n_obs * c->M));
+    HIP_TRY(c, dalloc(&c->d_hx1, n_obs));
     HIP_TRY(c, dalloc(&c->d_dy, (size_t)n_obs * c->M));
     HIP_TRY(c, dalloc(&c->d_dyo, n_obs));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -381,7 +382,7 @@ int prof_collect(Ctx* c) {
 namespace lbmletkf {
 Ctx::~Ctx() {
     void* ptrs[] = {f[0], f[1], plain_ruv, plain_nus, force_xy, d_force, d_step, d_status,
-                    d_site_node, d_loc_ptr, d_loc_site, d_loc_w, d_obs_coef, d_yo, d_hx, d_dy,
+                    d_site_node, d_loc_ptr, d_loc_site, d_loc_w, d_obs_coef, d_yo, d_hx, d_hx1, d_dy,
                     d_dyo, d_gram, d_bvec, d_dump, d_red, d_truth, d_sweeps};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -716,12 +717,12 @@ int lbmletkf_observe(lbmletkf_ctx* ctx, int member, unsigned long long seed,
     if (int st = setup_obs_impl(c)) return st;
     const int n_obs = c->obs.n_sites * c->obs.n_vars;
     // H(x) of the member into d_dyo (free outside an analysis), then noise into the target
-    HIP_TRY(c, launch_obs_moments(*c, c->f[c->cur], c->d_dyo, member, 1));
+    HIP_TRY(c, launch_obs_moments(*c, c->f[c->cur], c->d_hx1, member, 1));
     if (yo_on_device) {
-        HIP_TRY(c, launch_observe(*c, c->d_dyo, yo, seed, counter, noise));
+        HIP_TRY(c, launch_observe(*c, c->d_hx1, yo, seed, counter, noise));
         return LBMLETKF_OK;
     }
-    HIP_TRY(c, launch_observe(*c, c->d_dyo, c->d_yo, seed, counter, noise));
+    HIP_TRY(c, launch_observe(*c, c->d_hx1, c->d_yo, seed, counter, noise));
     if (int st = sync_and_check(c)) return st;
     HIP_TRY(c, hipMemcpy(yo, c->d_yo, sizeof(float) * n_obs, hipMemcpyDeviceToHost));
     return LBMLETKF_OK;

@@ -93,7 +93,8 @@ struct Ctx {
 
     // analysis buffers
     float* d_yo = nullptr;        // [n_obs]
-    float* d_hx = nullptr;        // [n_obs][M] H(x) of every member
+    double* d_hx = nullptr;       // [n_obs][M] H(x) of every member (FP64: rho ~ 1 + O(Ma^2))
+    double* d_hx1 = nullptr;      // [n_obs] H(x) of one member (observe)
     float* d_dy = nullptr;        // [n_obs][M] perturbations dY = Y - ybar
     float* d_dyo = nullptr;       // [n_obs] innovation yo - ybar
     float* d_gram = nullptr;      // plain: [nodes][M][M] C matrices
@@ -137,9 +138,9 @@ bool lbm_fused_supported(int nx, int ny);
 // ---- launchers (letkf.hip) ----------------------------------------------------
 hipError_t launch_local_count(Ctx& c, int* d_count);
 hipError_t launch_local_fill(Ctx& c);
-hipError_t launch_obs_moments(Ctx& c, const float* f, float* hx, int member0, int n_members);
+hipError_t launch_obs_moments(Ctx& c, const float* f, double* hx, int member0, int n_members);
 hipError_t launch_obs_prepare(Ctx& c);
-hipError_t launch_observe(Ctx& c, const float* hx_member, float* yo, uint64_t seed,
+hipError_t launch_observe(Ctx& c, const double* hx_member, float* yo, uint64_t seed,
                           uint64_t counter, int noise);
 hipError_t launch_gram_fp32(Ctx& c);
 hipError_t launch_solve_fp64(Ctx& c, float* f, unsigned long long* sweeps);

@@ -65,22 +65,41 @@ __device__ __forceinline__ float smagorinsky(float uE, float uW, float uN, float
     return p.les_coef * ss;
 }
 
+// Equilibrium populations feq_q(rho,u,v) (src/lbm.h:31-39).  The rest population
+// is taken as rho - sum of the others: in FP32 the weights 4/9, 1/9, 1/36 sum to
+// 1 + 8e-9, and with omega ~ 2 that bias would add ~1.6e-8 rho of mass per node
+// and step; the subtraction keeps sum_q feq = rho up to unbiased rounding.
+__device__ __forceinline__ void equilibrium9(float rho, float u, float v, const LbmParams& p,
+                                             float feq[Q]) {
+    const float uu = (u * u + v * v) * p.inv_c2;
+    float moving = 0.f;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        if (q == 4) continue;
+        const int ex = q % 3 - 1, ey = q / 3 - 1;
+        const float wq = (ex == 0 || ey == 0) ? W1 : W2;
+        const float uc = (u * float(ex) + v * float(ey)) * p.inv_c;
+        feq[q] = rho * wq * (1.f + 3.f * uc + 4.5f * uc * uc - 1.5f * uu);
+        moving += feq[q];
+    }
+    feq[4] = rho - moving;
+}
+
 // BGK relaxation towards feq plus the first-order force term with linear
-// friction (src/lbm_collision_srt.h:8-16, src/lbm.h:31-70).
+// friction (src/lbm_collision_srt.h:8-16, src/lbm.h:41-70).
 __device__ __forceinline__ void collide9(float fc[Q], float rho, float u, float v, float nut,
                                          float Fx, float Fy, const LbmParams& p) {
     const float tau = 0.5f + p.k_tau * (p.nu + nut);
     const float omega = 1.f / tau;
-    const float uu = (u * u + v * v) * p.inv_c2;
     const float fxp = (Fx - p.fric_over_dt * u) * p.dt;
     const float fyp = (Fy - p.fric_over_dt * v) * p.dt;
+    float feq[Q];
+    equilibrium9(rho, u, v, p, feq);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         const int ex = q % 3 - 1, ey = q / 3 - 1;
         const float wq = (ex == 0 && ey == 0) ? W0 : ((ex == 0 || ey == 0) ? W1 : W2);
-        const float uc = (u * float(ex) + v * float(ey)) * p.inv_c;
-        const float feq = rho * wq * (1.f + 3.f * uc + 4.5f * uc * uc - 1.5f * uu);
-        const float fq = fc[q] - omega * (fc[q] - feq);
+        const float fq = fc[q] - omega * (fc[q] - feq[q]);
         fc[q] = fq + wq * rho * p.force_scale * (float(ex) * fxp + float(ey) * fyp);
     }
 }
@@ -277,15 +296,10 @@ __global__ void k_set_feq(float* __restrict__ f, const float* __restrict__ rho,
                           const float* __restrict__ u, const float* __restrict__ v, LbmParams p) {
     const long long node = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (node >= p.nodes) return;
-    const float r = rho[node], uu = u[node], vv = v[node];
-    const float u2 = (uu * uu + vv * vv) * p.inv_c2;
+    float feq[Q];
+    equilibrium9(rho[node], u[node], v[node], p, feq);
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        const int ex = q % 3 - 1, ey = q / 3 - 1;
-        const float wq = (ex == 0 && ey == 0) ? W0 : ((ex == 0 || ey == 0) ? W1 : W2);
-        const float uc = (uu * float(ex) + vv * float(ey)) * p.inv_c;
-        f[q * p.nodes + node] = r * wq * (1.f + 3.f * uc + 4.5f * uc * uc - 1.5f * u2);
-    }
+    for (int q = 0; q < Q; ++q) f[q * p.nodes + node] = feq[q];
 }
 
 // rho, u, v of the stored populations (no streaming), src/letkf.cu:336-357

@@ -113,25 +113,27 @@ __global__ void k_local_fill(LocParams L, const int* __restrict__ ptr, int* __re
     });
 }
 
-// H(x): moments of each member at each observed site (src/letkf.cu:150-187).
-// hx[(s*n_vars + v)*n_members + k]
+// H(x): moments of each member at each observed site (src/letkf.cu:150-187),
+// hx[(s*n_vars + v)*n_members + k].  Evaluated in FP64 from the FP32
+// populations: the density is 1 + O(Ma^2), so its ensemble perturbation would
+// keep only a few bits in FP32.
 __global__ void k_obs_moments(const float* __restrict__ f, const int* __restrict__ site_node,
                               int n_sites, int member0, int n_members, long long member_stride,
-                              long long nodes, float c, int n_vars, int v0, int v1, int v2,
-                              float* __restrict__ hx) {
+                              long long nodes, double c, int n_vars, int v0, int v1, int v2,
+                              double* __restrict__ hx) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long long)n_sites * n_members) return;
     const int s = int(t / n_members), k = int(t % n_members);
     const float* fk = f + (long long)(member0 + k) * member_stride + site_node[s];
-    float fc[Q];
+    double fc[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) fc[q] = fk[q * nodes];
-    float r = 0.f;
+    double r = 0.0;
 #pragma unroll
     for (int q = 0; q < Q; ++q) r += fc[q];
-    const float u = c * ((fc[2] + fc[5] + fc[8]) - (fc[0] + fc[3] + fc[6])) / r;
-    const float v = c * ((fc[6] + fc[7] + fc[8]) - (fc[0] + fc[1] + fc[2])) / r;
-    const float val[3] = {u, v, r};
+    const double u = c * ((fc[2] + fc[5] + fc[8]) - (fc[0] + fc[3] + fc[6])) / r;
+    const double v = c * ((fc[6] + fc[7] + fc[8]) - (fc[0] + fc[1] + fc[2])) / r;
+    const double val[3] = {u, v, r};
     const int vid[3] = {v0, v1, v2};
     for (int a = 0; a < n_vars; ++a) hx[((long long)s * n_vars + a) * n_members + k] = val[vid[a]];
 }
@@ -141,47 +143,42 @@ __device__ inline double wave_sum(double x) {
     for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
     return x;
 }
-__device__ inline float wave_sumf(float x) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
-    return x;
-}
 
 // ybar, dY and the innovation per observation; one wavefront per observation
 // (src/util/letkf_solver.h:239-243).  A non-finite observation sets status bit 1.
-__global__ __launch_bounds__(64) void k_obs_prepare(const float* __restrict__ hx,
+__global__ __launch_bounds__(64) void k_obs_prepare(const double* __restrict__ hx,
                                                     const float* __restrict__ yo, int n_obs, int M,
                                                     float* __restrict__ dy,
                                                     float* __restrict__ dyo,
                                                     int* __restrict__ status) {
     const int o = blockIdx.x;
     const int k = threadIdx.x;
-    const float x = (k < M) ? hx[(long long)o * M + k] : 0.f;
-    const float ybar = wave_sumf(x) / float(M);
-    if (k < M) dy[(long long)o * M + k] = x - ybar;
+    const double x = (k < M) ? hx[(long long)o * M + k] : 0.0;
+    const double ybar = wave_sum(x) / double(M);
+    if (k < M) dy[(long long)o * M + k] = float(x - ybar);
     if (k == 0) {
         const float y = yo[o];
         if (!isfinite(y)) atomicOr(status, 2);
-        dyo[o] = y - ybar;
+        dyo[o] = float(double(y) - ybar);
     }
 }
 
 // y = H(x) + sigma_var * N(0,1), Philox keyed by (seed, counter) -- the OBSERVE
 // step of src/io.cu:160-246 restricted to the observed sites.
-__global__ void k_observe(const float* __restrict__ hx, float* __restrict__ yo, int n_obs,
-                          int n_vars, int v0, int v1, int v2, float sig_u, float sig_rho,
+__global__ void k_observe(const double* __restrict__ hx, float* __restrict__ yo, int n_obs,
+                          int n_vars, int v0, int v1, int v2, double sig_u, double sig_rho,
                           uint64_t seed, uint64_t counter, int noise) {
     const int o = blockIdx.x * blockDim.x + threadIdx.x;
     if (o >= n_obs) return;
-    float y = hx[o];
+    double y = hx[o];
     if (noise) {
         double z[4];
         philox_normal4(seed, uint32_t(counter), uint32_t(counter >> 32), uint32_t(o / 4), STREAM_OBS, z);
         const int vid[3] = {v0, v1, v2};
-        const float sig = vid[o % n_vars] == 2 ? sig_rho : sig_u;
-        y = float(double(y) + double(sig) * z[o % 4]);
+        const double sig = vid[o % n_vars] == 2 ? sig_rho : sig_u;
+        y += sig * z[o % 4];
     }
-    yo[o] = y;
+    yo[o] = float(y);
 }
 
 // ----------------------------------------------------------------------------
@@ -495,12 +492,12 @@ hipError_t launch_local_fill(Ctx& c) {
     return hipGetLastError();
 }
 
-hipError_t launch_obs_moments(Ctx& c, const float* f, float* hx, int member0, int n_members) {
+hipError_t launch_obs_moments(Ctx& c, const float* f, double* hx, int member0, int n_members) {
     const int nt = 256;
     const long long n = (long long)c.obs.n_sites * n_members;
     k_obs_moments<<<int((n + nt - 1) / nt), nt, 0, c.stream>>>(
         f, c.d_site_node, c.obs.n_sites, member0, n_members, c.lp.member_stride, c.nodes,
-        c.lp.c, c.obs.n_vars, c.obs.var_id[0], c.obs.var_id[1], c.obs.var_id[2], hx);
+        c.cfg.c, c.obs.n_vars, c.obs.var_id[0], c.obs.var_id[1], c.obs.var_id[2], hx);
     return hipGetLastError();
 }
 
@@ -511,13 +508,13 @@ hipError_t launch_obs_prepare(Ctx& c) {
     return hipGetLastError();
 }
 
-hipError_t launch_observe(Ctx& c, const float* hx_member, float* yo, uint64_t seed,
+hipError_t launch_observe(Ctx& c, const double* hx_member, float* yo, uint64_t seed,
                           uint64_t counter, int noise) {
     const int n_obs = c.obs.n_sites * c.obs.n_vars;
     const int nt = 256;
     k_observe<<<(n_obs + nt - 1) / nt, nt, 0, c.stream>>>(
         hx_member, yo, n_obs, c.obs.n_vars, c.obs.var_id[0], c.obs.var_id[1], c.obs.var_id[2],
-        float(c.cfg.obs_sigma_u), float(c.cfg.obs_sigma_rho), seed, counter, noise);
+        c.cfg.obs_sigma_u, c.cfg.obs_sigma_rho, seed, counter, noise);
     return hipGetLastError();
 }
 

@@ -112,7 +112,8 @@ lib = _load()
 
 
 def _ptr(a: np.ndarray) -> C.c_void_p:
-    return C.c_void_p(a.ctypes.data)
+    # the pointer object keeps a reference to the array for the duration of the call
+    return a.ctypes.data_as(C.c_void_p)
 
 
 def _check(status: int, ctx_handle=None):

@@ -36,8 +36,20 @@ def orc():
 
 
 def small_cfg(lb, preset="baseline", n=64, **kw):
+    """A preset on an n x n grid.  The baseline's Gaussian localization (L=12,
+    cutoff 43.8 on 256^2) is scaled with the grid when it would not fit."""
     cfg = lb.preset(preset)
     cfg.nx = cfg.ny = n
+    if cfg.loc_kind == lb.LOC_GAUSS and 2 * int(cfg.loc_cutoff) + 1 > n:
+        cfg.loc_scale = 12.0 * n / 256
+        cfg.loc_cutoff = 3.65 * cfg.loc_scale
     for k, v in kw.items():
         setattr(cfg, k, v)
     return cfg
+
+
+def f32_velocity_atol(cfg):
+    """Velocity resolution of FP32 populations: u = c sum(f e)/rho with f ~ 1/9,
+    each f rounded once when feq is evaluated and once when stored, carries an
+    absolute error of up to ~12 half-ulps of 1/9, i.e. ~5e-8 c; 1e-7 c bounds it."""
+    return 1e-7 * cfg.c

@@ -49,7 +49,9 @@ for _n, (_a, _r) in _sig.items():
 
 
 def _p(a):
-    return C.c_void_p(a.ctypes.data) if a is not None else None
+    # data_as() returns a pointer object that holds a reference to the array, so
+    # temporaries such as _p(_f64(x)) stay alive for the duration of the call
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
 
 
 def _c(cfg):

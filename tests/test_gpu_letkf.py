@@ -11,7 +11,7 @@ import math
 import numpy as np
 import pytest
 
-from conftest import small_cfg
+from conftest import f32_velocity_atol, small_cfg
 
 pytestmark = pytest.mark.gpu
 TOL_W = 1e-5
@@ -42,7 +42,8 @@ def test_gc_window_sets_bit_exact(lb, orc, p, rloc):
             s_gpu, w_gpu = ctx.local_obs(g)
             s_orc, w_orc = orc.local_obs(cfg, g)
             np.testing.assert_array_equal(s_gpu, s_orc)
-            np.testing.assert_allclose(w_gpu, w_orc.astype(np.float32), rtol=2e-7, atol=0)
+            # GC(2c) is a cancellation to ~1e-16 in either evaluation order
+            np.testing.assert_allclose(w_gpu, w_orc.astype(np.float32), rtol=2e-7, atol=1e-12)
 
 
 def test_gauss_sets_bit_exact_full_size(lb, orc):
@@ -65,12 +66,14 @@ def test_observe_matches_oracle(lb, orc):
     with ens, nat:
         f = nat.get_all_states()
         hx = orc.obs_moments(cfg, f)[:, 0]
-        np.testing.assert_allclose(nat.observe(0, 5, 1, noise=False), hx, rtol=2e-6, atol=1e-7)
+        # H(x) is evaluated in FP64 from the FP32 populations, as the oracle does
+        np.testing.assert_allclose(nat.observe(0, 5, 1, noise=False), hx, rtol=2e-7, atol=1e-12)
         y = nat.observe(0, 5, 1, noise=True)
-        np.testing.assert_allclose(y, orc.observe(cfg, hx, 5, 1, noise=True), rtol=2e-6, atol=1e-6)
-        # observations taken from reference-layout fields give the same vector
+        np.testing.assert_allclose(y, orc.observe(cfg, hx, 5, 1, noise=True), rtol=2e-7, atol=1e-9)
+        # observations taken from reference-layout fields (FP32 moments) give the same vector
         r, u, v = nat.get_macro(0)
-        np.testing.assert_allclose(nat.obs_from_fields(r, u, v), nat.observe(0, 5, 1, noise=False), rtol=1e-6)
+        np.testing.assert_allclose(nat.obs_from_fields(r, u, v), nat.observe(0, 5, 1, noise=False), rtol=1e-6,
+                                   atol=f32_velocity_atol(cfg))
 
 
 CASES = {
@@ -103,11 +106,15 @@ def test_analysis_matches_oracle(lb, orc, case):
     errw = max(np.abs(w[i] - wo[i]).max() / max(np.abs(wo[i]).max(), 1e-30) for i in range(nd))
     errev = max(np.abs(ev[i] - evo[i]).max() / np.abs(evo[i]).max() for i in range(nd))
     inc = _rel(fa - fb, fo - fb)
-    print(f"  W rel {errW:.2e}  w rel {errw:.2e}  evals rel {errev:.2e}  increment rel {inc:.2e}", flush=True)
+    # the analysed state is stored in FP32: allow 2 ulp of f on top of TOL_INC of the increment
+    ulp = np.spacing(np.abs(fo).astype(np.float32)).astype(np.float64)
+    excess = np.abs(fa - fo) - (2 * ulp + TOL_INC * np.abs(fo - fb).max())
+    print(f"  W rel {errW:.2e}  w rel {errw:.2e}  evals rel {errev:.2e}  increment rel {inc:.2e}  "
+          f"max |fa-fo|/ulp {np.max(np.abs(fa - fo) / ulp):.2f}", flush=True)
     assert errW < TOL_W
     assert errev < TOL_W
-    assert errw < 1e-3
-    assert inc < TOL_INC
+    assert errw < 1e-4
+    assert excess.max() <= 0
     # identities of the transform (hold for any dY with zero row means)
     beta = cfg.inflation
     for i in range(nd):

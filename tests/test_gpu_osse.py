@@ -23,6 +23,8 @@ def _run_osse(lb, orc, cfg, M, cycles, steps, with_oracle=True):
             if with_oracle:
                 for s in range(steps):
                     f_orc = orc.lbm_step(cfg, f_orc, c * steps + s)
+                fb = ens.get_all_states().astype(np.float64)
+                print(f"cycle {c}: forecast rel L2 gpu vs oracle {np.linalg.norm(fb - f_orc) / np.linalg.norm(f_orc):.2e}")
             yo = nat.observe(0, seed=99, counter=c, noise=True)
             ens.analysis(yo)
             tr, tu, tv = nat.get_macro(0)
@@ -30,7 +32,52 @@ def _run_osse(lb, orc, cfg, M, cycles, steps, with_oracle=True):
             if with_oracle:
                 orc.letkf_analysis(cfg, f_orc, yo.astype(np.float64))
                 rm_orc.append(orc.ensemble_stats(cfg, f_orc, tr, tu, tv))
+                fa = ens.get_all_states().astype(np.float64)
+                print(f"cycle {c}: analysis rel L2 gpu vs oracle {np.linalg.norm(fa - f_orc) / np.linalg.norm(f_orc):.2e}"
+                      f"  stats of gpu state by oracle {orc.ensemble_stats(cfg, fa, tr, tu, tv)[:2]}"
+                      f"  numpy {_numpy_stats(cfg, fa, tr, tu, tv)[:2]}  gpu again {ens.ensemble_stats(tr, tu, tv)[:2]}"
+                      f"  truth dtype {tu.dtype} shape {tu.shape}")
     return np.array(rm_gpu), (np.array(rm_orc) if with_oracle else None)
+
+
+def _numpy_stats(cfg, f, tr, tu, tv):
+    """ensemble_stat/main.cpp:108-177 written with numpy (biased spread)."""
+    c = cfg.c
+    ex = np.array([q % 3 - 1 for q in range(9)], float)
+    ey = np.array([q // 3 - 1 for q in range(9)], float)
+    rho = f.sum(1)
+    u = c * np.tensordot(f, ex, axes=([1], [0])) / rho
+    v = c * np.tensordot(f, ey, axes=([1], [0])) / rho
+    n2 = 2 * tu.size
+    rmse = np.sqrt((((u.mean(0) - tu) ** 2) + ((v.mean(0) - tv) ** 2)).sum() / n2)
+    spread = np.sqrt((u.var(0) + v.var(0)).sum() / n2)
+    return np.array([rmse, spread, np.sqrt(((rho.mean(0) - tr) ** 2).mean()), np.sqrt(rho.var(0).mean())])
+
+
+@pytest.mark.parametrize("truth", ["zero", "random", "member"])
+def test_ensemble_stats_matches_numpy(lb, orc, truth):
+    cfg = small_cfg(lb, "baseline", n=64)
+    M = 16
+    with lb.Context(cfg, M) as ens:
+        for k in range(M):
+            ens.init_member(k, 1000 + k)
+        ens.forecast(5)
+        f = ens.get_all_states().astype(np.float64)
+        rng = np.random.default_rng(3)
+        if truth == "zero":
+            tr, tu, tv = np.ones((64, 64)), np.zeros((64, 64)), np.zeros((64, 64))
+        elif truth == "random":
+            tr, tu, tv = (1 + 1e-3 * rng.standard_normal((64, 64)), 0.05 * rng.standard_normal((64, 64)),
+                          0.05 * rng.standard_normal((64, 64)))
+        else:
+            tr, tu, tv = (x.astype(np.float64) for x in ens.get_macro(3))
+        got = ens.ensemble_stats(tr, tu, tv)
+    want = _numpy_stats(cfg, f, tr, tu, tv)
+    orc_out = orc.ensemble_stats(cfg, f, tr, tu, tv)
+    print(f"truth={truth}: gpu {got}  numpy {want}  oracle {orc_out}")
+    np.testing.assert_allclose(orc_out[:2], want[:2], rtol=1e-9)
+    np.testing.assert_allclose(got[:2], want[:2], rtol=1e-5)
+    np.testing.assert_allclose(got[2:], want[2:], rtol=1e-3, atol=1e-9)
 
 
 @pytest.mark.parametrize("variant", ["plain", "mixed"])
@@ -45,9 +92,10 @@ def test_osse_rmse_follows_oracle(lb, orc, variant):
 
 
 def test_assimilation_beats_free_run(lb, orc):
-    cfg = small_cfg(lb, "baseline", n=128, obs_stride=8, loc_scale=12.0, loc_cutoff=40.0)
+    # the benchmark OSSE scaled to 128^2: (u,v) every 8th node, L = 6, cutoff 21.9
+    cfg = small_cfg(lb, "baseline", n=128)
     cfg.variant = lb.VARIANT_MIXED
-    M, cycles, steps = 16, 8, 50
+    M, cycles, steps = 32, 8, 100
     g, _ = _run_osse(lb, orc, cfg, M, cycles, steps, with_oracle=False)
     # free ensemble: same members, no analysis
     ens = lb.Context(cfg, M)

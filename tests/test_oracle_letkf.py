@@ -219,6 +219,16 @@ def test_analysis_without_local_obs_is_inflation(lb, orc):
     np.testing.assert_allclose(fa[:, :, 2, 1], xbar + math.sqrt(cfg.inflation) * (X - xbar), rtol=1e-13)
 
 
+def test_oracle_wrapper_keeps_converted_inputs_alive(lb, orc):
+    # float32 inputs are converted to float64 temporaries inside the wrapper; the
+    # temporaries must outlive the C call (a freed copy once gave wrong truths)
+    cfg, f, truth, _ = _small_osse(lb, orc, n=16, M=4)
+    tr, tu, tv = orc.macro(cfg, truth[0])
+    a = orc.ensemble_stats(cfg, f, tr, tu, tv)
+    b = orc.ensemble_stats(cfg, f, tr.astype(np.float32), tu.astype(np.float32), tv.astype(np.float32))
+    np.testing.assert_allclose(a, b, rtol=1e-6, atol=1e-12)
+
+
 def test_ensemble_stats_definition(lb, orc):
     cfg, f, truth, _ = _small_osse(lb, orc, n=16, M=5)
     rng = np.random.default_rng(9)
