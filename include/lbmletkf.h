@@ -89,6 +89,8 @@ typedef struct lbmletkf_config {
     int    variant;             /* LBMLETKF_VARIANT_*                                       */
     double jacobi_tol;          /* relative off-diagonal tolerance of the eigensolver       */
     int    jacobi_max_sweeps;   /* cusolver syevj uses tol 1e-6, 100 sweeps (eigensolver_wrapper.hpp:192-193) */
+    int    gram_passes;         /* MIXED: 1 = bf16 operands, 3 = split bf16 (hi*hi + hi*lo + lo*hi) */
+    int    jacobi_fp64_sweeps;  /* MIXED: at most this many FP64 sweeps after the FP32 phase */
 } lbmletkf_config;
 
 typedef struct lbmletkf_ctx lbmletkf_ctx;

@@ -214,6 +214,8 @@ int check_config(const lbmletkf_config* cfg, int n_members, std::string& why) {
     if (cfg->variant != LBMLETKF_VARIANT_PLAIN && cfg->variant != LBMLETKF_VARIANT_MIXED)
         return bad("unknown variant");
     if (!(cfg->jacobi_tol > 0) || cfg->jacobi_max_sweeps < 1) return bad("jacobi_tol > 0 and jacobi_max_sweeps >= 1 required");
+    if (cfg->gram_passes != 1 && cfg->gram_passes != 3) return bad("gram_passes must be 1 or 3");
+    if (cfg->jacobi_fp64_sweeps < 0 || cfg->jacobi_fp64_sweeps > 20) return bad("jacobi_fp64_sweeps must be in [0, 20]");
     return LBMLETKF_OK;
 }
 
@@ -383,7 +385,7 @@ namespace lbmletkf {
 Ctx::~Ctx() {
     void* ptrs[] = {f[0], f[1], plain_ruv, plain_nus, force_xy, d_force, d_step, d_status,
                     d_site_node, d_loc_ptr, d_loc_site, d_loc_w, d_obs_coef, d_yo, d_hx, d_hx1, d_dy,
-                    d_dyo, d_gram, d_bvec, d_dump, d_red, d_truth, d_sweeps};
+                    d_dyo, d_gram, d_bvec, d_dump, d_red, d_truth, d_sweeps, d_sweeps64};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (const ProfRec& r : prof_pending) {
@@ -441,6 +443,8 @@ int lbmletkf_config_preset(const char* name, lbmletkf_config* cfg) {
         c.variant = LBMLETKF_VARIANT_PLAIN;
         c.jacobi_tol = 1e-10;
         c.jacobi_max_sweeps = 100;
+        c.gram_passes = 3;
+        c.jacobi_fp64_sweeps = 3;
     } else if (n == "baseline") {
         c.nx = c.ny = 256;
         c.c = 1.0;
@@ -477,6 +481,8 @@ int lbmletkf_config_preset(const char* name, lbmletkf_config* cfg) {
         c.variant = LBMLETKF_VARIANT_MIXED;
         c.jacobi_tol = 1e-10;
         c.jacobi_max_sweeps = 100;
+        c.gram_passes = 3;
+        c.jacobi_fp64_sweeps = 3;
     } else {
         return fail(nullptr, LBMLETKF_ERR_ARG, "unknown preset: " + n);
     }
@@ -752,7 +758,8 @@ int lbmletkf_analysis(lbmletkf_ctx* ctx, const float* yo, int yo_on_device) {
     if (!yo) return fail(c, LBMLETKF_ERR_ARG, "yo is NULL");
     if (c->M < 2) return fail(c, LBMLETKF_ERR_ARG, "the LETKF needs at least two members ((M-1) I is singular for M=1)");
     if (int st = setup_obs_impl(c)) return st;
-    if (int st = ensure_plain_buffers(c)) return st;
+    if (c->cfg.variant == LBMLETKF_VARIANT_PLAIN)
+        if (int st = ensure_plain_buffers(c)) return st;
     const int n_obs = c->obs.n_sites * c->obs.n_vars;
     HIP_TRY(c, hipMemcpyAsync(c->d_yo, yo, sizeof(float) * n_obs,
                               yo_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
@@ -761,12 +768,22 @@ int lbmletkf_analysis(lbmletkf_ctx* ctx, const float* yo, int yo_on_device) {
     HIP_TRY(c, launch_obs_moments(*c, c->f[c->cur], c->d_hx, 0, c->M));
     HIP_TRY(c, launch_obs_prepare(*c));
     prof_end(c, PH_OBS, e0, 2, n_obs);
+    if (c->cfg.variant == LBMLETKF_VARIANT_MIXED) {
+        // fused: gather + MFMA Gram + FP32/FP64 Jacobi + assembly + update in one kernel
+        hipEvent_t e2 = prof_begin(c);
+        HIP_TRY(c, launch_letkf_mixed(*c, c->f[c->cur], c->prof ? c->d_sweeps : nullptr,
+                                      c->prof ? c->d_sweeps64 : nullptr, c->cfg.gram_passes));
+        prof_end(c, PH_SOLVE, e2, 1, 0);
+        c->dump_kind = 1;
+        return LBMLETKF_OK;
+    }
     hipEvent_t e1 = prof_begin(c);
     HIP_TRY(c, launch_gram_fp32(*c));
     prof_end(c, PH_GRAM, e1, 1, c->obs.n_entries * c->obs.n_vars);
     hipEvent_t e2 = prof_begin(c);
     HIP_TRY(c, launch_solve_fp64(*c, c->f[c->cur], c->prof ? c->d_sweeps : nullptr));
     prof_end(c, PH_SOLVE, e2, 1, 0);
+    c->dump_kind = 0;
     return LBMLETKF_OK;
 }
 
@@ -855,6 +872,39 @@ int lbmletkf_get_dump(lbmletkf_ctx* ctx, double* W, double* w, double* evals) {
     HIP_TRY(c, hipMemcpy(h.data(), c->d_dump, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
     for (int g = 0; g < c->dump_n; ++g) {
         const double* src = h.data() + per * g;
+        if (c->dump_kind == 1) {
+            // the fused kernel records V (row-major), d and b; the transform it applies
+            // is W = sqrt(M-1) V D^-1/2 V^T and w = V D^-1 V^T b (letkf_solver.h:288-306)
+            const double* V = src;
+            const double* d = src + size_t(M) * M;
+            const double* b = d + M;
+            std::vector<double> hv(M);
+            for (int i = 0; i < M; ++i) {
+                double s = 0;
+                for (int k = 0; k < M; ++k) s += V[size_t(k) * M + i] * b[k];
+                hv[i] = s / d[i];
+            }
+            const double sq = std::sqrt(double(M - 1));
+            for (int k = 0; k < M; ++k) {
+                if (w) {
+                    double s = 0;
+                    for (int i = 0; i < M; ++i) s += V[size_t(k) * M + i] * hv[i];
+                    w[size_t(M) * g + k] = s;
+                }
+                if (W)
+                    for (int l = 0; l < M; ++l) {
+                        double s = 0;
+                        for (int i = 0; i < M; ++i) s += V[size_t(k) * M + i] * V[size_t(l) * M + i] / std::sqrt(d[i]);
+                        W[size_t(M) * M * g + size_t(k) * M + l] = sq * s;
+                    }
+            }
+            if (evals) {
+                std::vector<double> e(d, d + M);
+                std::sort(e.begin(), e.end());
+                std::copy(e.begin(), e.end(), evals + size_t(M) * g);
+            }
+            continue;
+        }
         if (W) std::copy(src, src + size_t(M) * M, W + size_t(M) * M * g);
         if (w) std::copy(src + size_t(M) * M, src + size_t(M) * M + M, w + size_t(M) * g);
         if (evals) {
@@ -877,7 +927,9 @@ int lbmletkf_profile_enable(lbmletkf_ctx* ctx, int on) {
         c->prof_work[p] = 0;
     }
     if (!c->d_sweeps) HIP_TRY(c, dalloc(&c->d_sweeps, 1));
+    if (!c->d_sweeps64) HIP_TRY(c, dalloc(&c->d_sweeps64, 1));
     HIP_TRY(c, hipMemset(c->d_sweeps, 0, sizeof(unsigned long long)));
+    HIP_TRY(c, hipMemset(c->d_sweeps64, 0, sizeof(unsigned long long)));
     c->prof = on != 0;
     return LBMLETKF_OK;
 }
@@ -891,9 +943,18 @@ int lbmletkf_profile_read(lbmletkf_ctx* ctx, const char* phase, double* seconds,
     int ph = -1;
     for (int p = 0; p < PH_COUNT; ++p)
         if (std::strcmp(phase, names[p]) == 0) ph = p;
-    if (ph < 0) return fail(c, LBMLETKF_ERR_ARG, std::string("unknown phase: ") + phase);
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     if (int st = prof_collect(c)) return st;
+    if (std::strcmp(phase, "solve_fp64") == 0) {
+        // FP64 sweeps of the MIXED variant (its time is part of "solve")
+        unsigned long long sw = 0;
+        if (c->d_sweeps64) HIP_TRY(c, hipMemcpy(&sw, c->d_sweeps64, sizeof(sw), hipMemcpyDeviceToHost));
+        if (seconds) *seconds = 0.0;
+        if (launches) *launches = 0;
+        if (work) *work = (long long)sw;
+        return LBMLETKF_OK;
+    }
+    if (ph < 0) return fail(c, LBMLETKF_ERR_ARG, std::string("unknown phase: ") + phase);
     long long wk = c->prof_work[ph];
     if (ph == PH_SOLVE && c->d_sweeps) {
         unsigned long long sw = 0;

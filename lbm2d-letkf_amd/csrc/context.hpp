@@ -118,7 +118,9 @@ struct Ctx {
     double prof_sec[PH_COUNT] = {};
     long long prof_launch[PH_COUNT] = {};
     long long prof_work[PH_COUNT] = {};
-    unsigned long long* d_sweeps = nullptr;  // Jacobi sweeps summed over gridpoints
+    unsigned long long* d_sweeps = nullptr;    // Jacobi sweeps summed over gridpoints
+    unsigned long long* d_sweeps64 = nullptr;  // MIXED: FP64 sweeps summed over gridpoints
+    int dump_kind = 0;  // 0: W, w, evals (plain); 1: V, d, b (mixed), assembled on the host
 
     ~Ctx();
 };
@@ -144,5 +146,9 @@ hipError_t launch_observe(Ctx& c, const double* hx_member, float* yo, uint64_t s
                           uint64_t counter, int noise);
 hipError_t launch_gram_fp32(Ctx& c);
 hipError_t launch_solve_fp64(Ctx& c, float* f, unsigned long long* sweeps);
+
+// ---- launchers (letkf_mixed.hip) ----------------------------------------------
+hipError_t launch_letkf_mixed(Ctx& c, float* f, unsigned long long* sweeps32,
+                              unsigned long long* sweeps64, int passes);
 
 }  // namespace lbmletkf

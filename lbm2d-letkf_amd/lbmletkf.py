@@ -47,6 +47,7 @@ class Config(C.Structure):
         ("loc_scale", C.c_double), ("loc_cutoff", C.c_double), ("inflation", C.c_double),
         ("da_interval", C.c_int), ("variant", C.c_int),
         ("jacobi_tol", C.c_double), ("jacobi_max_sweeps", C.c_int),
+        ("gram_passes", C.c_int), ("jacobi_fp64_sweeps", C.c_int),
     ]
 
     def copy(self) -> "Config":

@@ -36,6 +36,7 @@ typedef struct orc_config {
     int    da_interval, variant;
     double jacobi_tol;
     int    jacobi_max_sweeps;
+    int    gram_passes, jacobi_fp64_sweeps;  /* GPU precision knobs; unused by the oracle */
 } orc_config;
 
 int  orc_sizeof_config(void);
