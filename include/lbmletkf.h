@@ -172,7 +172,9 @@ int lbmletkf_obs_from_fields(lbmletkf_ctx* ctx, const float* rho, const float* u
 int lbmletkf_analysis(lbmletkf_ctx* ctx, const float* yo, int yo_on_device);
 
 /* One full DA cycle: analysis(yo) then forecast(cfg.da_interval) -- the order of
- * src/main.cu:95-113.  In the MIXED variant the cycle is replayed as one hipGraph. */
+ * src/main.cu:95-113.  yo is a device pointer.  In the MIXED variant, with phase
+ * timing off, the kernels of the cycle are captured once into a hipGraph (per parity
+ * of the population double buffer) and replayed; set_dump drops the graphs. */
 int lbmletkf_cycle(lbmletkf_ctx* ctx, const float* yo_device);
 
 /* Ensemble statistics against a truth (ensemble_stat/main.cpp:108-177, with the
@@ -197,11 +199,16 @@ int lbmletkf_get_dump(lbmletkf_ctx* ctx, double* W, double* w, double* evals);
  * counterpart of the reference's util::timer tags (src/util/timer.hpp:17-85,
  * src/main.cu:95-115, src/letkf.cu:217-403).  enable(1) clears and starts them.
  * Phases: "forecast" (all LBM kernels incl. forcing), "obs" (H(x), innovations),
- * "gram" (C and b), "solve" (eigensolve, transform, state update).
+ * "gram" (PLAIN: C and b), "solve" (PLAIN: eigensolve, transform, state update;
+ * MIXED: gather, MFMA Gram and the FP32 Jacobi sweeps), "finish" (MIXED: FP64
+ * sweeps, transform, state update).
  * read() returns the accumulated seconds, the number of kernel launches and a
  * work count: node updates (forecast), observations (obs), local scalar
  * observations summed over gridpoints (gram), Jacobi sweeps summed over
- * gridpoints (solve).  read() synchronises the stream. */
+ * gridpoints (solve; FP32 sweeps in MIXED).  The pseudo-phase "solve_fp64" reads
+ * the MIXED variant's FP64 sweeps summed over gridpoints as its work count.
+ * read() synchronises the stream.  While timing is on, lbmletkf_cycle launches
+ * eagerly instead of replaying its graph. */
 int lbmletkf_profile_enable(lbmletkf_ctx* ctx, int on);
 int lbmletkf_profile_read(lbmletkf_ctx* ctx, const char* phase, double* seconds,
                           long long* launches, long long* work);

@@ -324,6 +324,16 @@ int ensure_plain_buffers(Ctx* c) {
     return LBMLETKF_OK;
 }
 
+// the MIXED variant's workspace between its two kernels (allocated outside any capture)
+int ensure_mixed_buffers(Ctx* c) {
+    const int N = mixed_width(c->M);
+    if (c->d_mix_u && c->mix_width == N) return LBMLETKF_OK;
+    HIP_TRY(c, dalloc(&c->d_mix_u, (size_t)c->nodes * N * N));
+    HIP_TRY(c, dalloc(&c->d_mix_b, (size_t)c->nodes * N));
+    c->mix_width = N;
+    return LBMLETKF_OK;
+}
+
 int ensure_scratch(Ctx* c) {
     if (c->d_truth) return LBMLETKF_OK;
     HIP_TRY(c, dalloc(&c->d_truth, 3 * c->nodes));
@@ -374,6 +384,101 @@ int prof_collect(Ctx* c) {
     return LBMLETKF_OK;
 }
 
+// n LBM steps of all members (src/lbm.h:16-29 applied n times): stream-ordered
+// launches only, so a cycle can be captured into a graph.  Advances cur and step.
+int forecast_launches(Ctx* c, int n_steps) {
+    const bool fused = c->cfg.variant == LBMLETKF_VARIANT_MIXED && lbm_fused_supported(c->nx, c->ny);
+    hipEvent_t ev = prof_begin(c);
+    for (int s = 0; s < n_steps; ++s) {
+        if (c->lp.has_force) HIP_TRY(c, launch_force_field(*c, s));
+        const float* fin = c->f[c->cur];
+        float* fout = c->f[1 - c->cur];
+        if (fused) {
+            HIP_TRY(c, launch_lbm_fused(*c, fin, fout));
+        } else {
+            for (int m = 0; m < c->M; ++m) HIP_TRY(c, launch_lbm_plain_member(*c, fin, fout, m));
+        }
+        c->cur = 1 - c->cur;
+    }
+    if (n_steps > 0) HIP_TRY(c, launch_step_advance(*c, n_steps));
+    const long long per_step = (fused ? 1 : 3LL * c->M) + (c->lp.has_force ? 1 : 0);
+    prof_end(c, PH_FORECAST, ev, per_step * n_steps + (n_steps > 0), (long long)n_steps * c->M * c->nodes);
+    c->step += n_steps;
+    return LBMLETKF_OK;
+}
+
+// The analysis kernels on the observation vector already in d_yo (the observation
+// network must be set up): stream-ordered launches only, apart from the one-time
+// allocation of the PLAIN variant's per-gridpoint matrices (that variant is never
+// captured into a graph).
+int analysis_launches(Ctx* c) {
+    if (c->cfg.variant == LBMLETKF_VARIANT_PLAIN)
+        if (int st = ensure_plain_buffers(c)) return st;
+    const int n_obs = c->obs.n_sites * c->obs.n_vars;
+    hipEvent_t e0 = prof_begin(c);
+    HIP_TRY(c, launch_obs_moments(*c, c->f[c->cur], c->d_hx, 0, c->M));
+    HIP_TRY(c, launch_obs_prepare(*c));
+    prof_end(c, PH_OBS, e0, 2, n_obs);
+    if (c->cfg.variant == LBMLETKF_VARIANT_MIXED) {
+        // k_mixed_solve (gather + MFMA Gram + FP32 sweeps), then k_mixed_finish
+        // (FP64 sweeps + assembly + update); the workspace is allocated by the caller
+        hipEvent_t e2 = prof_begin(c);
+        hipEvent_t mid_end = c->prof ? prof_event(c) : nullptr;
+        hipEvent_t mid_begin = c->prof ? prof_event(c) : nullptr;
+        HIP_TRY(c, launch_letkf_mixed(*c, c->f[c->cur], c->prof ? c->d_sweeps : nullptr,
+                                      c->prof ? c->d_sweeps64 : nullptr, c->cfg.gram_passes, mid_end,
+                                      mid_begin));
+        if (c->prof && e2 && mid_end && mid_begin) {
+            c->prof_pending.push_back({PH_SOLVE, e2, mid_end, 1, 0});
+            prof_end(c, PH_FINISH, mid_begin, 1, 0);
+        }
+        c->dump_kind = 1;
+        return LBMLETKF_OK;
+    }
+    hipEvent_t e1 = prof_begin(c);
+    HIP_TRY(c, launch_gram_fp32(*c));
+    prof_end(c, PH_GRAM, e1, 1, c->obs.n_entries * c->obs.n_vars);
+    hipEvent_t e2 = prof_begin(c);
+    HIP_TRY(c, launch_solve_fp64(*c, c->f[c->cur], c->prof ? c->d_sweeps : nullptr));
+    prof_end(c, PH_SOLVE, e2, 1, 0);
+    c->dump_kind = 0;
+    return LBMLETKF_OK;
+}
+
+// One MIXED cycle (analysis on d_yo, then da_interval steps) replayed from a graph.
+// The graph is captured on first use for the current parity of `cur` and rebuilt
+// after anything that changes a captured argument (dump range, observation buffers).
+// The captured work contains kernels only: no memset or memcpy nodes.
+int cycle_graph(Ctx* c) {
+    const int k = c->cur;
+    if (!c->cycle_exec[k]) {
+        const int cur0 = c->cur;
+        const long long step0 = c->step;
+        HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int st = analysis_launches(c);
+        if (st == LBMLETKF_OK) st = forecast_launches(c, c->cfg.da_interval);
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(c->stream, &g);
+        c->cur = cur0;
+        c->step = step0;
+        if (st != LBMLETKF_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return st;
+        }
+        HIP_TRY(c, ec);
+        hipGraphExec_t x = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIP_TRY(c, ei);
+        c->cycle_exec[k] = x;
+    }
+    HIP_TRY(c, hipGraphLaunch(c->cycle_exec[k], c->stream));
+    if (c->cfg.da_interval & 1) c->cur = 1 - c->cur;
+    c->step += c->cfg.da_interval;
+    c->dump_kind = 1;
+    return LBMLETKF_OK;
+}
+
 #define CHECK_CTX(c) \
     do { if (!(c)) return fail(nullptr, LBMLETKF_ERR_ARG, "context is NULL"); } while (0)
 #define CHECK_MEMBER(c, m) \
@@ -382,10 +487,20 @@ int prof_collect(Ctx* c) {
 }  // namespace
 
 namespace lbmletkf {
+void Ctx::drop_graphs() {
+    if ((cycle_exec[0] || cycle_exec[1]) && stream) (void)hipStreamSynchronize(stream);
+    for (hipGraphExec_t& x : cycle_exec) {
+        if (x) (void)hipGraphExecDestroy(x);
+        x = nullptr;
+    }
+}
+
 Ctx::~Ctx() {
+    drop_graphs();
     void* ptrs[] = {f[0], f[1], plain_ruv, plain_nus, force_xy, d_force, d_step, d_status,
                     d_site_node, d_loc_ptr, d_loc_site, d_loc_w, d_obs_coef, d_yo, d_hx, d_hx1, d_dy,
-                    d_dyo, d_gram, d_bvec, d_dump, d_red, d_truth, d_sweeps, d_sweeps64};
+                    d_dyo, d_gram, d_bvec, d_mix_u, d_mix_b, d_dump, d_red, d_truth, d_sweeps,
+                    d_sweeps64};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (const ProfRec& r : prof_pending) {
@@ -444,7 +559,7 @@ int lbmletkf_config_preset(const char* name, lbmletkf_config* cfg) {
         c.jacobi_tol = 1e-10;
         c.jacobi_max_sweeps = 100;
         c.gram_passes = 3;
-        c.jacobi_fp64_sweeps = 3;
+        c.jacobi_fp64_sweeps = 1;
     } else if (n == "baseline") {
         c.nx = c.ny = 256;
         c.c = 1.0;
@@ -482,7 +597,7 @@ int lbmletkf_config_preset(const char* name, lbmletkf_config* cfg) {
         c.jacobi_tol = 1e-10;
         c.jacobi_max_sweeps = 100;
         c.gram_passes = 3;
-        c.jacobi_fp64_sweeps = 3;
+        c.jacobi_fp64_sweeps = 1;
     } else {
         return fail(nullptr, LBMLETKF_ERR_ARG, "unknown preset: " + n);
     }
@@ -662,24 +777,7 @@ int lbmletkf_forecast(lbmletkf_ctx* ctx, int n_steps) {
     Ctx* c = reinterpret_cast<Ctx*>(ctx);
     CHECK_CTX(c);
     if (n_steps < 0) return fail(c, LBMLETKF_ERR_ARG, "n_steps must be >= 0");
-    const bool fused = c->cfg.variant == LBMLETKF_VARIANT_MIXED && lbm_fused_supported(c->nx, c->ny);
-    hipEvent_t ev = prof_begin(c);
-    for (int s = 0; s < n_steps; ++s) {
-        if (c->lp.has_force) HIP_TRY(c, launch_force_field(*c, s));
-        const float* fin = c->f[c->cur];
-        float* fout = c->f[1 - c->cur];
-        if (fused) {
-            HIP_TRY(c, launch_lbm_fused(*c, fin, fout));
-        } else {
-            for (int m = 0; m < c->M; ++m) HIP_TRY(c, launch_lbm_plain_member(*c, fin, fout, m));
-        }
-        c->cur = 1 - c->cur;
-    }
-    if (n_steps > 0) HIP_TRY(c, launch_step_advance(*c, n_steps));
-    const long long per_step = (fused ? 1 : 3LL * c->M) + (c->lp.has_force ? 1 : 0);
-    prof_end(c, PH_FORECAST, ev, per_step * n_steps + (n_steps > 0), (long long)n_steps * c->M * c->nodes);
-    c->step += n_steps;
-    return LBMLETKF_OK;
+    return forecast_launches(c, n_steps);
 }
 
 int lbmletkf_set_step(lbmletkf_ctx* ctx, long long step) {
@@ -758,39 +856,29 @@ int lbmletkf_analysis(lbmletkf_ctx* ctx, const float* yo, int yo_on_device) {
     if (!yo) return fail(c, LBMLETKF_ERR_ARG, "yo is NULL");
     if (c->M < 2) return fail(c, LBMLETKF_ERR_ARG, "the LETKF needs at least two members ((M-1) I is singular for M=1)");
     if (int st = setup_obs_impl(c)) return st;
-    if (c->cfg.variant == LBMLETKF_VARIANT_PLAIN)
-        if (int st = ensure_plain_buffers(c)) return st;
+    if (c->cfg.variant == LBMLETKF_VARIANT_MIXED)
+        if (int st = ensure_mixed_buffers(c)) return st;
     const int n_obs = c->obs.n_sites * c->obs.n_vars;
     HIP_TRY(c, hipMemcpyAsync(c->d_yo, yo, sizeof(float) * n_obs,
                               yo_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
                               c->stream));
-    hipEvent_t e0 = prof_begin(c);
-    HIP_TRY(c, launch_obs_moments(*c, c->f[c->cur], c->d_hx, 0, c->M));
-    HIP_TRY(c, launch_obs_prepare(*c));
-    prof_end(c, PH_OBS, e0, 2, n_obs);
-    if (c->cfg.variant == LBMLETKF_VARIANT_MIXED) {
-        // fused: gather + MFMA Gram + FP32/FP64 Jacobi + assembly + update in one kernel
-        hipEvent_t e2 = prof_begin(c);
-        HIP_TRY(c, launch_letkf_mixed(*c, c->f[c->cur], c->prof ? c->d_sweeps : nullptr,
-                                      c->prof ? c->d_sweeps64 : nullptr, c->cfg.gram_passes));
-        prof_end(c, PH_SOLVE, e2, 1, 0);
-        c->dump_kind = 1;
-        return LBMLETKF_OK;
-    }
-    hipEvent_t e1 = prof_begin(c);
-    HIP_TRY(c, launch_gram_fp32(*c));
-    prof_end(c, PH_GRAM, e1, 1, c->obs.n_entries * c->obs.n_vars);
-    hipEvent_t e2 = prof_begin(c);
-    HIP_TRY(c, launch_solve_fp64(*c, c->f[c->cur], c->prof ? c->d_sweeps : nullptr));
-    prof_end(c, PH_SOLVE, e2, 1, 0);
-    c->dump_kind = 0;
-    return LBMLETKF_OK;
+    return analysis_launches(c);
 }
 
 int lbmletkf_cycle(lbmletkf_ctx* ctx, const float* yo_device) {
-    if (int st = lbmletkf_analysis(ctx, yo_device, 1)) return st;
     Ctx* c = reinterpret_cast<Ctx*>(ctx);
-    return lbmletkf_forecast(ctx, c->cfg.da_interval);
+    CHECK_CTX(c);
+    if (!yo_device) return fail(c, LBMLETKF_ERR_ARG, "yo is NULL");
+    if (c->M < 2) return fail(c, LBMLETKF_ERR_ARG, "the LETKF needs at least two members ((M-1) I is singular for M=1)");
+    if (int st = setup_obs_impl(c)) return st;
+    if (c->cfg.variant == LBMLETKF_VARIANT_MIXED)
+        if (int st = ensure_mixed_buffers(c)) return st;
+    const int n_obs = c->obs.n_sites * c->obs.n_vars;
+    HIP_TRY(c, hipMemcpyAsync(c->d_yo, yo_device, sizeof(float) * n_obs, hipMemcpyDeviceToDevice, c->stream));
+    // timed phases need their events between the kernels: run eagerly then
+    if (c->cfg.variant == LBMLETKF_VARIANT_MIXED && !c->prof) return cycle_graph(c);
+    if (int st = analysis_launches(c)) return st;
+    return forecast_launches(c, c->cfg.da_interval);
 }
 
 int lbmletkf_ensemble_stats(lbmletkf_ctx* ctx, const float* truth_rho, const float* truth_u,
@@ -847,6 +935,7 @@ int lbmletkf_set_dump(lbmletkf_ctx* ctx, int g0, int count) {
     CHECK_CTX(c);
     if (count < 0 || g0 < 0 || (long long)g0 + count > c->nodes) return fail(c, LBMLETKF_ERR_ARG, "dump range out of bounds");
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->drop_graphs();  // the captured analysis names the dump buffer and range
     if (c->d_dump) {
         HIP_TRY(c, hipFree(c->d_dump));
         c->d_dump = nullptr;
@@ -939,7 +1028,7 @@ int lbmletkf_profile_read(lbmletkf_ctx* ctx, const char* phase, double* seconds,
     Ctx* c = reinterpret_cast<Ctx*>(ctx);
     CHECK_CTX(c);
     if (!phase) return fail(c, LBMLETKF_ERR_ARG, "phase is NULL");
-    static const char* names[PH_COUNT] = {"forecast", "obs", "gram", "solve"};
+    static const char* names[PH_COUNT] = {"forecast", "obs", "gram", "solve", "finish"};
     int ph = -1;
     for (int p = 0; p < PH_COUNT; ++p)
         if (std::strcmp(phase, names[p]) == 0) ph = p;

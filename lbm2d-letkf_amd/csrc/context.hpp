@@ -47,7 +47,8 @@ struct ObsNet {
     long long n_entries = 0;            // total local (site, weight) pairs
 };
 
-enum Phase { PH_FORECAST = 0, PH_OBS, PH_GRAM, PH_SOLVE, PH_COUNT };
+// MIXED: PH_SOLVE = gather + Gram + FP32 sweeps, PH_FINISH = FP64 sweeps + assembly + update
+enum Phase { PH_FORECAST = 0, PH_OBS, PH_GRAM, PH_SOLVE, PH_FINISH, PH_COUNT };
 
 struct ProfRec {
     int phase;
@@ -99,6 +100,10 @@ struct Ctx {
     float* d_dyo = nullptr;       // [n_obs] innovation yo - ybar
     float* d_gram = nullptr;      // plain: [nodes][M][M] C matrices
     float* d_bvec = nullptr;      // plain: [nodes][M] dY^T R^-1 dyo
+    // MIXED workspace between its two kernels (N = mix_width, M padded to 16/32/64)
+    float* d_mix_u = nullptr;     // [nodes][N][N] FP32 columns of U = C V after the FP32 sweeps
+    float* d_mix_b = nullptr;     // [nodes][N] b = dY^T R^-1 dyo
+    int mix_width = 0;
 
     // transform dump
     int dump_g0 = 0, dump_n = 0;
@@ -121,6 +126,11 @@ struct Ctx {
     unsigned long long* d_sweeps = nullptr;    // Jacobi sweeps summed over gridpoints
     unsigned long long* d_sweeps64 = nullptr;  // MIXED: FP64 sweeps summed over gridpoints
     int dump_kind = 0;  // 0: W, w, evals (plain); 1: V, d, b (mixed), assembled on the host
+
+    // MIXED cycle (analysis + da_interval steps) as an instantiated hipGraph, one per
+    // parity of `cur` (the captured kernels name the current population buffer)
+    hipGraphExec_t cycle_exec[2] = {nullptr, nullptr};
+    void drop_graphs();
 
     ~Ctx();
 };
@@ -148,7 +158,10 @@ hipError_t launch_gram_fp32(Ctx& c);
 hipError_t launch_solve_fp64(Ctx& c, float* f, unsigned long long* sweeps);
 
 // ---- launchers (letkf_mixed.hip) ----------------------------------------------
+int mixed_width(int M);
+// ev_mid_end / ev_mid_begin (may be null) are recorded between the two kernels
 hipError_t launch_letkf_mixed(Ctx& c, float* f, unsigned long long* sweeps32,
-                              unsigned long long* sweeps64, int passes);
+                              unsigned long long* sweeps64, int passes, hipEvent_t ev_mid_end,
+                              hipEvent_t ev_mid_begin);
 
 }  // namespace lbmletkf

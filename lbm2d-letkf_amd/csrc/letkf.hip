@@ -519,6 +519,7 @@ hipError_t launch_observe(Ctx& c, const double* hx_member, float* yo, uint64_t s
 }
 
 hipError_t launch_gram_fp32(Ctx& c) {
+    if (!c.d_gram || !c.d_bvec || !c.d_loc_ptr || !c.d_dy || !c.d_dyo) return hipErrorInvalidValue;
     const float diag = float(double(c.M - 1) / c.cfg.inflation);
     k_gram_fp32<<<int(c.nodes), GRAM_NT, 0, c.stream>>>(c.d_loc_ptr, c.d_loc_site, c.d_loc_w,
                                                         c.d_dy, c.d_dyo, c.d_obs_coef,
@@ -528,6 +529,7 @@ hipError_t launch_gram_fp32(Ctx& c) {
 }
 
 hipError_t launch_solve_fp64(Ctx& c, float* f, unsigned long long* sweeps) {
+    if (!c.d_gram || !c.d_bvec || !f || (c.dump_n > 0 && !c.d_dump)) return hipErrorInvalidValue;
     SolveParams sp;
     sp.sweeps = sweeps;
     sp.M = c.M;

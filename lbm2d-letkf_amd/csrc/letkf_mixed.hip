@@ -1,21 +1,31 @@
-// Mixed-precision fused LETKF analysis for gfx950: one wavefront per gridpoint
-// does the whole local analysis of src/util/letkf_solver.h:235-349 without a
-// round trip through HBM:
+// Mixed-precision LETKF analysis for gfx950: the local analysis of
+// src/util/letkf_solver.h:235-349 at every gridpoint, one wavefront per gridpoint,
+// in two kernels:
 //
+// k_mixed_solve (FP32, ~150 VGPRs, 3 waves per SIMD)
 //  1. gather: the rows of dY (and the innovation) of the gridpoint's local
 //     observations, scaled by sqrt(R^-1 o localization), straight from L2;
 //  2. Gram on the matrix cores: C = (M-1)/beta I + G^T G and b = G^T (sqrt(Rl^-1) dyo)
 //     with v_mfma_f32_16x16x32_bf16 (FP32 accumulation); PASSES = 3 splits each
 //     operand into bf16 hi + lo parts (hi*hi + hi*lo + lo*hi);
-//  3. cyclic Jacobi in FP32: A in LDS (16.6 KB at N = 64), the eigenvector
-//     matrix V in registers with one column per lane, so a rotation of the
-//     column pair (p, p^m) is a lane-pair exchange (__shfl_xor by m);
-//  4. Jacobi sweeps in FP64 on the converged matrix (packed upper triangle in
-//     the same LDS, V widened to FP64 in registers);
+//  3. cyclic one-sided (Hestenes) Jacobi in FP32 on U = C, register resident:
+//     lane j holds column j, and the rotation of the column pair (p, p^m) of
+//     round m needs only the partner column (__shfl_xor by m) and one FMA per
+//     element.  No LDS and no barrier inside a sweep.  As C is symmetric positive
+//     definite, U -> C V = V D, so the eigenpairs are d_j = ||u_j||, v_j = u_j / d_j
+//     and V is never accumulated.  U and b go to a workspace in HBM.
+//
+// k_mixed_finish (FP64, one wave per SIMD when it sweeps)
+//  4. the same sweeps in FP64 on the widened columns (partner column held in
+//     registers) until the pairwise cosines fall below the eigensolver tolerance
+//     or the sweep cap is reached;
 //  5. inverse-square-root assembly in FP64 without forming W:
 //       Xa = xbar + dX (W + w 1^T) = xbar + sqrt(M-1) (dX V) D^-1/2 V^T + (dX w) 1^T,
 //       w = V D^-1 V^T b,
 //     with V transposed through LDS a quarter of its columns at a time.
+//
+// The workspace round trip (4 N^2 bytes per gridpoint each way) costs ~0.3 ms at
+// 256^2 x 64 and lets each kernel have the register budget of its own phase.
 #include "context.hpp"
 
 namespace lbmletkf {
@@ -31,8 +41,8 @@ struct MixedParams {
     float diag;             // (M-1)/beta
     float coef[3];          // R^-1 per observed variable
     double sqm1;            // sqrt(M-1)
-    float tol32_2;          // FP32 phase: stop when off(A)^2 <= tol32^2 ||A||^2
-    double tol64_2;         // FP64 phase
+    float tol32;            // FP32 phase: stop after a sweep whose largest pair cosine is below
+    double tol64;           // FP64 phase: the same test, then at most max64 sweeps
     int max32, max64;
     int dump_g0, dump_n;
     unsigned long long* sweeps32;
@@ -45,155 +55,94 @@ __device__ inline double wave_sum_d(double x) {
     return x;
 }
 
-// pair k of round m (XOR schedule): p has bit hb(m) clear, q = p ^ m
-__device__ inline void xpair(int k, int m, int& p, int& q) {
-    const int hb = 31 - __clz(m);
-    p = ((k >> hb) << (hb + 1)) | (k & ((1 << hb) - 1));
-    q = p ^ m;
-}
-// index of the pair whose low member is p (bit hb(m) of p clear) in round m
-__device__ inline int pair_of(int p, int m) {
-    const int c = p;
-    const int hb = 31 - __clz(m);
-    return ((c >> (hb + 1)) << hb) | (c & ((1 << hb) - 1));
-}
-
-// FP32 symmetric matrix stored full (row stride N+1), writes mirrored
-template <int N>
-struct FullF32 {
-    float* a;
-    __device__ float ld(int i, int j) const { return a[i * (N + 1) + j]; }
-    __device__ void st_pair(int i, int j, float x) const {
-        a[i * (N + 1) + j] = x;
-        a[j * (N + 1) + i] = x;
-    }
-    __device__ void st(int i, int j, float x) const { a[i * (N + 1) + j] = x; }
-};
-
-// FP64 symmetric matrix stored as packed upper triangle
-template <int N>
-struct TriF64 {
-    double* a;
-    __device__ static int idx(int i, int j) {
-        const int lo = min(i, j), hi = max(i, j);
-        return lo * N - (lo * (lo - 1)) / 2 + (hi - lo);
-    }
-    __device__ double ld(int i, int j) const { return a[idx(i, j)]; }
-    __device__ void st_pair(int i, int j, double x) const { a[idx(i, j)] = x; }
-    __device__ void st(int i, int j, double x) const { a[idx(i, j)] = x; }
-};
-
-template <int N>
-struct Blocks {
-    static constexpr int NP = N / 2;
-    static constexpr int NBLK = NP * (NP + 1) / 2;
-    static constexpr int PER_LANE = (NBLK + 63) / 64;
-};
-
-// One sweep of N-1 XOR rounds.  v[] is the lane's column of V.
-template <int N, class T, class Store>
-__device__ void jacobi_sweep(const Store& A, T* rc, T* rs, T* rt, T (&v)[N], const int (&bI)[Blocks<N>::PER_LANE],
-                             const int (&bJ)[Blocks<N>::PER_LANE], int lane) {
-    constexpr int NP = N / 2;
-    for (int m = 1; m < N; ++m) {
-        if (lane < NP) {
-            int p, q;
-            xpair(lane, m, p, q);
-            const T app = A.ld(p, p), aqq = A.ld(q, q), apq = A.ld(p, q);
-            T c = T(1), s = T(0), t = T(0);
-            if (apq != T(0)) {
-                const T theta = (aqq - app) / (T(2) * apq);
-                t = (theta >= T(0) ? T(1) : T(-1)) / (fabs(theta) + sqrt(theta * theta + T(1)));
-                c = T(1) / sqrt(t * t + T(1));
-                s = t * c;
-            }
-            rc[lane] = c;
-            rs[lane] = s;
-            rt[lane] = t;
-        }
-        __syncthreads();
-        // two-sided update of the 2x2 blocks (I <= J) owned by this lane
+template <class T>
+__device__ inline T wave_max(T x) {
 #pragma unroll
-        for (int u = 0; u < Blocks<N>::PER_LANE; ++u) {
-            const int I = bI[u], J = bJ[u];
-            if (I < 0) continue;
-            int p, q;
-            xpair(I, m, p, q);
-            if (I == J) {
-                const T apq = A.ld(p, q), t = rt[I];
-                A.st(p, p, A.ld(p, p) - t * apq);
-                A.st(q, q, A.ld(q, q) + t * apq);
-                A.st_pair(p, q, T(0));
-            } else {
-                int r, s2;
-                xpair(J, m, r, s2);
-                const T cI = rc[I], sI = rs[I], cJ = rc[J], sJ = rs[J];
-                const T b11 = A.ld(p, r), b12 = A.ld(p, s2), b21 = A.ld(q, r), b22 = A.ld(q, s2);
-                const T x11 = cI * b11 - sI * b21, x12 = cI * b12 - sI * b22;
-                const T x21 = sI * b11 + cI * b21, x22 = sI * b12 + cI * b22;
-                A.st_pair(p, r, cJ * x11 - sJ * x12);
-                A.st_pair(p, s2, sJ * x11 + cJ * x12);
-                A.st_pair(q, r, cJ * x21 - sJ * x22);
-                A.st_pair(q, s2, sJ * x21 + cJ * x22);
-            }
-        }
-        // V <- V J: column `lane` mixes with column lane^m (both below N, as m < N)
-        if (lane < N) {
-            const bool low = (lane & (1 << (31 - __clz(m)))) == 0;
-            // the pair is indexed by its low member p; the high member is p ^ m,
-            // which also differs from p in the bits of m below its top bit
-            const int k = pair_of(low ? lane : (lane ^ m), m);
-            const T c = rc[k], s = low ? -rs[k] : rs[k];
-#pragma unroll
-            for (int i = 0; i < N; ++i) {
-                const T other = __shfl_xor(v[i], m);
-                v[i] = c * v[i] + s * other;
-            }
-        }
-        __syncthreads();
-    }
+    for (int off = 32; off > 0; off >>= 1) x = fmax(x, __shfl_xor(x, off));
+    return x;
 }
 
-// off-diagonal and total squared Frobenius norms
-template <int N, class T, class Store>
-__device__ void frob(const Store& A, int lane, double& off, double& tot) {
-    off = 0.0;
-    tot = 0.0;
-    for (int idx = lane; idx < N * N; idx += 64) {
-        const int i = idx / N, j = idx % N;
-        const double x = double(A.ld(i, j));
-        tot += x * x;
-        if (i != j) off += x * x;
-    }
-    off = wave_sum_d(off);
-    tot = wave_sum_d(tot);
-}
+// Rotation parameters: the FP32 sweeps use the hardware reciprocal, square root and
+// reciprocal square root (about 1 ulp); the FP64 sweeps round correctly.  Any
+// (c, t) accurate to a few ulp gives a rotation orthogonal to working precision.
+__device__ inline float rot_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ inline double rot_rcp(double x) { return 1.0 / x; }
+__device__ inline float rot_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ inline double rot_sqrt(double x) { return sqrt(x); }
+__device__ inline float rot_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ inline double rot_rsq(double x) { return 1.0 / sqrt(x); }
 
+// symmetric N x N matrix as a packed upper triangle (rows of decreasing length)
 template <int N>
-struct MixedLds {
-    static constexpr int A32_BYTES = N * (N + 1) * 4;
-    static constexpr int TRI_BYTES = N * (N + 1) / 2 * 8;
-    static constexpr int TQ = 16;  // columns of V transposed at a time
-    static constexpr int T_STRIDE = TQ + 1;
-    static constexpr int ASM_BYTES = N * T_STRIDE * 8 + Q * N * 8 + Q * N * 4 + N * 8;
-    static constexpr int MAIN_BYTES =
-        (A32_BYTES > TRI_BYTES ? (A32_BYTES > ASM_BYTES ? A32_BYTES : ASM_BYTES)
-                               : (TRI_BYTES > ASM_BYTES ? TRI_BYTES : ASM_BYTES));
-};
+__device__ inline int tri_idx(int i, int j) {
+    const int lo = min(i, j), hi = max(i, j);
+    return lo * N - (lo * (lo - 1)) / 2 + (hi - lo);
+}
 
+// One round of the one-sided (Hestenes) Jacobi method on the column pairs
+// (p, p ^ m), p with the top bit of m clear: over the N - 1 rounds m = 1..N-1 of a
+// sweep every pair of columns meets once.  Lane j holds column j of U as sc * u[]
+// and al = ||column||^2.  The rotation that makes the two columns orthogonal is
+// the Jacobi rotation of their 2x2 Gram matrix [[a_p, g], [g, a_q]]
+// (t = sgn(z) / (|z| + sqrt(1 + z^2)), z = (a_q - a_p) / 2g, c = 1 / sqrt(1 + t^2)):
+//   u_p' = c (u_p - t u_q),  u_q' = c (u_q + t u_p),
+// applied as one FMA per element with the factor c folded into the column scale.
+// Both lanes of a pair form g, z, t from the same operands in the same order, so
+// they agree on the rotation bit for bit.  Returns |g| / sqrt(a_p a_q) before the
+// rotation.  The partner column is fetched once and held in registers.
+template <int N, class T>
+__device__ inline T hestenes_round(T (&u)[N], T& sc, T& al, int m, int lane) {
+    const bool low = (lane & (1 << (31 - __clz(m)))) == 0;
+    T o[N];
+    T g4[4] = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        o[i] = __shfl_xor(u[i], m);
+        g4[i & 3] = fma(u[i], o[i], g4[i & 3]);
+    }
+    const T sc_o = __shfl_xor(sc, m), al_o = __shfl_xor(al, m);
+    const T gam = (sc * sc_o) * ((g4[0] + g4[1]) + (g4[2] + g4[3]));
+    const T ap = low ? al : al_o, aq = low ? al_o : al;
+    T t = T(0), c = T(1);
+    if (gam != T(0)) {
+        // |z| -> inf (tiny g) gives t -> 0 through the reciprocal of inf
+        const T z = (aq - ap) * rot_rcp(T(2) * gam);
+        t = (z >= T(0) ? T(1) : T(-1)) * rot_rcp(fabs(z) + rot_sqrt(z * z + T(1)));
+        c = rot_rsq(t * t + T(1));
+    }
+    const T coef = (low ? -t : t) * (sc_o * rot_rcp(sc));
+#pragma unroll
+    for (int i = 0; i < N; ++i) u[i] = fma(coef, o[i], u[i]);
+    sc *= c;
+    al = low ? ap - t * gam : aq + t * gam;
+    const T den = ap * aq;
+    return den > T(0) ? fabs(gam) * rot_rsq(den) : T(0);
+}
+
+// fold the column scale into u and refresh the squared norm
+template <int N, class T>
+__device__ inline void hestenes_renorm(T (&u)[N], T& sc, T& al) {
+    T a4[4] = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        u[i] *= sc;
+        a4[i & 3] = fma(u[i], u[i], a4[i & 3]);
+    }
+    sc = T(1);
+    al = (a4[0] + a4[1]) + (a4[2] + a4[3]);
+}
+
+// ---------------------------------------------------------------------------------
+// kernel 1: gather, MFMA Gram, FP32 sweeps.  ws_u[g][i][j] = U[i][j], ws_b[g][i] = b_i
+// ---------------------------------------------------------------------------------
 template <int N, int PASSES>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_letkf_mixed(
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_mixed_solve(
     const int* __restrict__ loc_ptr, const int* __restrict__ loc_site, const float* __restrict__ loc_w,
-    const float* __restrict__ dy, const float* __restrict__ dyo, float* __restrict__ f, MixedParams mp,
-    double* __restrict__ dump, int* __restrict__ status) {
+    const float* __restrict__ dy, const float* __restrict__ dyo, float* __restrict__ ws_u,
+    float* __restrict__ ws_b, MixedParams mp) {
     constexpr int NT = N / 16;              // member tiles of 16
     constexpr int NTILE = NT * (NT + 1) / 2; // upper-triangular tiles of C
-    constexpr int NP = N / 2;
-    __shared__ __align__(16) unsigned char smem[MixedLds<N>::MAIN_BYTES];
-    __shared__ float rc32[NP], rs32[NP], rt32[NP];
-    __shared__ double rc64[NP], rs64[NP], rt64[NP];
-    __shared__ double bvec[N];
-    __shared__ double xbar[Q];
+    __shared__ float ctri[N * (N + 1) / 2];
 
     const int g = blockIdx.x;
     const int lane = threadIdx.x;
@@ -260,8 +209,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
             }
         }
     }
-    // result tile (I,J): lane holds rows 16I + 4*kq + r, column 16J + col
-    FullF32<N> A32{reinterpret_cast<float*>(smem)};
+    // result tile (I,J), I <= J: lane holds rows 16I + 4*kq + r, column 16J + col,
+    // so row <= column except inside the diagonal tiles
     {
         int tix = 0;
 #pragma unroll
@@ -273,99 +222,109 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
                     const int row = 16 * I + 4 * kq + r, cc = 16 * J + col;
                     float val = acc[tix][r];
                     if (row == cc) val += (row < M) ? mp.diag : 1.f;
-                    A32.st(row, cc, val);
-                    if (I != J) A32.st(cc, row, val);
+                    if (row <= cc) ctri[tri_idx<N>(row, cc)] = val;
                 }
             }
         }
+        float* bg = ws_b + (long long)g * N;
 #pragma unroll
         for (int I = 0; I < NT; ++I)
             if (col == 0) {
 #pragma unroll
-                for (int r = 0; r < 4; ++r) bvec[16 * I + 4 * kq + r] = double(acc[NTILE + I][r]);
+                for (int r = 0; r < 4; ++r) bg[16 * I + 4 * kq + r] = acc[NTILE + I][r];
             }
     }
     __syncthreads();
 
-    // ---- 3. FP32 Jacobi -----------------------------------------------------------
-    int bI[Blocks<N>::PER_LANE], bJ[Blocks<N>::PER_LANE];
+    // ---- 3. FP32 one-sided Jacobi on U = C ---------------------------------------------
+    // lanes >= N (N < 64) hold zero columns: their pair Gram is zero and they never rotate
+    float u[N];
 #pragma unroll
-    for (int u = 0; u < Blocks<N>::PER_LANE; ++u) {
-        const int b = lane + 64 * u;
-        bI[u] = -1;
-        bJ[u] = -1;
-        if (b < Blocks<N>::NBLK) {
-            int I = 0, rem = b;
-            while (rem >= NP - I) {
-                rem -= NP - I;
-                ++I;
-            }
-            bI[u] = I;
-            bJ[u] = I + rem;
-        }
+    for (int i = 0; i < N; ++i) {
+        const float x = ctri[tri_idx<N>(i, lane & (N - 1))];
+        u[i] = (lane < N) ? x : 0.f;
     }
-    float v[N];
-#pragma unroll
-    for (int i = 0; i < N; ++i) v[i] = (i == lane) ? 1.f : 0.f;
+    float sc = 1.f, al = 0.f;
     int sw32 = 0;
-    double prev_off = 0.0;
-    for (; sw32 < mp.max32; ++sw32) {
-        double off, tot;
-        frob<N, float>(A32, lane, off, tot);
-        // stop at the tolerance, or once FP32 rounding stalls the (by then
-        // quadratic) decrease below a relative off-norm of 1e-4
-        if (off <= double(mp.tol32_2) * tot || (sw32 > 0 && off <= 1e-8 * tot && off > 0.25 * prev_off)) break;
-        prev_off = off;
-        jacobi_sweep<N, float>(A32, rc32, rs32, rt32, v, bI, bJ, lane);
+    while (sw32 < mp.max32) {
+        hestenes_renorm<N, float>(u, sc, al);
+        float mc = 0.f;
+        for (int m = 1; m < N; ++m) mc = fmaxf(mc, hestenes_round<N, float>(u, sc, al, m, lane));
+        mc = wave_max(mc);
+        ++sw32;
+        if (mc <= mp.tol32) break;
     }
+    if (lane == 0 && mp.sweeps32) atomicAdd(mp.sweeps32, (unsigned long long)sw32);
+    if (lane < N) {
+        float* ug = ws_u + (long long)g * N * N;
+#pragma unroll
+        for (int i = 0; i < N; ++i) ug[i * N + lane] = u[i] * sc;
+    }
+}
 
-    // ---- 4. FP64 sweeps on the packed triangle ------------------------------------
-    constexpr int TRI = N * (N + 1) / 2;
-    constexpr int TRI_PER_LANE = (TRI + 63) / 64;
-    float tmp[TRI_PER_LANE];
-#pragma unroll
-    for (int u = 0; u < TRI_PER_LANE; ++u) {
-        const int t = lane + 64 * u;
-        tmp[u] = 0.f;
-        if (t < TRI) {
-            // row i of packed index t: start(i) = i N - i (i-1)/2 <= t < start(i+1)
-            const float b2 = float(2 * N + 1);
-            int i = int((b2 - sqrtf(b2 * b2 - 8.f * float(t))) * 0.5f);
-            i = max(0, min(i, N - 1));
-            while (i > 0 && i * N - (i * (i - 1)) / 2 > t) --i;
-            while (i < N - 1 && (i + 1) * N - ((i + 1) * i) / 2 <= t) ++i;
-            const int j = i + (t - (i * N - (i * (i - 1)) / 2));
-            tmp[u] = A32.ld(i, j);
-        }
-    }
-    __syncthreads();
-    TriF64<N> A64{reinterpret_cast<double*>(smem)};
-#pragma unroll
-    for (int u = 0; u < TRI_PER_LANE; ++u) {
-        const int t = lane + 64 * u;
-        if (t < TRI) A64.a[t] = double(tmp[u]);
-    }
+template <int N>
+struct FinishLds {
+    static constexpr int TQ = 16;  // columns of V transposed at a time
+    static constexpr int T_STRIDE = TQ + 1;
+    static constexpr int BYTES = N * T_STRIDE * 8 + Q * N * 8 + Q * N * 4 + N * 8;
+};
+
+// ---------------------------------------------------------------------------------
+// kernel 2: FP64 sweeps (SWEEP64), eigenpairs, assembly, analysis update of f
+// ---------------------------------------------------------------------------------
+template <int N, bool SWEEP64>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SWEEP64 ? 1 : 2))) void k_mixed_finish(
+    const float* __restrict__ ws_u, const float* __restrict__ ws_b, float* __restrict__ f, MixedParams mp,
+    double* __restrict__ dump, int* __restrict__ status) {
+    __shared__ __align__(16) unsigned char smem[FinishLds<N>::BYTES];
+    __shared__ double bvec[N];
+    __shared__ double xbar[Q];
+
+    const int g = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int M = mp.M;
+
     double vd[N];
+    {
+        const float* ug = ws_u + (long long)g * N * N;
+        const int l = lane & (N - 1);
 #pragma unroll
-    for (int i = 0; i < N; ++i) vd[i] = double(v[i]);
-    __syncthreads();
-    int sw64 = 0;
-    for (; sw64 < mp.max64; ++sw64) {
-        double off, tot;
-        frob<N, double>(A64, lane, off, tot);
-        if (off <= mp.tol64_2 * tot) break;
-        jacobi_sweep<N, double>(A64, rc64, rs64, rt64, vd, bI, bJ, lane);
+        for (int i = 0; i < N; ++i) {
+            const float x = ug[i * N + l];
+            vd[i] = (lane < N) ? double(x) : 0.0;
+        }
+        if (lane < N) bvec[lane] = double(ws_b[(long long)g * N + lane]);
     }
-    if (lane == 0) {
-        if (mp.sweeps32) atomicAdd(mp.sweeps32, (unsigned long long)sw32);
-        if (mp.sweeps64) atomicAdd(mp.sweeps64, (unsigned long long)sw64);
+    double scd = 1.0, ald = 0.0;
+
+    // ---- 4. FP64 sweeps on the widened columns ------------------------------------------
+    if constexpr (SWEEP64) {
+        int sw64 = 0;
+        while (sw64 < mp.max64) {
+            hestenes_renorm<N, double>(vd, scd, ald);
+            double mc = 0.0;
+            for (int m = 1; m < N; ++m) mc = fmax(mc, hestenes_round<N, double>(vd, scd, ald, m, lane));
+            mc = wave_max(mc);
+            ++sw64;
+            if (mc <= mp.tol64) break;
+        }
+        if (lane == 0 && mp.sweeps64) atomicAdd(mp.sweeps64, (unsigned long long)sw64);
     }
+    // eigenpairs: d = ||u||, v = u / d  (U = C V with orthogonal columns)
+    hestenes_renorm<N, double>(vd, scd, ald);
+    const double lam = (lane < N) ? sqrt(ald) : 1.0;
+    {
+        const double inv = 1.0 / lam;
+#pragma unroll
+        for (int i = 0; i < N; ++i) vd[i] *= inv;
+    }
+    __syncthreads();  // bvec
 
     // ---- 5. assembly in FP64 --------------------------------------------------------
     // column layout (lane = eigen-index i): d_i, h_i = (V^T b)_i / d_i, s_i = sqrt(M-1)/sqrt(d_i)
     double d_i = 1.0, h_i = 0.0, s_i = 0.0;
     if (lane < N) {
-        d_i = A64.ld(lane, lane);
+        d_i = lam;
         double gs = 0.0;
 #pragma unroll
         for (int r = 0; r < N; ++r) gs += vd[r] * bvec[r];
@@ -381,8 +340,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
         dg[M * M + lane] = d_i;
         dg[M * M + M + lane] = bvec[lane];
     }
-    __syncthreads();  // A64 is dead from here on; its LDS is reused
-    constexpr int TQ = MixedLds<N>::TQ, TS = MixedLds<N>::T_STRIDE;
+    constexpr int TQ = FinishLds<N>::TQ, TS = FinishLds<N>::T_STRIDE;
     double* Tq = reinterpret_cast<double*>(smem);       // [N][TS] chunk of V columns
     double* Zs = Tq + N * TS;                            // [Q][N] s_i (dX V)[q][i]
     float* dXs = reinterpret_cast<float*>(Zs + Q * N);   // [Q][N]
@@ -460,8 +418,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
 
 }  // namespace
 
+int mixed_width(int M) {
+    int N = 16;
+    while (N < M) N *= 2;
+    return N;
+}
+
 hipError_t launch_letkf_mixed(Ctx& c, float* f, unsigned long long* sweeps32, unsigned long long* sweeps64,
-                              int passes) {
+                              int passes, hipEvent_t ev_mid_end, hipEvent_t ev_mid_begin) {
+    const int N = mixed_width(c.M);
+    if (!c.d_loc_ptr || !c.d_dy || !c.d_dyo || !f || (c.dump_n > 0 && !c.d_dump)) return hipErrorInvalidValue;
+    if (!c.d_mix_u || !c.d_mix_b || c.mix_width != N) return hipErrorInvalidValue;
     MixedParams mp;
     mp.M = c.M;
     mp.nodes = int(c.nodes);
@@ -470,36 +437,51 @@ hipError_t launch_letkf_mixed(Ctx& c, float* f, unsigned long long* sweeps32, un
     mp.diag = float(double(c.M - 1) / c.cfg.inflation);
     for (int a = 0; a < 3; ++a) mp.coef[a] = c.obs.inv_sigma2[a];
     mp.sqm1 = sqrt(double(c.M - 1));
-    mp.tol32_2 = 1e-6f * 1e-6f;
-    mp.tol64_2 = c.cfg.jacobi_tol * c.cfg.jacobi_tol;
-    mp.max32 = 40;
-    mp.max64 = 4;
+    // Cyclic Jacobi converges quadratically near the end: a sweep that starts at a
+    // largest pair cosine of 3e-4 leaves about 1e-7, the FP32 rounding floor, so the
+    // FP32 phase stops there and the FP64 sweeps continue from that state.
+    mp.tol32 = 3e-4f;
+    mp.tol64 = c.cfg.jacobi_tol;
+    mp.max32 = c.cfg.jacobi_max_sweeps;
+    mp.max64 = c.cfg.jacobi_fp64_sweeps;
     mp.dump_g0 = c.dump_g0;
     mp.dump_n = c.dump_n;
     mp.sweeps32 = sweeps32;
     mp.sweeps64 = sweeps64;
     const dim3 grid(unsigned(c.nodes)), block(64);
-    int N = 16;
-    while (N < c.M) N *= 2;
-#define LAUNCH_MIXED(NN, PP)                                                                          \
-    k_letkf_mixed<NN, PP><<<grid, block, 0, c.stream>>>(c.d_loc_ptr, c.d_loc_site, c.d_loc_w, c.d_dy, \
-                                                         c.d_dyo, f, mp, c.d_dump, c.d_status)
-    if (passes == 3) {
-        switch (N) {
-            case 16: LAUNCH_MIXED(16, 3); break;
-            case 32: LAUNCH_MIXED(32, 3); break;
-            case 64: LAUNCH_MIXED(64, 3); break;
-            default: return hipErrorInvalidValue;
-        }
-    } else {
-        switch (N) {
-            case 16: LAUNCH_MIXED(16, 1); break;
-            case 32: LAUNCH_MIXED(32, 1); break;
-            case 64: LAUNCH_MIXED(64, 1); break;
-            default: return hipErrorInvalidValue;
-        }
+    const bool sweep64 = mp.max64 > 0;
+#define LAUNCH_SOLVE(NN, PP)                                                                           \
+    k_mixed_solve<NN, PP><<<grid, block, 0, c.stream>>>(c.d_loc_ptr, c.d_loc_site, c.d_loc_w, c.d_dy, \
+                                                         c.d_dyo, c.d_mix_u, c.d_mix_b, mp)
+#define LAUNCH_FINISH(NN)                                                                                     \
+    (sweep64 ? k_mixed_finish<NN, true><<<grid, block, 0, c.stream>>>(c.d_mix_u, c.d_mix_b, f, mp, c.d_dump,  \
+                                                                       c.d_status)                            \
+             : k_mixed_finish<NN, false><<<grid, block, 0, c.stream>>>(c.d_mix_u, c.d_mix_b, f, mp, c.d_dump, \
+                                                                        c.d_status))
+    switch (N * 10 + (passes == 3 ? 3 : 1)) {
+        case 163: LAUNCH_SOLVE(16, 3); break;
+        case 161: LAUNCH_SOLVE(16, 1); break;
+        case 323: LAUNCH_SOLVE(32, 3); break;
+        case 321: LAUNCH_SOLVE(32, 1); break;
+        case 643: LAUNCH_SOLVE(64, 3); break;
+        case 641: LAUNCH_SOLVE(64, 1); break;
+        default: return hipErrorInvalidValue;
     }
-#undef LAUNCH_MIXED
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    for (hipEvent_t ev : {ev_mid_end, ev_mid_begin}) {
+        if (!ev) continue;
+        e = hipEventRecord(ev, c.stream);
+        if (e != hipSuccess) return e;
+    }
+    switch (N) {
+        case 16: LAUNCH_FINISH(16); break;
+        case 32: LAUNCH_FINISH(32); break;
+        case 64: LAUNCH_FINISH(64); break;
+        default: return hipErrorInvalidValue;
+    }
+#undef LAUNCH_SOLVE
+#undef LAUNCH_FINISH
     return hipGetLastError();
 }
 

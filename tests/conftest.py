@@ -53,3 +53,45 @@ def f32_velocity_atol(cfg):
     each f rounded once when feq is evaluated and once when stored, carries an
     absolute error of up to ~12 half-ulps of 1/9, i.e. ~5e-8 c; 1e-7 c bounds it."""
     return 1e-7 * cfg.c
+
+
+class HipBuffers:
+    """Device buffers for tests that pass device pointers through the C-ABI
+    (libamdhip64 through ctypes: hipMalloc / hipMemcpy / hipFree)."""
+
+    def __init__(self):
+        import ctypes as C
+
+        self.C = C
+        self.l = C.CDLL("libamdhip64.so")
+        self.l.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        self.l.hipFree.argtypes = [C.c_void_p]
+        self.l.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        self.live = []
+
+    def _ck(self, st, what):
+        if st != 0:
+            raise RuntimeError(f"{what} failed with HIP status {st}")
+
+    def from_host(self, a):
+        """A device copy of the contiguous array a; freed by free_all()."""
+        import numpy as np
+
+        a = np.ascontiguousarray(a)
+        p = self.C.c_void_p()
+        self._ck(self.l.hipMalloc(self.C.byref(p), a.nbytes), "hipMalloc")
+        self.live.append(p.value)
+        self._ck(self.l.hipMemcpy(p, a.ctypes.data_as(self.C.c_void_p), a.nbytes, 1), "hipMemcpy H2D")
+        return p.value
+
+    def free_all(self):
+        for p in self.live:
+            self.l.hipFree(self.C.c_void_p(p))
+        self.live = []
+
+
+@pytest.fixture
+def hipbuf():
+    h = HipBuffers()
+    yield h
+    h.free_all()

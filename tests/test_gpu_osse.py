@@ -110,3 +110,74 @@ def test_assimilation_beats_free_run(lb, orc):
     print(f"rmse DA {g[-1, 0]:.4e} vs free {free[0]:.4e}; first cycle {g[0, 0]:.4e}")
     assert g[-1, 0] < 0.5 * free[0]
     assert g[-1, 0] < g[0, 0]
+
+
+def test_graph_cycle_matches_eager(lb, hipbuf):
+    # MIXED cycles replayed from the captured graph (phase timing off) against the same
+    # cycles launched eagerly (phase timing on): identical kernels, identical bits.
+    # Replays are separated by host synchronisation and state reads; an odd interval
+    # makes the two parities of the double buffer alternate, and set_dump in between
+    # forces a fresh capture.
+    cfg = small_cfg(lb, "baseline", n=64, obs_stride=4, loc_scale=6.0, loc_cutoff=14.0, da_interval=7)
+    M = 16
+    nat = lb.Context(cfg, 1)
+    nat.init_member(0, 7)
+    obs = []
+    for c in range(6):
+        nat.forecast(cfg.da_interval)
+        obs.append(nat.observe(0, seed=5, counter=c, noise=True))
+    nat.close()
+    yo_dev = [hipbuf.from_host(y) for y in obs]
+    runs = []
+    for timed in (False, True):
+        ens = lb.Context(cfg, M)
+        for k in range(M):
+            ens.init_member(k, 1000 + k)
+        ens.profile_enable(timed)
+        states, dumps = [], []
+        for c in range(6):
+            if c == 3:
+                ens.set_dump(64 * 32 + 5, 4)
+            ens.cycle_device(yo_dev[c])
+            ens.synchronize()
+            states.append(ens.get_all_states())
+            if c >= 3:
+                dumps.append(ens.get_dump())
+        assert ens.get_step() == 6 * cfg.da_interval
+        ens.close()
+        runs.append((states, dumps))
+    for c in range(6):
+        print(f"cycle {c}: graph vs eager max |diff| {np.abs(runs[0][0][c] - runs[1][0][c]).max():.3e}")
+        np.testing.assert_array_equal(runs[0][0][c], runs[1][0][c])
+    for a, b in zip(runs[0][1], runs[1][1]):
+        for x, y in zip(a, b):
+            np.testing.assert_array_equal(x, y)
+
+
+def test_plain_cycle_matches_analysis_then_forecast(lb, hipbuf):
+    # lbmletkf_cycle on the PLAIN variant (eager launches; its per-gridpoint matrices
+    # are allocated by the first analysis, whichever entry point runs it) against
+    # analysis() followed by forecast(): identical kernels, identical bits
+    cfg = small_cfg(lb, "baseline", n=64, obs_stride=8, loc_scale=6.0, loc_cutoff=20.0, da_interval=5,
+                    variant=lb.VARIANT_PLAIN)
+    M = 8
+    nat = lb.Context(cfg, 1)
+    nat.init_member(0, 7)
+    nat.forecast(cfg.da_interval)
+    yo = nat.observe(0, seed=5, counter=0, noise=True)
+    nat.close()
+    yo_dev = hipbuf.from_host(yo)
+    out = []
+    for use_cycle in (True, False):
+        ens = lb.Context(cfg, M)
+        for k in range(M):
+            ens.init_member(k, 1000 + k)
+        if use_cycle:
+            ens.cycle_device(yo_dev)
+        else:
+            ens.analysis(yo)
+            ens.forecast(cfg.da_interval)
+        ens.synchronize()
+        out.append(ens.get_all_states())
+        ens.close()
+    np.testing.assert_array_equal(out[0], out[1])
