@@ -104,6 +104,7 @@ struct Ctx {
     float* d_mix_u = nullptr;     // [nodes][N][N] FP32 columns of U = C V after the FP32 sweeps
     float* d_mix_b = nullptr;     // [nodes][N] b = dY^T R^-1 dyo
     int mix_width = 0;
+    int mix_chain = 8;            // consecutive gridpoints per wavefront in k_mixed_solve
 
     // transform dump
     int dump_g0 = 0, dump_n = 0;

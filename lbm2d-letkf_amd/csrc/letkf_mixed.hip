@@ -2,7 +2,8 @@
 // src/util/letkf_solver.h:235-349 at every gridpoint, one wavefront per gridpoint,
 // in two kernels:
 //
-// k_mixed_solve (FP32, ~150 VGPRs, 3 waves per SIMD)
+// k_mixed_solve (FP32, ~150 VGPRs, C in 16 KB of LDS; a chain of neighbouring
+// gridpoints per wavefront, each warm-started from its predecessor's eigenvectors)
 //  1. gather: the rows of dY (and the innovation) of the gridpoint's local
 //     observations, scaled by sqrt(R^-1 o localization), straight from L2;
 //  2. Gram on the matrix cores: C = (M-1)/beta I + G^T G and b = G^T (sqrt(Rl^-1) dyo)
@@ -45,6 +46,7 @@ struct MixedParams {
     double tol64;           // FP64 phase: the same test, then at most max64 sweeps
     int max32, max64;
     int dump_g0, dump_n;
+    int chain;              // k_mixed_solve: consecutive gridpoints per wavefront
     unsigned long long* sweeps32;
     unsigned long long* sweeps64;
 };
@@ -71,13 +73,6 @@ __device__ inline float rot_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ inline double rot_sqrt(double x) { return sqrt(x); }
 __device__ inline float rot_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ inline double rot_rsq(double x) { return 1.0 / sqrt(x); }
-
-// symmetric N x N matrix as a packed upper triangle (rows of decreasing length)
-template <int N>
-__device__ inline int tri_idx(int i, int j) {
-    const int lo = min(i, j), hi = max(i, j);
-    return lo * N - (lo * (lo - 1)) / 2 + (hi - lo);
-}
 
 // One round of the one-sided (Hestenes) Jacobi method on the column pairs
 // (p, p ^ m), p with the top bit of m clear: over the N - 1 rounds m = 1..N-1 of a
@@ -133,23 +128,17 @@ __device__ inline void hestenes_renorm(T (&u)[N], T& sc, T& al) {
 }
 
 // ---------------------------------------------------------------------------------
-// kernel 1: gather, MFMA Gram, FP32 sweeps.  ws_u[g][i][j] = U[i][j], ws_b[g][i] = b_i
+// gather + MFMA Gram of gridpoint g: C (full, row stride N) into LDS, b into ws_b
 // ---------------------------------------------------------------------------------
 template <int N, int PASSES>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_mixed_solve(
-    const int* __restrict__ loc_ptr, const int* __restrict__ loc_site, const float* __restrict__ loc_w,
-    const float* __restrict__ dy, const float* __restrict__ dyo, float* __restrict__ ws_u,
-    float* __restrict__ ws_b, MixedParams mp) {
+__device__ inline void gram_to_lds(int g, int lane, const int* __restrict__ loc_ptr,
+                                   const int* __restrict__ loc_site, const float* __restrict__ loc_w,
+                                   const float* __restrict__ dy, const float* __restrict__ dyo,
+                                   float* __restrict__ ws_b, const MixedParams& mp, float* cfull) {
     constexpr int NT = N / 16;              // member tiles of 16
     constexpr int NTILE = NT * (NT + 1) / 2; // upper-triangular tiles of C
-    __shared__ float ctri[N * (N + 1) / 2];
-
-    const int g = blockIdx.x;
-    const int lane = threadIdx.x;
     const int M = mp.M;
     const int nv = mp.n_vars;
-
-    // ---- 1+2. gather and Gram on MFMA -------------------------------------------
     f32x4 acc[NTILE + NT];
 #pragma unroll
     for (int a = 0; a < NTILE + NT; ++a) acc[a] = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -209,56 +198,113 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
             }
         }
     }
-    // result tile (I,J), I <= J: lane holds rows 16I + 4*kq + r, column 16J + col,
-    // so row <= column except inside the diagonal tiles
-    {
-        int tix = 0;
+    // result tile (I,J), I <= J: lane holds rows 16I + 4*kq + r, column 16J + col;
+    // the tiles below the diagonal are the mirrors of those above
+    int tix = 0;
 #pragma unroll
-        for (int I = 0; I < NT; ++I) {
+    for (int I = 0; I < NT; ++I) {
 #pragma unroll
-            for (int J = I; J < NT; ++J, ++tix) {
+        for (int J = I; J < NT; ++J, ++tix) {
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int row = 16 * I + 4 * kq + r, cc = 16 * J + col;
-                    float val = acc[tix][r];
-                    if (row == cc) val += (row < M) ? mp.diag : 1.f;
-                    if (row <= cc) ctri[tri_idx<N>(row, cc)] = val;
-                }
+            for (int r = 0; r < 4; ++r) {
+                const int row = 16 * I + 4 * kq + r, cc = 16 * J + col;
+                float val = acc[tix][r];
+                if (row == cc) val += (row < M) ? mp.diag : 1.f;
+                cfull[row * N + cc] = val;
+                if (I != J) cfull[cc * N + row] = val;
             }
         }
-        float* bg = ws_b + (long long)g * N;
-#pragma unroll
-        for (int I = 0; I < NT; ++I)
-            if (col == 0) {
-#pragma unroll
-                for (int r = 0; r < 4; ++r) bg[16 * I + 4 * kq + r] = acc[NTILE + I][r];
-            }
     }
-    __syncthreads();
+    float* bg = ws_b + (long long)g * N;
+#pragma unroll
+    for (int I = 0; I < NT; ++I)
+        if (col == 0) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) bg[16 * I + 4 * kq + r] = acc[NTILE + I][r];
+        }
+}
 
-    // ---- 3. FP32 one-sided Jacobi on U = C ---------------------------------------------
-    // lanes >= N (N < 64) hold zero columns: their pair Gram is zero and they never rotate
-    float u[N];
+// ---------------------------------------------------------------------------------
+// kernel 1: gather, MFMA Gram, FP32 sweeps for a chain of mp.chain consecutive
+// gridpoints.  The first starts from U = C; each later one from U = C V_prev, the
+// eigenvectors of its neighbour (V_prev = the neighbour's U with unit columns),
+// which leaves far fewer sweeps: the local observation sets of neighbouring
+// gridpoints share almost all sites and weights.  U = C V with V orthogonal either
+// way, so the eigenpairs still come out as column norms and directions.
+// ws_u[g][i][j] = U[i][j], ws_b[g][i] = b_i.
+// ---------------------------------------------------------------------------------
+template <int N, int PASSES>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_mixed_solve(
+    const int* __restrict__ loc_ptr, const int* __restrict__ loc_site, const float* __restrict__ loc_w,
+    const float* __restrict__ dy, const float* __restrict__ dyo, float* __restrict__ ws_u,
+    float* __restrict__ ws_b, MixedParams mp) {
+    __shared__ __align__(16) float cfull[N * N];
+    const int lane = threadIdx.x;
+    const int l = lane & (N - 1);
+    const int g_begin = blockIdx.x * mp.chain;
+    const int g_end = min(g_begin + mp.chain, mp.nodes);
+    for (int g = g_begin; g < g_end; ++g) {
+        __syncthreads();  // the previous gridpoint's reads of cfull are done
+        gram_to_lds<N, PASSES>(g, lane, loc_ptr, loc_site, loc_w, dy, dyo, ws_b, mp, cfull);
+        __syncthreads();
+        float u[N];
+        if (g > g_begin) {
+            // V_prev: the previous gridpoint's U (written by this lane) with unit columns
+            float v[N];
+            const float* up = ws_u + (long long)(g - 1) * N * N;
+            float n4[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int i = 0; i < N; ++i) {
-        const float x = ctri[tri_idx<N>(i, lane & (N - 1))];
-        u[i] = (lane < N) ? x : 0.f;
-    }
-    float sc = 1.f, al = 0.f;
-    int sw32 = 0;
-    while (sw32 < mp.max32) {
-        hestenes_renorm<N, float>(u, sc, al);
-        float mc = 0.f;
-        for (int m = 1; m < N; ++m) mc = fmaxf(mc, hestenes_round<N, float>(u, sc, al, m, lane));
-        mc = wave_max(mc);
-        ++sw32;
-        if (mc <= mp.tol32) break;
-    }
-    if (lane == 0 && mp.sweeps32) atomicAdd(mp.sweeps32, (unsigned long long)sw32);
-    if (lane < N) {
-        float* ug = ws_u + (long long)g * N * N;
+            for (int i = 0; i < N; ++i) {
+                v[i] = (lane < N) ? up[i * N + l] : 0.f;
+                n4[i & 3] = fmaf(v[i], v[i], n4[i & 3]);
+            }
+            const float nn = (n4[0] + n4[1]) + (n4[2] + n4[3]);
+            const float inv = nn > 0.f ? rot_rsq(nn) : 0.f;
 #pragma unroll
-        for (int i = 0; i < N; ++i) ug[i * N + lane] = u[i] * sc;
+            for (int i = 0; i < N; ++i) v[i] *= inv;
+            // U0 = C V_prev row by row: row i of U0 replaces row i of C, which no later
+            // row reads (each row is read in full, as a broadcast, before it is written)
+            for (int i = 0; i < N; ++i) {
+                const float4* crow = reinterpret_cast<const float4*>(cfull + i * N);
+                float a4[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int k = 0; k < N / 4; ++k) {
+                    const float4 c4 = crow[k];
+                    a4[0] = fmaf(c4.x, v[4 * k], a4[0]);
+                    a4[1] = fmaf(c4.y, v[4 * k + 1], a4[1]);
+                    a4[2] = fmaf(c4.z, v[4 * k + 2], a4[2]);
+                    a4[3] = fmaf(c4.w, v[4 * k + 3], a4[3]);
+                }
+                __builtin_amdgcn_wave_barrier();
+                if (lane < N) cfull[i * N + lane] = (a4[0] + a4[1]) + (a4[2] + a4[3]);
+            }
+            __syncthreads();
+        }
+        // column `lane` (C symmetric, so row and column read alike)
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            const float x = cfull[i * N + l];
+            u[i] = (lane < N) ? x : 0.f;
+        }
+
+        // ---- 3. FP32 one-sided Jacobi ------------------------------------------------
+        // lanes >= N (N < 64) hold zero columns: their pair Gram is zero, they never rotate
+        float sc = 1.f, al = 0.f;
+        int sw32 = 0;
+        while (sw32 < mp.max32) {
+            hestenes_renorm<N, float>(u, sc, al);
+            float mc = 0.f;
+            for (int m = 1; m < N; ++m) mc = fmaxf(mc, hestenes_round<N, float>(u, sc, al, m, lane));
+            mc = wave_max(mc);
+            ++sw32;
+            if (mc <= mp.tol32) break;
+        }
+        if (lane == 0 && mp.sweeps32) atomicAdd(mp.sweeps32, (unsigned long long)sw32);
+        if (lane < N) {
+            float* ug = ws_u + (long long)g * N * N;
+#pragma unroll
+            for (int i = 0; i < N; ++i) ug[i * N + lane] = u[i] * sc;
+        }
     }
 }
 
@@ -448,10 +494,12 @@ hipError_t launch_letkf_mixed(Ctx& c, float* f, unsigned long long* sweeps32, un
     mp.dump_n = c.dump_n;
     mp.sweeps32 = sweeps32;
     mp.sweeps64 = sweeps64;
+    mp.chain = c.mix_chain;
     const dim3 grid(unsigned(c.nodes)), block(64);
+    const dim3 grid_solve(unsigned((c.nodes + mp.chain - 1) / mp.chain));
     const bool sweep64 = mp.max64 > 0;
 #define LAUNCH_SOLVE(NN, PP)                                                                           \
-    k_mixed_solve<NN, PP><<<grid, block, 0, c.stream>>>(c.d_loc_ptr, c.d_loc_site, c.d_loc_w, c.d_dy, \
+    k_mixed_solve<NN, PP><<<grid_solve, block, 0, c.stream>>>(c.d_loc_ptr, c.d_loc_site, c.d_loc_w, c.d_dy, \
                                                          c.d_dyo, c.d_mix_u, c.d_mix_b, mp)
 #define LAUNCH_FINISH(NN)                                                                                     \
     (sweep64 ? k_mixed_finish<NN, true><<<grid, block, 0, c.stream>>>(c.d_mix_u, c.d_mix_b, f, mp, c.d_dump,  \
