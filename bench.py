@@ -238,11 +238,26 @@ def main():
     ap.add_argument("--variants", default="plain,mixed")
     ap.add_argument("--prof-cycles", type=int, default=3, help="extra phase-timed cycles (eager launches)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run 3 device-to-device copies of MIB MiB (HIP's blit kernel): the "
+                         "calibration of the FETCH_SIZE / WRITE_SIZE counters, then exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this build targets one MI355X; "
               "the multi-GPU layout is described in DESIGN.md", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        hip = Hip()
+        n = args.copy_probe << 20
+        a, b = hip.malloc(n), hip.malloc(n)
+        hip.l.hipMemset(C.c_void_p(a), 1, C.c_size_t(n))
+        for _ in range(3):
+            hip.ck(hip.l.hipMemcpy(C.c_void_p(b), C.c_void_p(a), C.c_size_t(n), 3), "hipMemcpy D2D")
+        hip.sync()
+        hip.free(a)
+        hip.free(b)
+        print(json.dumps({"copy_probe_bytes": n, "copies": 3}), flush=True)
+        return
 
     def log(msg):
         print(msg, file=sys.stderr, flush=True)

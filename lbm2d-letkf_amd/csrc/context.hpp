@@ -105,6 +105,7 @@ struct Ctx {
     float* d_mix_b = nullptr;     // [nodes][N] b = dY^T R^-1 dyo
     int mix_width = 0;
     int mix_chain = 8;            // consecutive gridpoints per wavefront in k_mixed_solve
+    int lbm_tiles_per_block = 1;  // 64x16 tiles per workgroup in k_lbm_fused
 
     // transform dump
     int dump_g0 = 0, dump_n = 0;

@@ -10,7 +10,9 @@
 //   * lbm_fused: one launch advances every member by one step.  A 64x16 tile of
 //     one member is streamed into registers, its streamed velocity (plus a one-node
 //     halo) goes to LDS for the strain stencil, and the collided populations are
-//     written once: 36 B read + 36 B written per node update.
+//     written once: 36 B read + 36 B written per node update.  A workgroup may
+//     take several tiles of a row band in turn (one per workgroup is fastest at
+//     256^2 x 64: the tiles of one workgroup do not overlap their loads).
 //   * the plain form keeps the reference's three passes (stream+macro, les,
 //     collide) and is launched member by member, as one reference rank would.
 #include "context.hpp"
@@ -86,7 +88,9 @@ __device__ __forceinline__ void equilibrium9(float rho, float u, float v, const 
 }
 
 // BGK relaxation towards feq plus the first-order force term with linear
-// friction (src/lbm_collision_srt.h:8-16, src/lbm.h:41-70).
+// friction (src/lbm_collision_srt.h:8-16, src/lbm.h:41-70).  FORCE = false is for
+// runs with neither forcing nor friction, where the force term is exactly zero.
+template <bool FORCE>
 __device__ __forceinline__ void collide9(float fc[Q], float rho, float u, float v, float nut,
                                          float Fx, float Fy, const LbmParams& p) {
     const float tau = 0.5f + p.k_tau * (p.nu + nut);
@@ -100,7 +104,10 @@ __device__ __forceinline__ void collide9(float fc[Q], float rho, float u, float 
         const int ex = q % 3 - 1, ey = q / 3 - 1;
         const float wq = (ex == 0 && ey == 0) ? W0 : ((ex == 0 || ey == 0) ? W1 : W2);
         const float fq = fc[q] - omega * (fc[q] - feq[q]);
-        fc[q] = fq + wq * rho * p.force_scale * (float(ex) * fxp + float(ey) * fyp);
+        if constexpr (FORCE)
+            fc[q] = fq + wq * rho * p.force_scale * (float(ex) * fxp + float(ey) * fyp);
+        else
+            fc[q] = fq;
     }
 }
 
@@ -112,82 +119,89 @@ constexpr int ROWS_PER_PASS = NT / TX;        // 4
 constexpr int NODES_PER_THREAD = TY / ROWS_PER_PASS;  // 4
 constexpr int HALO = 2 * (TX + 2) + 2 * TY;   // 164
 
+template <bool FORCE>
 __global__ __launch_bounds__(NT) void k_lbm_fused(const float* __restrict__ fin,
                                                   float* __restrict__ fout,
                                                   const float* __restrict__ force, LbmParams p,
-                                                  int tiles_x, int tiles_per_member) {
-    // XCD-aware tile order: workgroups are dealt round-robin to the 8 XCDs, so
-    // give each XCD a contiguous run of tiles (neighbouring tiles share halo rows
-    // in that XCD's L2).
+                                                  int tiles_x, int tiles_per_member, int tiles_per_block) {
+    // Each workgroup advances tiles_per_block consecutive tiles (a row band of one
+    // member), so the grid is one resident generation with no tail.  XCD-aware
+    // order: workgroups are dealt round-robin to the 8 XCDs, so each XCD gets a
+    // contiguous run of bands (neighbouring bands share halo rows in its L2).
     const int nb = gridDim.x;
     const int b = blockIdx.x;
-    const int t = (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
-    const int member = t / tiles_per_member;
-    const int tt = t - member * tiles_per_member;
-    const int x0 = (tt % tiles_x) * TX;
-    const int y0 = (tt / tiles_x) * TY;
+    const int bb = (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
     const int nx = p.nx, ny = p.ny;
     const long long nodes = p.nodes;
-    const float* f = fin + (long long)member * p.member_stride;
-    float* fo = fout + (long long)member * p.member_stride;
 
     __shared__ float su[TY + 2][TX + 2];
     __shared__ float sv[TY + 2][TX + 2];
 
     const int lx = threadIdx.x % TX;
     const int ly0 = threadIdx.x / TX;
-    const int i = x0 + lx;
-    const int im = wrap(i - 1, nx), ip = wrap(i + 1, nx);
 
-    float fc[NODES_PER_THREAD][Q];
-    float rr[NODES_PER_THREAD], uu[NODES_PER_THREAD], vv[NODES_PER_THREAD];
+    for (int k = 0; k < tiles_per_block; ++k) {
+        const int t = bb * tiles_per_block + k;
+        const int member = t / tiles_per_member;
+        const int tt = t - member * tiles_per_member;
+        const int x0 = (tt % tiles_x) * TX;
+        const int y0 = (tt / tiles_x) * TY;
+        const float* f = fin + (long long)member * p.member_stride;
+        float* fo = fout + (long long)member * p.member_stride;
+        const int i = x0 + lx;
+        const int im = wrap(i - 1, nx), ip = wrap(i + 1, nx);
+
+        __syncthreads();  // the previous tile's stencil reads of su/sv are done
+        float fc[NODES_PER_THREAD][Q];
+        float rr[NODES_PER_THREAD], uu[NODES_PER_THREAD], vv[NODES_PER_THREAD];
 #pragma unroll
-    for (int r = 0; r < NODES_PER_THREAD; ++r) {
-        const int ly = ly0 + r * ROWS_PER_PASS;
-        const int j = y0 + ly;
-        const long long r0 = (long long)j * nx;
-        const long long rm = (long long)wrap(j - 1, ny) * nx;
-        const long long rp = (long long)wrap(j + 1, ny) * nx;
-        pull9(f, nodes, im, i, ip, rm, r0, rp, fc[r]);
-        moments(fc[r], p.c, rr[r], uu[r], vv[r]);
-        su[ly + 1][lx + 1] = uu[r];
-        sv[ly + 1][lx + 1] = vv[r];
-    }
-    // halo ring of streamed velocities
-    if (threadIdx.x < HALO) {
-        int h = threadIdx.x, hx, hy;
-        if (h < TX + 2) { hx = h - 1; hy = -1; }
-        else if (h < 2 * (TX + 2)) { hx = h - (TX + 2) - 1; hy = TY; }
-        else if (h < 2 * (TX + 2) + TY) { hx = -1; hy = h - 2 * (TX + 2); }
-        else { hx = TX; hy = h - 2 * (TX + 2) - TY; }
-        const int gi = wrap(x0 + hx, nx), gj = wrap(y0 + hy, ny);
-        const int gim = wrap(gi - 1, nx), gip = wrap(gi + 1, nx);
-        const long long r0 = (long long)gj * nx;
-        const long long rm = (long long)wrap(gj - 1, ny) * nx;
-        const long long rp = (long long)wrap(gj + 1, ny) * nx;
-        float fh[Q], rh, uh, vh;
-        pull9(f, nodes, gim, gi, gip, rm, r0, rp, fh);
-        moments(fh, p.c, rh, uh, vh);
-        su[hy + 1][hx + 1] = uh;
-        sv[hy + 1][hx + 1] = vh;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < NODES_PER_THREAD; ++r) {
-        const int ly = ly0 + r * ROWS_PER_PASS;
-        const int j = y0 + ly;
-        const float nut = smagorinsky(su[ly + 1][lx + 2], su[ly + 1][lx], su[ly + 2][lx + 1],
-                                      su[ly][lx + 1], sv[ly + 1][lx + 2], sv[ly + 1][lx],
-                                      sv[ly + 2][lx + 1], sv[ly][lx + 1], p);
-        const long long node = (long long)j * nx + i;
-        float Fx = 0.f, Fy = 0.f;
-        if (p.has_force) {
-            Fx = force[node];
-            Fy = force[nodes + node];
+        for (int r = 0; r < NODES_PER_THREAD; ++r) {
+            const int ly = ly0 + r * ROWS_PER_PASS;
+            const int j = y0 + ly;
+            const long long r0 = (long long)j * nx;
+            const long long rm = (long long)wrap(j - 1, ny) * nx;
+            const long long rp = (long long)wrap(j + 1, ny) * nx;
+            pull9(f, nodes, im, i, ip, rm, r0, rp, fc[r]);
+            moments(fc[r], p.c, rr[r], uu[r], vv[r]);
+            su[ly + 1][lx + 1] = uu[r];
+            sv[ly + 1][lx + 1] = vv[r];
         }
-        collide9(fc[r], rr[r], uu[r], vv[r], nut, Fx, Fy, p);
+        // halo ring of streamed velocities
+        if (threadIdx.x < HALO) {
+            int h = threadIdx.x, hx, hy;
+            if (h < TX + 2) { hx = h - 1; hy = -1; }
+            else if (h < 2 * (TX + 2)) { hx = h - (TX + 2) - 1; hy = TY; }
+            else if (h < 2 * (TX + 2) + TY) { hx = -1; hy = h - 2 * (TX + 2); }
+            else { hx = TX; hy = h - 2 * (TX + 2) - TY; }
+            const int gi = wrap(x0 + hx, nx), gj = wrap(y0 + hy, ny);
+            const int gim = wrap(gi - 1, nx), gip = wrap(gi + 1, nx);
+            const long long r0 = (long long)gj * nx;
+            const long long rm = (long long)wrap(gj - 1, ny) * nx;
+            const long long rp = (long long)wrap(gj + 1, ny) * nx;
+            float fh[Q], rh, uh, vh;
+            pull9(f, nodes, gim, gi, gip, rm, r0, rp, fh);
+            moments(fh, p.c, rh, uh, vh);
+            su[hy + 1][hx + 1] = uh;
+            sv[hy + 1][hx + 1] = vh;
+        }
+        __syncthreads();
 #pragma unroll
-        for (int q = 0; q < Q; ++q) fo[q * nodes + node] = fc[r][q];
+        for (int r = 0; r < NODES_PER_THREAD; ++r) {
+            const int ly = ly0 + r * ROWS_PER_PASS;
+            const int j = y0 + ly;
+            const float nut = smagorinsky(su[ly + 1][lx + 2], su[ly + 1][lx], su[ly + 2][lx + 1],
+                                          su[ly][lx + 1], sv[ly + 1][lx + 2], sv[ly + 1][lx],
+                                          sv[ly + 2][lx + 1], sv[ly][lx + 1], p);
+            const long long node = (long long)j * nx + i;
+            float Fx = 0.f, Fy = 0.f;
+            if (FORCE && p.has_force) {
+                Fx = force[node];
+                Fy = force[nodes + node];
+            }
+            collide9<FORCE>(fc[r], rr[r], uu[r], vv[r], nut, Fx, Fy, p);
+#pragma unroll
+            for (int q = 0; q < Q; ++q) fo[q * nodes + node] = fc[r][q];
+        }
     }
 }
 
@@ -237,7 +251,7 @@ __global__ void k_collide(float* __restrict__ fn, const float* __restrict__ ruv,
         Fx = force[node];
         Fy = force[p.nodes + node];
     }
-    collide9(fc, ruv[node], ruv[p.nodes + node], ruv[2 * p.nodes + node], nus[node], Fx, Fy, p);
+    collide9<true>(fc, ruv[node], ruv[p.nodes + node], ruv[2 * p.nodes + node], nus[node], Fx, Fy, p);
 #pragma unroll
     for (int q = 0; q < Q; ++q) fn[q * p.nodes + node] = fc[q];
 }
@@ -383,8 +397,15 @@ hipError_t launch_step_advance(Ctx& c, int n) {
 hipError_t launch_lbm_fused(Ctx& c, const float* fin, float* fout) {
     const int tiles_x = c.nx / TX, tiles_y = c.ny / TY;
     const int tiles = tiles_x * tiles_y;
-    const int nb = tiles * c.M;
-    k_lbm_fused<<<nb, NT, 0, c.stream>>>(fin, fout, c.force_xy, c.lp, tiles_x, tiles);
+    const int total = tiles * c.M;
+    const int per_block = c.lbm_tiles_per_block;
+    if (per_block < 1 || total % per_block != 0) return hipErrorInvalidValue;
+    const int nb = total / per_block;
+    const bool force = c.lp.has_force || c.lp.fric_over_dt != 0.f;
+    if (force)
+        k_lbm_fused<true><<<nb, NT, 0, c.stream>>>(fin, fout, c.force_xy, c.lp, tiles_x, tiles, per_block);
+    else
+        k_lbm_fused<false><<<nb, NT, 0, c.stream>>>(fin, fout, c.force_xy, c.lp, tiles_x, tiles, per_block);
     return hipGetLastError();
 }
 

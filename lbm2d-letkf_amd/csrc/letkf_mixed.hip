@@ -64,6 +64,11 @@ __device__ inline T wave_max(T x) {
     return x;
 }
 
+// Workgroups are dealt round-robin to the 8 XCDs; this order gives each XCD a
+// contiguous range of gridpoints, so the strided member reads and writes of
+// neighbouring gridpoints share cache lines in that XCD's L2.
+__device__ inline int xcd_order(int b, int nb) { return (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b; }
+
 // Rotation parameters: the FP32 sweeps use the hardware reciprocal, square root and
 // reciprocal square root (about 1 ulp); the FP64 sweeps round correctly.  Any
 // (c, t) accurate to a few ulp gives a rotation orthogonal to working precision.
@@ -241,7 +246,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
     __shared__ __align__(16) float cfull[N * N];
     const int lane = threadIdx.x;
     const int l = lane & (N - 1);
-    const int g_begin = blockIdx.x * mp.chain;
+    const int g_begin = xcd_order(blockIdx.x, gridDim.x) * mp.chain;
     const int g_end = min(g_begin + mp.chain, mp.nodes);
     for (int g = g_begin; g < g_end; ++g) {
         __syncthreads();  // the previous gridpoint's reads of cfull are done
@@ -326,7 +331,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SWEEP64 ? 1 
     __shared__ double bvec[N];
     __shared__ double xbar[Q];
 
-    const int g = blockIdx.x;
+    const int g = xcd_order(blockIdx.x, gridDim.x);
     const int lane = threadIdx.x;
     const int M = mp.M;
 
@@ -484,9 +489,10 @@ hipError_t launch_letkf_mixed(Ctx& c, float* f, unsigned long long* sweeps32, un
     for (int a = 0; a < 3; ++a) mp.coef[a] = c.obs.inv_sigma2[a];
     mp.sqm1 = sqrt(double(c.M - 1));
     // Cyclic Jacobi converges quadratically near the end: a sweep that starts at a
-    // largest pair cosine of 3e-4 leaves about 1e-7, the FP32 rounding floor, so the
-    // FP32 phase stops there and the FP64 sweeps continue from that state.
-    mp.tol32 = 3e-4f;
+    // largest pair cosine of 3e-4 leaves about 1e-7, the FP32 rounding floor.  With
+    // an FP64 sweep to follow, the FP32 phase can stop one sweep earlier (3e-3
+    // leaves ~1e-5, which the FP64 sweep squares away); the W error is unchanged.
+    mp.tol32 = c.cfg.jacobi_fp64_sweeps > 0 ? 3e-3f : 3e-4f;
     mp.tol64 = c.cfg.jacobi_tol;
     mp.max32 = c.cfg.jacobi_max_sweeps;
     mp.max64 = c.cfg.jacobi_fp64_sweeps;
