@@ -16,10 +16,11 @@
 //     definite, U -> C V = V D, so the eigenpairs are d_j = ||u_j||, v_j = u_j / d_j
 //     and V is never accumulated.  U and b go to a workspace in HBM.
 //
-// k_mixed_finish (FP64, one wave per SIMD when it sweeps)
-//  4. the same sweeps in FP64 on the widened columns (partner column held in
-//     registers) until the pairwise cosines fall below the eigensolver tolerance
-//     or the sweep cap is reached;
+// k_mixed_finish (FP64, two wavefronts per gridpoint, each with half the rows)
+//  4. the same sweeps in FP64 on the widened columns (partner half-column held in
+//     registers, pair inner products summed across the two waves through LDS)
+//     until the pairwise cosines fall below the eigensolver tolerance or the sweep
+//     cap is reached;
 //  5. inverse-square-root assembly in FP64 without forming W:
 //       Xa = xbar + dX (W + w 1^T) = xbar + sqrt(M-1) (dX V) D^-1/2 V^T + (dX w) 1^T,
 //       w = V D^-1 V^T b,
@@ -313,124 +314,189 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
     }
 }
 
+// ---------------------------------------------------------------------------------
+// kernel 2: two wavefronts per gridpoint.  Wave w holds rows [w N/2, (w+1) N/2) of
+// every column (lane j = column j), so a pair inner product is the sum of the two
+// waves' partial sums, exchanged through LDS with one barrier per round (the
+// exchange buffer alternates between rounds).  Both waves then form the same
+// rotation from the same operands in the same order.
+// ---------------------------------------------------------------------------------
 template <int N>
 struct FinishLds {
-    static constexpr int TQ = 16;  // columns of V transposed at a time
+    static constexpr int H = N / 2;    // rows per wave
+    static constexpr int TQ = 16;      // columns of V transposed at a time
     static constexpr int T_STRIDE = TQ + 1;
-    static constexpr int BYTES = N * T_STRIDE * 8 + Q * N * 8 + Q * N * 4 + N * 8;
 };
 
-// ---------------------------------------------------------------------------------
-// kernel 2: FP64 sweeps (SWEEP64), eigenpairs, assembly, analysis update of f
-// ---------------------------------------------------------------------------------
+// Sum of the two waves' values of x at this lane (same order in both waves).
+__device__ inline double pair_sum(double x, double* buf, int w, int lane) {
+    buf[w * 64 + lane] = x;
+    __syncthreads();
+    return buf[lane] + buf[64 + lane];
+}
+
+// one rotation round of the split columns; `part` is this round's exchange buffer
+template <int H>
+__device__ inline double hestenes_round_split(double (&u)[H], double& sc, double& al, int m, int lane, int w,
+                                              double* part) {
+    const bool low = (lane & (1 << (31 - __clz(m)))) == 0;
+    double o[H];
+    double g4[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int i = 0; i < H; ++i) {
+        o[i] = __shfl_xor(u[i], m);
+        g4[i & 3] = fma(u[i], o[i], g4[i & 3]);
+    }
+    const double gp = pair_sum((g4[0] + g4[1]) + (g4[2] + g4[3]), part, w, lane);
+    const double sc_o = __shfl_xor(sc, m), al_o = __shfl_xor(al, m);
+    const double gam = (sc * sc_o) * gp;
+    const double ap = low ? al : al_o, aq = low ? al_o : al;
+    double t = 0.0, c = 1.0;
+    if (gam != 0.0) {
+        const double z = (aq - ap) / (2.0 * gam);
+        t = (z >= 0.0 ? 1.0 : -1.0) / (fabs(z) + sqrt(z * z + 1.0));
+        c = 1.0 / sqrt(t * t + 1.0);
+    }
+    const double coef = (low ? -t : t) * (sc_o / sc);
+#pragma unroll
+    for (int i = 0; i < H; ++i) u[i] = fma(coef, o[i], u[i]);
+    sc *= c;
+    al = low ? ap - t * gam : aq + t * gam;
+    const double den = ap * aq;
+    return den > 0.0 ? fabs(gam) / sqrt(den) : 0.0;
+}
+
+// fold the column scale into the wave's rows and refresh the squared column norm
+template <int H>
+__device__ inline void renorm_split(double (&u)[H], double& sc, double& al, int lane, int w, double* buf) {
+    double a4[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int i = 0; i < H; ++i) {
+        u[i] *= sc;
+        a4[i & 3] = fma(u[i], u[i], a4[i & 3]);
+    }
+    sc = 1.0;
+    al = pair_sum((a4[0] + a4[1]) + (a4[2] + a4[3]), buf, w, lane);
+}
+
 template <int N, bool SWEEP64>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SWEEP64 ? 1 : 2))) void k_mixed_finish(
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3))) void k_mixed_finish(
     const float* __restrict__ ws_u, const float* __restrict__ ws_b, float* __restrict__ f, MixedParams mp,
     double* __restrict__ dump, int* __restrict__ status) {
-    __shared__ __align__(16) unsigned char smem[FinishLds<N>::BYTES];
+    constexpr int H = FinishLds<N>::H, TQ = FinishLds<N>::TQ, TS = FinishLds<N>::T_STRIDE;
+    __shared__ double xpart[3][2 * 64];  // pair-sum buffers: two alternating rounds + norms/h
     __shared__ double bvec[N];
     __shared__ double xbar[Q];
+    __shared__ double Tq[N * TS];        // a TQ-column chunk of V, row-major
+    __shared__ double Zp[2][Q * N];      // each wave's part of s_i (dX V)[q][i]
+    __shared__ float dXs[Q * N];
+    __shared__ double hv[N];
 
     const int g = xcd_order(blockIdx.x, gridDim.x);
-    const int lane = threadIdx.x;
+    const int w = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
     const int M = mp.M;
+    const int l = lane & (N - 1);
+    const int r0 = w * H;
 
-    double vd[N];
+    double ud[H];
     {
         const float* ug = ws_u + (long long)g * N * N;
-        const int l = lane & (N - 1);
 #pragma unroll
-        for (int i = 0; i < N; ++i) {
-            const float x = ug[i * N + l];
-            vd[i] = (lane < N) ? double(x) : 0.0;
+        for (int i = 0; i < H; ++i) {
+            const float x = ug[(r0 + i) * N + l];
+            ud[i] = (lane < N) ? double(x) : 0.0;
         }
-        if (lane < N) bvec[lane] = double(ws_b[(long long)g * N + lane]);
+        if (w == 0 && lane < N) bvec[lane] = double(ws_b[(long long)g * N + lane]);
     }
-    double scd = 1.0, ald = 0.0;
+    double sc = 1.0, al = 0.0;
 
     // ---- 4. FP64 sweeps on the widened columns ------------------------------------------
     if constexpr (SWEEP64) {
         int sw64 = 0;
         while (sw64 < mp.max64) {
-            hestenes_renorm<N, double>(vd, scd, ald);
+            renorm_split<H>(ud, sc, al, lane, w, xpart[2]);
             double mc = 0.0;
-            for (int m = 1; m < N; ++m) mc = fmax(mc, hestenes_round<N, double>(vd, scd, ald, m, lane));
-            mc = wave_max(mc);
+            for (int m = 1; m < N; ++m)
+                mc = fmax(mc, hestenes_round_split<H>(ud, sc, al, m, lane, w, xpart[m & 1]));
+            mc = wave_max(mc);  // identical in both waves, so both leave the loop together
             ++sw64;
             if (mc <= mp.tol64) break;
         }
-        if (lane == 0 && mp.sweeps64) atomicAdd(mp.sweeps64, (unsigned long long)sw64);
+        if (threadIdx.x == 0 && mp.sweeps64) atomicAdd(mp.sweeps64, (unsigned long long)sw64);
     }
     // eigenpairs: d = ||u||, v = u / d  (U = C V with orthogonal columns)
-    hestenes_renorm<N, double>(vd, scd, ald);
-    const double lam = (lane < N) ? sqrt(ald) : 1.0;
+    __syncthreads();
+    renorm_split<H>(ud, sc, al, lane, w, xpart[2]);
+    const double lam = (lane < N) ? sqrt(al) : 1.0;
     {
         const double inv = 1.0 / lam;
 #pragma unroll
-        for (int i = 0; i < N; ++i) vd[i] *= inv;
+        for (int i = 0; i < H; ++i) ud[i] *= inv;
     }
-    __syncthreads();  // bvec
 
     // ---- 5. assembly in FP64 --------------------------------------------------------
     // column layout (lane = eigen-index i): d_i, h_i = (V^T b)_i / d_i, s_i = sqrt(M-1)/sqrt(d_i)
-    double d_i = 1.0, h_i = 0.0, s_i = 0.0;
-    if (lane < N) {
-        d_i = lam;
-        double gs = 0.0;
+    double hpart = 0.0;
 #pragma unroll
-        for (int r = 0; r < N; ++r) gs += vd[r] * bvec[r];
-        h_i = gs / d_i;
-        s_i = mp.sqm1 / sqrt(d_i);
-    }
+    for (int r = 0; r < H; ++r) hpart += ud[r] * bvec[r0 + r];
+    __syncthreads();
+    const double hs = pair_sum(hpart, xpart[2], w, lane);
+    const double d_i = lam;
+    const double h_i = (lane < N) ? hs / d_i : 0.0;
+    const double s_i = (lane < N) ? mp.sqm1 / sqrt(d_i) : 0.0;
     const bool dumping = (g >= mp.dump_g0) && (g < mp.dump_g0 + mp.dump_n);
     if (dumping && lane < M) {
         double* dg = dump + (long long)(g - mp.dump_g0) * (M * M + 2 * M);
 #pragma unroll
-        for (int k = 0; k < N; ++k)  // static indices keep vd[] in registers
-            if (k < M) dg[k * M + lane] = vd[k];  // V[k][lane]
-        dg[M * M + lane] = d_i;
-        dg[M * M + M + lane] = bvec[lane];
+        for (int k = 0; k < H; ++k)  // static indices keep ud[] in registers
+            if (r0 + k < M) dg[(r0 + k) * M + lane] = ud[k];  // V[r0+k][lane]
+        if (w == 0) {
+            dg[M * M + lane] = d_i;
+            dg[M * M + M + lane] = bvec[lane];
+        }
     }
-    constexpr int TQ = FinishLds<N>::TQ, TS = FinishLds<N>::T_STRIDE;
-    double* Tq = reinterpret_cast<double*>(smem);       // [N][TS] chunk of V columns
-    double* Zs = Tq + N * TS;                            // [Q][N] s_i (dX V)[q][i]
-    float* dXs = reinterpret_cast<float*>(Zs + Q * N);   // [Q][N]
-    double* hv = reinterpret_cast<double*>(dXs + Q * N); // [N]
-    // members: row layout, lane = member l
-    float xk[Q];
+    // members: row layout in wave 0, lane = member
+    float xk[Q], dxk[Q];
     const int member = lane < M ? lane : 0;
     const long long base = (long long)member * mp.member_stride + g;
 #pragma unroll
     for (int q = 0; q < Q; ++q) xk[q] = 0.f;
-    if (lane < M) {
+    if (w == 0 && lane < M) {
 #pragma unroll
         for (int q = 0; q < Q; ++q) xk[q] = f[base + (long long)q * mp.nodes];
     }
+    if (w == 0) {
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        const double s = wave_sum_d(double(xk[q]));
-        if (lane == 0) xbar[q] = s / double(M);
+        for (int q = 0; q < Q; ++q) {
+            const double s = wave_sum_d(double(xk[q]));
+            if (lane == 0) xbar[q] = s / double(M);
+        }
+        if (lane < N) hv[lane] = h_i;
     }
-    if (lane < N) hv[lane] = h_i;
     __syncthreads();
-    float dxk[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         dxk[q] = (lane < M) ? float(double(xk[q]) - xbar[q]) : 0.f;
-        if (lane < N) dXs[q * N + lane] = dxk[q];
+        if (w == 0 && lane < N) dXs[q * N + lane] = dxk[q];
     }
     __syncthreads();
-    // Z'[q][i] = s_i sum_r dX[q][r] V[r][i]   (column layout)
+    // Z'[q][i] = s_i sum_r dX[q][r] V[r][i], each wave over its rows  (column layout)
     if (lane < N) {
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
             double z = 0.0;
 #pragma unroll
-            for (int r = 0; r < N; ++r) z += double(dXs[q * N + r]) * vd[r];
-            Zs[q * N + lane] = s_i * z;
+            for (int r = 0; r < H; ++r) z += double(dXs[q * N + r0 + r]) * ud[r];
+            Zp[w][q * N + lane] = s_i * z;
         }
     }
-    // dXa[q][l] = sum_i Z'[q][i] V[l][i],  w_l = sum_i V[l][i] h_i   (row layout)
+    __syncthreads();
+    if (w == 0 && lane < N) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) Zp[0][q * N + lane] += Zp[1][q * N + lane];
+    }
+    // dXa[q][l] = sum_i Z'[q][i] V[l][i],  w_l = sum_i V[l][i] h_i   (row layout, wave 0)
     double dxa[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) dxa[q] = 0.0;
@@ -440,31 +506,33 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SWEEP64 ? 1 
         __syncthreads();
         if (lane >= c0 && lane < c0 + TQ) {
 #pragma unroll
-            for (int r = 0; r < N; ++r) Tq[r * TS + (lane - c0)] = vd[r];
+            for (int r = 0; r < H; ++r) Tq[(r0 + r) * TS + (lane - c0)] = ud[r];
         }
         __syncthreads();
-        if (lane < N) {
-#pragma unroll
+        if (w == 0 && lane < N) {
+#pragma unroll 2
             for (int ii = 0; ii < TQ; ++ii) {
                 const double vli = Tq[lane * TS + ii];  // V[lane][c0+ii]
                 wl += vli * hv[c0 + ii];
 #pragma unroll
-                for (int q = 0; q < Q; ++q) dxa[q] += Zs[q * N + c0 + ii] * vli;
+                for (int q = 0; q < Q; ++q) dxa[q] += Zp[0][q * N + c0 + ii] * vli;
             }
         }
     }
     // Xa = xbar + dXa + (dX w) 1^T
-    bool bad = false;
+    if (w == 0) {
+        bool bad = false;
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        const double dxw = wave_sum_d((lane < M) ? double(dxk[q]) * wl : 0.0);
-        if (lane < M) {
-            const float xa = float(xbar[q] + dxa[q] + dxw);
-            bad |= !isfinite(xa);
-            f[base + (long long)q * mp.nodes] = xa;
+        for (int q = 0; q < Q; ++q) {
+            const double dxw = wave_sum_d((lane < M) ? double(dxk[q]) * wl : 0.0);
+            if (lane < M) {
+                const float xa = float(xbar[q] + dxa[q] + dxw);
+                bad |= !isfinite(xa);
+                f[base + (long long)q * mp.nodes] = xa;
+            }
         }
+        if (bad) atomicOr(status, 1);
     }
-    if (bad) atomicOr(status, 1);
 }
 
 }  // namespace
@@ -501,17 +569,17 @@ hipError_t launch_letkf_mixed(Ctx& c, float* f, unsigned long long* sweeps32, un
     mp.sweeps32 = sweeps32;
     mp.sweeps64 = sweeps64;
     mp.chain = c.mix_chain;
-    const dim3 grid(unsigned(c.nodes)), block(64);
+    const dim3 grid(unsigned(c.nodes)), block(64), block2(128);
     const dim3 grid_solve(unsigned((c.nodes + mp.chain - 1) / mp.chain));
     const bool sweep64 = mp.max64 > 0;
 #define LAUNCH_SOLVE(NN, PP)                                                                           \
     k_mixed_solve<NN, PP><<<grid_solve, block, 0, c.stream>>>(c.d_loc_ptr, c.d_loc_site, c.d_loc_w, c.d_dy, \
                                                          c.d_dyo, c.d_mix_u, c.d_mix_b, mp)
 #define LAUNCH_FINISH(NN)                                                                                     \
-    (sweep64 ? k_mixed_finish<NN, true><<<grid, block, 0, c.stream>>>(c.d_mix_u, c.d_mix_b, f, mp, c.d_dump,  \
-                                                                       c.d_status)                            \
-             : k_mixed_finish<NN, false><<<grid, block, 0, c.stream>>>(c.d_mix_u, c.d_mix_b, f, mp, c.d_dump, \
-                                                                        c.d_status))
+    (sweep64 ? k_mixed_finish<NN, true><<<grid, block2, 0, c.stream>>>(c.d_mix_u, c.d_mix_b, f, mp, c.d_dump,  \
+                                                                        c.d_status)                             \
+             : k_mixed_finish<NN, false><<<grid, block2, 0, c.stream>>>(c.d_mix_u, c.d_mix_b, f, mp, c.d_dump, \
+                                                                         c.d_status))
     switch (N * 10 + (passes == 3 ? 3 : 1)) {
         case 163: LAUNCH_SOLVE(16, 3); break;
         case 161: LAUNCH_SOLVE(16, 1); break;

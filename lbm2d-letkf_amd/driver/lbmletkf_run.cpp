@@ -1,0 +1,311 @@
+// OSSE front-end on the C-ABI: the reference's run modes (src/main.cu:22-150,
+// src/config.h:21-150) with its raw output layout (src/io.cu:22-258).
+//
+//   lbmletkf_run nature   [options]   nature run, writes io/nature/ens0000/*
+//   lbmletkf_run observe  [options]   nature run + noisy observations io/observed/ens0000/*
+//   lbmletkf_run da_letkf [options]   ensemble with LETKF analyses read from io/observed
+//   lbmletkf_run free     [options]   the same ensemble without analyses (DA_DUMMY)
+//
+// Main loop, as the reference's: for tt = 0 .. outputs, t = tt*iiter:
+//   analysis if t > 0 and (t/iiter) % daprune == 0      (src/letkf.cu:120-123)
+//   output   if t % (iiter*ioprune) == 0                (src/io.cu:22-23)
+//   iiter LBM steps of every member
+//
+// Options:
+//   --preset ref|baseline   configuration preset (default ref)
+//   --members M             ensemble size for da_letkf/free (default 8)
+//   --ens-offset K          ensemble id of the first member (default 1; the nature run is id 0)
+//   --prefix DIR            output root (default io)
+//   --outputs N             number of output intervals in the main loop (default 200)
+//   --iiter n               LBM steps per output interval (default da_interval / daprune)
+//   --ioprune n             write every n-th interval (default 10)
+//   --daprune n             analyse every n-th interval (default 10)
+//   --spinup n              LBM steps before the main loop (default 0)
+//   --nx n                  grid size (square), overriding the preset
+//   --variant plain|mixed   LETKF variant (default: the preset's)
+//   --obs-stride p          observation lattice stride (DA_XYPRUNE)
+//   --loc-scale L           Gaussian localization scale (cells)
+//   --loc-cutoff d          Gaussian localization cutoff (cells)
+//   --inflation b           multiplicative covariance inflation (LETKF_COVINF)
+//   --obs-seed S            seed of the observation noise (default 334, src/io.cu:162)
+//   --no-nan                keep values at unobserved nodes (default: NaN, OBS_XYPRUNE_NAN)
+//   --stats                 also write the ensemble mean/std files (ENSEMBLE_STAT)
+//   --timing                print the phase timers at the end
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/lbmletkf.h"
+#include "datio.hpp"
+
+namespace {
+
+using lbmletkf_io::ens_dir;
+using lbmletkf_io::make_dirs;
+using lbmletkf_io::read_field;
+using lbmletkf_io::step_file;
+using lbmletkf_io::write_field;
+
+struct Options {
+    std::string mode;
+    std::string preset = "ref";
+    int members = 8;
+    int ens_offset = 1;
+    std::string prefix = "io";
+    int outputs = 200;
+    int iiter = 0;
+    int ioprune = 10;
+    int daprune = 10;
+    long long spinup = 0;
+    int nx = 0;
+    int variant = -1;
+    int obs_stride = 0;
+    double loc_scale = 0, loc_cutoff = 0, inflation = 0;
+    unsigned obs_seed = 334;
+    bool nan_unobserved = true;
+    bool stats = false;
+    bool timing = false;
+};
+
+[[noreturn]] void usage(const char* msg) {
+    std::fprintf(stderr, "lbmletkf_run: %s\n", msg);
+    std::fprintf(stderr, "usage: lbmletkf_run nature|observe|da_letkf|free [--preset ref|baseline] [--members M]\n"
+                         "       [--ens-offset K] [--prefix DIR] [--outputs N] [--iiter n] [--ioprune n]\n"
+                         "       [--daprune n] [--spinup n] [--nx n] [--variant plain|mixed] [--obs-stride p]\n"
+                         "       [--loc-scale L] [--loc-cutoff d] [--inflation b] [--obs-seed S] [--no-nan]\n"
+                         "       [--stats] [--timing]\n");
+    std::exit(2);
+}
+
+Options parse(int argc, char** argv) {
+    if (argc < 2) usage("missing mode");
+    Options o;
+    o.mode = argv[1];
+    if (o.mode != "nature" && o.mode != "observe" && o.mode != "da_letkf" && o.mode != "free")
+        usage(("unknown mode " + o.mode).c_str());
+    for (int a = 2; a < argc; ++a) {
+        const std::string k = argv[a];
+        auto val = [&]() -> std::string {
+            if (a + 1 >= argc) usage(("missing value for " + k).c_str());
+            return argv[++a];
+        };
+        if (k == "--preset") o.preset = val();
+        else if (k == "--members") o.members = std::stoi(val());
+        else if (k == "--ens-offset") o.ens_offset = std::stoi(val());
+        else if (k == "--prefix") o.prefix = val();
+        else if (k == "--outputs") o.outputs = std::stoi(val());
+        else if (k == "--iiter") o.iiter = std::stoi(val());
+        else if (k == "--ioprune") o.ioprune = std::stoi(val());
+        else if (k == "--daprune") o.daprune = std::stoi(val());
+        else if (k == "--spinup") o.spinup = std::stoll(val());
+        else if (k == "--nx") o.nx = std::stoi(val());
+        else if (k == "--variant") {
+            const std::string v = val();
+            if (v == "plain") o.variant = LBMLETKF_VARIANT_PLAIN;
+            else if (v == "mixed") o.variant = LBMLETKF_VARIANT_MIXED;
+            else usage("variant must be plain or mixed");
+        } else if (k == "--obs-stride") o.obs_stride = std::stoi(val());
+        else if (k == "--loc-scale") o.loc_scale = std::stod(val());
+        else if (k == "--loc-cutoff") o.loc_cutoff = std::stod(val());
+        else if (k == "--inflation") o.inflation = std::stod(val());
+        else if (k == "--obs-seed") o.obs_seed = unsigned(std::stoul(val()));
+        else if (k == "--no-nan") o.nan_unobserved = false;
+        else if (k == "--stats") o.stats = true;
+        else if (k == "--timing") o.timing = true;
+        else usage(("unknown option " + k).c_str());
+    }
+    if (o.members < 1 || o.outputs < 0 || o.ioprune < 1 || o.daprune < 1 || o.spinup < 0)
+        usage("counts must be positive");
+    return o;
+}
+
+// a context with the error text of the library on failure
+struct Ctx {
+    lbmletkf_ctx* h = nullptr;
+    Ctx(const lbmletkf_config& cfg, int m) {
+        if (int st = lbmletkf_create(&cfg, m, &h))
+            throw std::runtime_error("lbmletkf_create failed (" + std::to_string(st) + "): " +
+                                     lbmletkf_last_error(nullptr));
+    }
+    ~Ctx() { lbmletkf_destroy(h); }
+    void ck(int st, const char* what) const {
+        if (st) throw std::runtime_error(std::string(what) + " failed (" + std::to_string(st) + "): " +
+                                         lbmletkf_last_error(h));
+    }
+};
+
+struct Fields {
+    std::vector<float> r, u, v, vor;
+    explicit Fields(size_t n) : r(n), u(n), v(n), vor(n) {}
+};
+
+// macroscopic fields and vorticity of one member
+void member_fields(const Ctx& c, int m, int nx, int ny, float dx, Fields& f) {
+    c.ck(lbmletkf_get_macro(c.h, m, f.r.data(), f.u.data(), f.v.data()), "get_macro");
+    lbmletkf_io::vorticity(f.u.data(), f.v.data(), nx, ny, dx, f.vor.data());
+}
+
+void write_fields(const std::string& dir, long long step, const Fields& f) {
+    make_dirs(dir);
+    const size_t n = f.r.size();
+    write_field(step_file(dir, "rho", step), f.r.data(), n);
+    write_field(step_file(dir, "u", step), f.u.data(), n);
+    write_field(step_file(dir, "v", step), f.v.data(), n);
+    write_field(step_file(dir, "vor", step), f.vor.data(), n);
+}
+
+// Observed fields of the reference's OBSERVE mode with OBSERVE_ERROR_RUV and
+// OBS_XYPRUNE_NAN (src/io.cu:158-235): per node in order rho, u, v, each truth plus
+// sigma times a draw of std::normal_distribution<float>(0, 1) on one
+// std::mt19937(seed) engine that lives for the whole run; vorticity from the noisy
+// velocities; NaN off the observation lattice (i or j not a multiple of the stride).
+void observe_fields(std::mt19937& eng, std::normal_distribution<float>& nd, const Fields& truth,
+                    const lbmletkf_config& cfg, bool nan_off_lattice, Fields& obs) {
+    const int nx = cfg.nx, ny = cfg.ny;
+    lbmletkf_io::add_obs_noise(eng, nd, truth.r.data(), truth.u.data(), truth.v.data(), truth.r.size(),
+                               float(cfg.obs_sigma_rho), float(cfg.obs_sigma_u), obs.r.data(), obs.u.data(),
+                               obs.v.data());
+    lbmletkf_io::vorticity(obs.u.data(), obs.v.data(), nx, ny, float(cfg.dx), obs.vor.data());
+    if (!nan_off_lattice) return;
+    const int p = cfg.obs_stride, off = cfg.obs_offset;
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i)
+            if ((i - off) % p != 0 || (j - off) % p != 0 || i < off || j < off) {
+                const size_t ij = size_t(i) + size_t(nx) * j;
+                obs.r[ij] = obs.u[ij] = obs.v[ij] = obs.vor[ij] = NAN;
+            }
+}
+
+// ensemble mean of u, v and mean / standard deviation of the vorticity (src/io.cu:90-145)
+struct EnsembleStat {
+    std::vector<double> su, sv, svor, svor2;
+    int count = 0;
+    explicit EnsembleStat(size_t n) : su(n), sv(n), svor(n), svor2(n) {}
+    void add(const Fields& f) {
+        for (size_t i = 0; i < su.size(); ++i) {
+            su[i] += f.u[i];
+            sv[i] += f.v[i];
+            svor[i] += f.vor[i];
+            svor2[i] += double(f.vor[i]) * f.vor[i];
+        }
+        ++count;
+    }
+    void write(const std::string& prefix, long long step) const {
+        const size_t n = su.size();
+        std::vector<float> mu(n), mv(n), mvor(n), sdvor(n);
+        for (size_t i = 0; i < n; ++i) {
+            mu[i] = float(su[i] / count);
+            mv[i] = float(sv[i] / count);
+            const double m = svor[i] / count;
+            mvor[i] = float(m);
+            const double var = svor2[i] / count - m * m;
+            sdvor[i] = float(var > 0 ? std::sqrt(var) : 0.0);
+        }
+        const std::string d = prefix + "/calc";
+        make_dirs(d);
+        write_field(step_file(d, "ens_mean_u", step), mu.data(), n);
+        write_field(step_file(d, "ens_mean_v", step), mv.data(), n);
+        write_field(step_file(d, "ens_vor_mean", step), mvor.data(), n);
+        write_field(step_file(d, "ens_vor_std", step), sdvor.data(), n);
+    }
+};
+
+int run(const Options& o) {
+    lbmletkf_config cfg;
+    if (lbmletkf_config_preset(o.preset.c_str(), &cfg)) throw std::runtime_error(lbmletkf_last_error(nullptr));
+    if (o.nx > 0) cfg.nx = cfg.ny = o.nx;
+    if (o.variant >= 0) cfg.variant = o.variant;
+    if (o.obs_stride > 0) cfg.obs_stride = o.obs_stride;
+    if (o.loc_scale > 0) cfg.loc_scale = o.loc_scale;
+    if (o.loc_cutoff > 0) cfg.loc_cutoff = o.loc_cutoff;
+    if (o.inflation > 0) cfg.inflation = o.inflation;
+    const int iiter = o.iiter > 0 ? o.iiter : std::max(1, cfg.da_interval / o.daprune);
+    cfg.da_interval = iiter * o.daprune;
+    const bool ensemble = o.mode == "da_letkf" || o.mode == "free";
+    const int M = ensemble ? o.members : 1;
+    const int first_id = ensemble ? o.ens_offset : 0;
+    if (int st = lbmletkf_config_check(&cfg, M)) {
+        throw std::runtime_error("configuration rejected (" + std::to_string(st) + "): " + lbmletkf_last_error(nullptr));
+    }
+    const int nx = cfg.nx, ny = cfg.ny;
+    const size_t n = size_t(nx) * ny;
+    std::printf("lbmletkf_run %s: preset %s, %dx%d, %d member(s) from id %d, iiter %d, outputs %d, ioprune %d, "
+                "daprune %d, %s\n",
+                o.mode.c_str(), o.preset.c_str(), nx, ny, M, first_id, iiter, o.outputs, o.ioprune, o.daprune,
+                lbmletkf_version());
+
+    Ctx c(cfg, M);
+    for (int m = 0; m < M; ++m) c.ck(lbmletkf_init_member(c.h, m, (unsigned long long)(first_id + m)), "init_member");
+    if (o.spinup > 0) {
+        std::printf("spin-up: %lld steps\n", o.spinup);
+        c.ck(lbmletkf_forecast(c.h, int(o.spinup)), "forecast");
+        c.ck(lbmletkf_set_step(c.h, 0), "set_step");
+    }
+    if (o.timing) c.ck(lbmletkf_profile_enable(c.h, 1), "profile_enable");
+
+    std::mt19937 eng(o.obs_seed);
+    std::normal_distribution<float> nd(0.f, 1.f);
+    Fields f(n), obs(n);
+    int n_obs = 0, n_sites = 0;
+    if (o.mode == "da_letkf") c.ck(lbmletkf_obs_count(c.h, &n_sites, &n_obs), "obs_count");
+    std::vector<float> yo(size_t(std::max(n_obs, 1)));
+
+    for (int tt = 0; tt <= o.outputs; ++tt) {
+        const long long t = (long long)tt * iiter;
+        const long long step = t / iiter;
+        if (o.mode == "da_letkf" && t > 0 && step % o.daprune == 0) {
+            const std::string od = ens_dir(o.prefix, "observed", 0);
+            read_field(step_file(od, "rho", step), obs.r.data(), n);
+            read_field(step_file(od, "u", step), obs.u.data(), n);
+            read_field(step_file(od, "v", step), obs.v.data(), n);
+            c.ck(lbmletkf_obs_from_fields(c.h, obs.r.data(), obs.u.data(), obs.v.data(), yo.data()), "obs_from_fields");
+            c.ck(lbmletkf_analysis(c.h, yo.data(), 0), "analysis");
+            std::printf("t=%lld: analysis with %d observations\n", t, n_obs);
+        }
+        if (t % ((long long)iiter * o.ioprune) == 0) {
+            EnsembleStat es(o.stats ? n : 0);
+            for (int m = 0; m < M; ++m) {
+                member_fields(c, m, nx, ny, float(cfg.dx), f);
+                if (ensemble) {
+                    write_fields(ens_dir(o.prefix, "calc", first_id + m), step, f);
+                    if (o.stats) es.add(f);
+                } else {
+                    write_fields(ens_dir(o.prefix, "nature", 0), step, f);
+                    if (o.mode == "observe") {
+                        observe_fields(eng, nd, f, cfg, o.nan_unobserved, obs);
+                        write_fields(ens_dir(o.prefix, "observed", 0), step, obs);
+                    }
+                }
+            }
+            if (ensemble && o.stats) es.write(o.prefix, step);
+            std::printf("t=%lld: output step %lld\n", t, step);
+        }
+        if (tt < o.outputs) c.ck(lbmletkf_forecast(c.h, iiter), "forecast");
+    }
+    c.ck(lbmletkf_synchronize(c.h), "synchronize");
+    if (o.timing) {
+        for (const char* ph : {"forecast", "obs", "gram", "solve", "finish"}) {
+            double sec = 0;
+            long long launches = 0, work = 0;
+            c.ck(lbmletkf_profile_read(c.h, ph, &sec, &launches, &work), "profile_read");
+            std::printf("timer %-8s %10.3f ms  launches %lld\n", ph, sec * 1e3, launches);
+        }
+    }
+    return 0;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    try {
+        return run(parse(argc, argv));
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "lbmletkf_run: %s\n", e.what());
+        return 1;
+    }
+}
