@@ -46,6 +46,10 @@ extern "C" {
 /* localization kinds */
 #define LBMLETKF_LOC_GC_WINDOW       0   /* reference window + Gaspari-Cohn R-localization, src/letkf.cu:44-56,97-113 */
 #define LBMLETKF_LOC_GAUSS           1   /* exp(-d^2 / 2L^2) on periodic distance, hard cutoff */
+/* eddy-viscosity closures: Smagorinsky (src/lbm.cu:24-65) or the learned per-node MLP */
+#define LBMLETKF_CLOSURE_SMAGORINSKY 0
+#define LBMLETKF_CLOSURE_MLP         1
+
 /* analysis variants */
 #define LBMLETKF_VARIANT_PLAIN       0   /* per-member LBM launches, FP32 Gram on VALU, FP64 Jacobi */
 #define LBMLETKF_VARIANT_MIXED       1   /* batched LBM + fused LETKF, mixed precision */
@@ -91,6 +95,11 @@ typedef struct lbmletkf_config {
     int    jacobi_max_sweeps;   /* cusolver syevj uses tol 1e-6, 100 sweeps (eigensolver_wrapper.hpp:192-193) */
     int    gram_passes;         /* MIXED: 1 = bf16 operands, 3 = split bf16 (hi*hi + hi*lo + lo*hi) */
     int    jacobi_fp64_sweeps;  /* MIXED: at most this many FP64 sweeps after the FP32 phase */
+    /* eddy-viscosity closure of the fused LBM kernel (MIXED variant) */
+    int    closure_kind;         /* LBMLETKF_CLOSURE_*                                       */
+    double closure_strain_scale; /* MLP inputs are strain components / this scale            */
+    double closure_nu_scale;     /* MLP output nu_t = this scale * softplus(z)               */
+    double closure_nu_max;       /* nu_t above this (or not finite) falls back to Smagorinsky */
 } lbmletkf_config;
 
 typedef struct lbmletkf_ctx lbmletkf_ctx;
@@ -212,6 +221,63 @@ int lbmletkf_get_dump(lbmletkf_ctx* ctx, double* W, double* w, double* evals);
 int lbmletkf_profile_enable(lbmletkf_ctx* ctx, int on);
 int lbmletkf_profile_read(lbmletkf_ctx* ctx, const char* phase, double* seconds,
                           long long* launches, long long* work);
+
+/* ---- learned eddy-viscosity closure ------------------------------------------
+ * No counterpart in the reference (its LES model is Smagorinsky only,
+ * src/lbm.cu:24-65): a per-node network 18 -> 64 -> 64 -> 1 (tanh-form GELU,
+ * softplus output) maps the 3x3 stencil of the streamed velocity's strain
+ * (a = (ux - vy)/2, b = (uy + vx)/2, divided by closure_strain_scale; stencil point
+ * s = (dj+1)*3 + (di+1) gives inputs 2s, 2s+1) to nu_t = closure_nu_scale *
+ * softplus(z).  Parameters are one FP32 vector of LBMLETKF_CLOSURE_NPARAM values:
+ * W1[64][18], b1[64], W2[64][64], b2[64], w3[64], b3. */
+#define LBMLETKF_CLOSURE_NPARAM 5441
+/* training sample: 18 inputs, a, b (unscaled), t1 = (tau11 - tau22)/2, t2 = tau12 */
+#define LBMLETKF_CLOSURE_SAMPLE 22
+
+/* Initial parameters from a seed (Glorot-uniform weights, zero biases, output
+ * layer scaled by 0.1 and b3 = softplus^-1(1): nu_t ~ closure_nu_scale). */
+int lbmletkf_closure_init_params(unsigned long long seed, float* params);
+
+/* Load the network into a context created with closure_kind = MLP; every later
+ * LBM step of that context (forecast, cycle) runs the fused kernel with it
+ * (weights in bf16, FP32 accumulation).  Nodes where the network's nu_t is not
+ * finite or outside [0, closure_nu_max] use the Smagorinsky nu_t.  Requires nx a
+ * multiple of 64 and ny of 16. */
+int lbmletkf_set_closure(lbmletkf_ctx* ctx, const float* params);
+
+/* nu_t (after the fallback) that the next step would use for one member, [j][i]
+ * host array: the same kernel as the forecast, its populations discarded. */
+int lbmletkf_closure_nut(lbmletkf_ctx* ctx, int member, float* nut);
+
+/* Box filter of one member by an integer factor: rho_bar, u_bar, v_bar on the
+ * (nx/factor) x (ny/factor) grid, host arrays [J][I]. */
+int lbmletkf_filtered_macro(lbmletkf_ctx* fine, int member, int factor, float* rho, float* u, float* v);
+
+/* Trainer: samples in device memory, parameters and Adam state, the forward and
+ * backward kernels.  coarse_cfg describes the coarse grid (dx, closure scales). */
+typedef struct lbmletkf_trainer lbmletkf_trainer;
+int lbmletkf_trainer_create(const lbmletkf_config* coarse_cfg, long long capacity, lbmletkf_trainer** out);
+int lbmletkf_trainer_destroy(lbmletkf_trainer* tr);
+/* Append the (nx/factor)*(ny/factor) samples of one member of a fine-grid context:
+ * box-filtered velocity and the subgrid stress tau_ij = <u_i u_j> - <u_i><u_j>. */
+int lbmletkf_trainer_add_samples(lbmletkf_trainer* tr, lbmletkf_ctx* fine, int member, int factor);
+int lbmletkf_trainer_set_samples(lbmletkf_trainer* tr, const float* samples, long long n);
+int lbmletkf_trainer_get_samples(lbmletkf_trainer* tr, long long first, long long n, float* samples);
+int lbmletkf_trainer_count(lbmletkf_trainer* tr, long long* n);
+/* set_params also clears the Adam moments */
+int lbmletkf_trainer_set_params(lbmletkf_trainer* tr, const float* params);
+int lbmletkf_trainer_get_params(lbmletkf_trainer* tr, float* params);
+/* Mean loss w_norm ((2 nu_t a + t1)^2 + (2 nu_t b + t2)^2) over samples
+ * [first, first+n) and its gradient (NPARAM floats, may be NULL); the loss is the
+ * model error of tau_dev = -2 nu_t S_dev.  w_norm <= 0 takes 1 / mean(t1^2 + t2^2)
+ * over all samples. */
+int lbmletkf_trainer_loss_grad(lbmletkf_trainer* tr, long long first, long long n, double w_norm, double* loss,
+                               float* grad);
+/* steps Adam updates (beta 0.9, 0.999, eps 1e-8) on minibatches walking a
+ * permutation of the samples; loss_history (steps floats, may be NULL) receives
+ * each minibatch's loss before its update. */
+int lbmletkf_trainer_train(lbmletkf_trainer* tr, int batch, int steps, double lr, double w_norm,
+                           float* loss_history);
 
 /* Stream access for callers that time or order work around the context. */
 void* lbmletkf_get_stream(lbmletkf_ctx* ctx);

@@ -29,6 +29,14 @@ int fail(Ctx* c, int code, const std::string& msg) {
     return code;
 }
 
+}  // namespace
+
+namespace lbmletkf {
+int report_error(Ctx* c, int code, const std::string& msg) { return fail(c, code, msg); }
+}  // namespace lbmletkf
+
+namespace {
+
 #define HIP_TRY(ctx, expr)                                                              \
     do {                                                                                \
         hipError_t e_ = (expr);                                                         \
@@ -216,6 +224,13 @@ int check_config(const lbmletkf_config* cfg, int n_members, std::string& why) {
     if (!(cfg->jacobi_tol > 0) || cfg->jacobi_max_sweeps < 1) return bad("jacobi_tol > 0 and jacobi_max_sweeps >= 1 required");
     if (cfg->gram_passes != 1 && cfg->gram_passes != 3) return bad("gram_passes must be 1 or 3");
     if (cfg->jacobi_fp64_sweeps < 0 || cfg->jacobi_fp64_sweeps > 20) return bad("jacobi_fp64_sweeps must be in [0, 20]");
+    if (cfg->closure_kind != LBMLETKF_CLOSURE_SMAGORINSKY && cfg->closure_kind != LBMLETKF_CLOSURE_MLP)
+        return bad("unknown closure_kind");
+    if (cfg->closure_kind == LBMLETKF_CLOSURE_MLP &&
+        !(cfg->closure_strain_scale > 0 && cfg->closure_nu_scale > 0 && cfg->closure_nu_max > 0))
+        return bad("the MLP closure needs positive strain, nu and nu_max scales");
+    if (cfg->closure_kind == LBMLETKF_CLOSURE_MLP && (cfg->nx % 64 || cfg->ny % 16))
+        return bad("the MLP closure needs nx a multiple of 64 and ny a multiple of 16");
     return LBMLETKF_OK;
 }
 
@@ -387,13 +402,17 @@ int prof_collect(Ctx* c) {
 // n LBM steps of all members (src/lbm.h:16-29 applied n times): stream-ordered
 // launches only, so a cycle can be captured into a graph.  Advances cur and step.
 int forecast_launches(Ctx* c, int n_steps) {
-    const bool fused = c->cfg.variant == LBMLETKF_VARIANT_MIXED && lbm_fused_supported(c->nx, c->ny);
+    const bool mlp = c->cfg.closure_kind == LBMLETKF_CLOSURE_MLP;
+    if (mlp && !c->closure_ready) return fail(c, LBMLETKF_ERR_STATE, "closure_kind is MLP: call lbmletkf_set_closure first");
+    const bool fused = mlp || (c->cfg.variant == LBMLETKF_VARIANT_MIXED && lbm_fused_supported(c->nx, c->ny));
     hipEvent_t ev = prof_begin(c);
     for (int s = 0; s < n_steps; ++s) {
         if (c->lp.has_force) HIP_TRY(c, launch_force_field(*c, s));
         const float* fin = c->f[c->cur];
         float* fout = c->f[1 - c->cur];
-        if (fused) {
+        if (mlp) {
+            HIP_TRY(c, launch_lbm_mlp(*c, fin, fout, 0, (c->nx / 64) * (c->ny / 16) * c->M, nullptr));
+        } else if (fused) {
             HIP_TRY(c, launch_lbm_fused(*c, fin, fout));
         } else {
             for (int m = 0; m < c->M; ++m) HIP_TRY(c, launch_lbm_plain_member(*c, fin, fout, m));
@@ -500,7 +519,7 @@ Ctx::~Ctx() {
     void* ptrs[] = {f[0], f[1], plain_ruv, plain_nus, force_xy, d_force, d_step, d_status,
                     d_site_node, d_loc_ptr, d_loc_site, d_loc_w, d_obs_coef, d_yo, d_hx, d_hx1, d_dy,
                     d_dyo, d_gram, d_bvec, d_mix_u, d_mix_b, d_dump, d_red, d_truth, d_sweeps,
-                    d_sweeps64};
+                    d_sweeps64, d_closure};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (const ProfRec& r : prof_pending) {
@@ -560,6 +579,10 @@ int lbmletkf_config_preset(const char* name, lbmletkf_config* cfg) {
         c.jacobi_max_sweeps = 100;
         c.gram_passes = 3;
         c.jacobi_fp64_sweeps = 1;
+        c.closure_kind = LBMLETKF_CLOSURE_SMAGORINSKY;
+        c.closure_strain_scale = 1e-2 * c.c / c.dx;  // ~ u_rms k0 2pi/n at 256^2, k0 = 8
+        c.closure_nu_scale = 1e-4 * c.c * c.dx;      // Smagorinsky nu_t at that strain, Cs dx = 0.1 dx
+        c.closure_nu_max = 0.05 * c.c * c.dx;
     } else if (n == "baseline") {
         c.nx = c.ny = 256;
         c.c = 1.0;
@@ -598,6 +621,10 @@ int lbmletkf_config_preset(const char* name, lbmletkf_config* cfg) {
         c.jacobi_max_sweeps = 100;
         c.gram_passes = 3;
         c.jacobi_fp64_sweeps = 1;
+        c.closure_kind = LBMLETKF_CLOSURE_SMAGORINSKY;
+        c.closure_strain_scale = 1e-2 * c.c / c.dx;  // ~ u_rms k0 2pi/n at 256^2, k0 = 8
+        c.closure_nu_scale = 1e-4 * c.c * c.dx;      // Smagorinsky nu_t at that strain, Cs dx = 0.1 dx
+        c.closure_nu_max = 0.05 * c.c * c.dx;
     } else {
         return fail(nullptr, LBMLETKF_ERR_ARG, "unknown preset: " + n);
     }

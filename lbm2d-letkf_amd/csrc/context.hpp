@@ -50,6 +50,8 @@ struct ObsNet {
 // MIXED: PH_SOLVE = gather + Gram + FP32 sweeps, PH_FINISH = FP64 sweeps + assembly + update
 enum Phase { PH_FORECAST = 0, PH_OBS, PH_GRAM, PH_SOLVE, PH_FINISH, PH_COUNT };
 
+struct ClosureWeights;
+
 struct ProfRec {
     int phase;
     hipEvent_t a, b;
@@ -134,12 +136,23 @@ struct Ctx {
     hipGraphExec_t cycle_exec[2] = {nullptr, nullptr};
     void drop_graphs();
 
+    // learned closure (closure_kind = MLP): the network's weight tiles
+    ClosureWeights* d_closure = nullptr;
+    bool closure_ready = false;
+
     ~Ctx();
 };
+
+// status code + message for the last-error strings of the C-ABI (ctx may be null)
+int report_error(Ctx* c, int code, const std::string& msg);
 
 // ---- launchers (lbm.hip) ------------------------------------------------------
 hipError_t launch_force_field(Ctx& c, int step_offset);
 hipError_t launch_lbm_fused(Ctx& c, const float* fin, float* fout);
+// one step with the learned closure for tiles [tile0, tile0 + n_tiles) of the
+// 64x16 tiling (member-major); nut_out (may be NULL) receives nu_t of those
+// tiles' nodes, indexed by node within the member
+hipError_t launch_lbm_mlp(Ctx& c, const float* fin, float* fout, int tile0, int n_tiles, float* nut_out);
 hipError_t launch_lbm_plain_member(Ctx& c, const float* fin, float* fout, int member);
 hipError_t launch_set_feq(Ctx& c, int member, const float* d_rho, const float* d_u, const float* d_v);
 hipError_t launch_macro(Ctx& c, const float* f_member, float* rho, float* u, float* v);

@@ -16,100 +16,14 @@
 //   * the plain form keeps the reference's three passes (stream+macro, les,
 //     collide) and is launched member by member, as one reference rank would.
 #include "context.hpp"
+#include "lbm_node.hpp"
 #include "philox.hpp"
 
 namespace lbmletkf {
 
 namespace {
 
-constexpr float W0 = 4.f / 9.f, W1 = 1.f / 9.f, W2 = 1.f / 36.f;
-
-__device__ __forceinline__ int wrap(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
-
-// Pull streaming of the 9 populations into node (i,j): fc[q] = f[q](i-ex_q, j-ey_q)
-// with ex_q = q%3-1, ey_q = q/3-1 (src/config.h:256-258, src/lbm.cu:193-198).
-// im/ip are the wrapped columns i-1/i+1, rm/r0/rp the row offsets of j-1/j/j+1.
-__device__ __forceinline__ void pull9(const float* __restrict__ f, long long nodes, int im, int i,
-                                      int ip, long long rm, long long r0, long long rp,
-                                      float fc[Q]) {
-    fc[0] = f[0 * nodes + rp + ip];
-    fc[1] = f[1 * nodes + rp + i];
-    fc[2] = f[2 * nodes + rp + im];
-    fc[3] = f[3 * nodes + r0 + ip];
-    fc[4] = f[4 * nodes + r0 + i];
-    fc[5] = f[5 * nodes + r0 + im];
-    fc[6] = f[6 * nodes + rm + ip];
-    fc[7] = f[7 * nodes + rm + i];
-    fc[8] = f[8 * nodes + rm + im];
-}
-
-// rho, u = c sum f ex / rho, v = c sum f ey / rho   (src/lbm.h:72-109)
-__device__ __forceinline__ void moments(const float fc[Q], float c, float& rho, float& u, float& v) {
-    float r = 0.f;
-#pragma unroll
-    for (int q = 0; q < Q; ++q) r += fc[q];
-    const float jx = (fc[2] + fc[5] + fc[8]) - (fc[0] + fc[3] + fc[6]);
-    const float jy = (fc[6] + fc[7] + fc[8]) - (fc[0] + fc[1] + fc[2]);
-    rho = r;
-    u = c * jx / r;
-    v = c * jy / r;
-}
-
-// nu_t = Cs^2 dx^2 sqrt(2 S:S) from central differences (src/lbm.cu:147-171).
-__device__ __forceinline__ float smagorinsky(float uE, float uW, float uN, float uS, float vE,
-                                             float vW, float vN, float vS, const LbmParams& p) {
-    const float ux = (uE - uW) * p.inv_2dx;
-    const float uy = (uN - uS) * p.inv_2dx;
-    const float vx = (vE - vW) * p.inv_2dx;
-    const float vy = (vN - vS) * p.inv_2dx;
-    const float s01 = 0.5f * (uy + vx);
-    const float ss = sqrtf(2.f * (ux * ux + 2.f * s01 * s01 + vy * vy));
-    return p.les_coef * ss;
-}
-
-// Equilibrium populations feq_q(rho,u,v) (src/lbm.h:31-39).  The rest population
-// is taken as rho - sum of the others: in FP32 the weights 4/9, 1/9, 1/36 sum to
-// 1 + 8e-9, and with omega ~ 2 that bias would add ~1.6e-8 rho of mass per node
-// and step; the subtraction keeps sum_q feq = rho up to unbiased rounding.
-__device__ __forceinline__ void equilibrium9(float rho, float u, float v, const LbmParams& p,
-                                             float feq[Q]) {
-    const float uu = (u * u + v * v) * p.inv_c2;
-    float moving = 0.f;
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        if (q == 4) continue;
-        const int ex = q % 3 - 1, ey = q / 3 - 1;
-        const float wq = (ex == 0 || ey == 0) ? W1 : W2;
-        const float uc = (u * float(ex) + v * float(ey)) * p.inv_c;
-        feq[q] = rho * wq * (1.f + 3.f * uc + 4.5f * uc * uc - 1.5f * uu);
-        moving += feq[q];
-    }
-    feq[4] = rho - moving;
-}
-
-// BGK relaxation towards feq plus the first-order force term with linear
-// friction (src/lbm_collision_srt.h:8-16, src/lbm.h:41-70).  FORCE = false is for
-// runs with neither forcing nor friction, where the force term is exactly zero.
-template <bool FORCE>
-__device__ __forceinline__ void collide9(float fc[Q], float rho, float u, float v, float nut,
-                                         float Fx, float Fy, const LbmParams& p) {
-    const float tau = 0.5f + p.k_tau * (p.nu + nut);
-    const float omega = 1.f / tau;
-    const float fxp = (Fx - p.fric_over_dt * u) * p.dt;
-    const float fyp = (Fy - p.fric_over_dt * v) * p.dt;
-    float feq[Q];
-    equilibrium9(rho, u, v, p, feq);
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        const int ex = q % 3 - 1, ey = q / 3 - 1;
-        const float wq = (ex == 0 && ey == 0) ? W0 : ((ex == 0 || ey == 0) ? W1 : W2);
-        const float fq = fc[q] - omega * (fc[q] - feq[q]);
-        if constexpr (FORCE)
-            fc[q] = fq + wq * rho * p.force_scale * (float(ex) * fxp + float(ey) * fyp);
-        else
-            fc[q] = fq;
-    }
-}
+using namespace lbmnode;
 
 // ----------------------------------------------------------------------------
 // fused, batched step

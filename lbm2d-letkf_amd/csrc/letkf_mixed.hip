@@ -73,12 +73,12 @@ __device__ inline int xcd_order(int b, int nb) { return (nb % 8 == 0) ? (b % 8) 
 // Rotation parameters: the FP32 sweeps use the hardware reciprocal, square root and
 // reciprocal square root (about 1 ulp); the FP64 sweeps round correctly.  Any
 // (c, t) accurate to a few ulp gives a rotation orthogonal to working precision.
-__device__ inline float rot_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
-__device__ inline double rot_rcp(double x) { return 1.0 / x; }
-__device__ inline float rot_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
-__device__ inline double rot_sqrt(double x) { return sqrt(x); }
-__device__ inline float rot_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
-__device__ inline double rot_rsq(double x) { return 1.0 / sqrt(x); }
+[[maybe_unused]] __device__ inline float rot_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+[[maybe_unused]] __device__ inline double rot_rcp(double x) { return 1.0 / x; }
+[[maybe_unused]] __device__ inline float rot_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+[[maybe_unused]] __device__ inline double rot_sqrt(double x) { return sqrt(x); }
+[[maybe_unused]] __device__ inline float rot_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+[[maybe_unused]] __device__ inline double rot_rsq(double x) { return 1.0 / sqrt(x); }
 
 // One round of the one-sided (Hestenes) Jacobi method on the column pairs
 // (p, p ^ m), p with the top bit of m clear: over the N - 1 rounds m = 1..N-1 of a

@@ -25,6 +25,8 @@ IC_REFERENCE, IC_K4_SPECTRUM = 0, 1
 OBS_U, OBS_V, OBS_RHO = 1, 2, 4
 LOC_GC_WINDOW, LOC_GAUSS = 0, 1
 VARIANT_PLAIN, VARIANT_MIXED = 0, 1
+CLOSURE_SMAGORINSKY, CLOSURE_MLP = 0, 1
+CLOSURE_NPARAM, CLOSURE_SAMPLE = 5441, 22
 
 
 class Config(C.Structure):
@@ -48,6 +50,8 @@ class Config(C.Structure):
         ("da_interval", C.c_int), ("variant", C.c_int),
         ("jacobi_tol", C.c_double), ("jacobi_max_sweeps", C.c_int),
         ("gram_passes", C.c_int), ("jacobi_fp64_sweeps", C.c_int),
+        ("closure_kind", C.c_int), ("closure_strain_scale", C.c_double), ("closure_nu_scale", C.c_double),
+        ("closure_nu_max", C.c_double),
     ]
 
     def copy(self) -> "Config":
@@ -99,6 +103,20 @@ def _load():
         "lbmletkf_profile_enable": ([P, I], I),
         "lbmletkf_profile_read": ([P, C.c_char_p, C.POINTER(D), C.POINTER(C.c_longlong),
                                    C.POINTER(C.c_longlong)], I),
+        "lbmletkf_closure_init_params": ([C.c_ulonglong, P], I),
+        "lbmletkf_set_closure": ([P, P], I),
+        "lbmletkf_closure_nut": ([P, I, P], I),
+        "lbmletkf_filtered_macro": ([P, I, I, P, P, P], I),
+        "lbmletkf_trainer_create": ([C.POINTER(Config), C.c_longlong, C.POINTER(P)], I),
+        "lbmletkf_trainer_destroy": ([P], I),
+        "lbmletkf_trainer_add_samples": ([P, P, I, I], I),
+        "lbmletkf_trainer_set_samples": ([P, P, C.c_longlong], I),
+        "lbmletkf_trainer_get_samples": ([P, C.c_longlong, C.c_longlong, P], I),
+        "lbmletkf_trainer_count": ([P, C.POINTER(C.c_longlong)], I),
+        "lbmletkf_trainer_set_params": ([P, P], I),
+        "lbmletkf_trainer_get_params": ([P, P], I),
+        "lbmletkf_trainer_loss_grad": ([P, C.c_longlong, C.c_longlong, D, C.POINTER(D), P], I),
+        "lbmletkf_trainer_train": ([P, I, I, D, D, P], I),
         "lbmletkf_get_stream": ([P], P),
         "lbmletkf_synchronize": ([P], I),
     }
@@ -287,12 +305,105 @@ class Context:
         self._ck(lib.lbmletkf_profile_read(self.h, phase.encode(), C.byref(sec), C.byref(n), C.byref(w)))
         return sec.value, n.value, w.value
 
+    def set_closure(self, params: np.ndarray):
+        """Load the learned closure (closure_kind = MLP contexts)."""
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        if p.size != CLOSURE_NPARAM:
+            raise ValueError(f"expected {CLOSURE_NPARAM} parameters")
+        self._ck(lib.lbmletkf_set_closure(self.h, _ptr(p)))
+
+    def closure_nut(self, member: int) -> np.ndarray:
+        """nu_t [ny, nx] the next step would use for one member (after the fallback)."""
+        out = np.empty((self.cfg.ny, self.cfg.nx), np.float32)
+        self._ck(lib.lbmletkf_closure_nut(self.h, member, _ptr(out)))
+        return out
+
+    def filtered_macro(self, member: int, factor: int):
+        """Box-filtered (rho, u, v) of one member on the grid coarsened by factor."""
+        shape = (self.cfg.ny // factor, self.cfg.nx // factor)
+        r, u, v = (np.empty(shape, np.float32) for _ in range(3))
+        self._ck(lib.lbmletkf_filtered_macro(self.h, member, factor, _ptr(r), _ptr(u), _ptr(v)))
+        return r, u, v
+
     @property
     def stream(self) -> int:
         return lib.lbmletkf_get_stream(self.h) or 0
 
     def synchronize(self):
         self._ck(lib.lbmletkf_synchronize(self.h))
+
+
+def closure_init_params(seed: int) -> np.ndarray:
+    """Initial closure parameters (Glorot-uniform from a Philox stream)."""
+    p = np.empty(CLOSURE_NPARAM, np.float32)
+    _check(lib.lbmletkf_closure_init_params(seed, _ptr(p)))
+    return p
+
+
+class Trainer:
+    """Closure trainer: samples, parameters and Adam state on the GPU."""
+
+    def __init__(self, coarse_cfg: Config, capacity: int):
+        h = C.c_void_p()
+        _check(lib.lbmletkf_trainer_create(C.byref(coarse_cfg), capacity, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib.lbmletkf_trainer_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def add_samples(self, fine: Context, member: int, factor: int):
+        _check(lib.lbmletkf_trainer_add_samples(self.h, fine.h, member, factor))
+
+    def set_samples(self, samples: np.ndarray):
+        s = np.ascontiguousarray(samples, dtype=np.float32).reshape(-1, CLOSURE_SAMPLE)
+        _check(lib.lbmletkf_trainer_set_samples(self.h, _ptr(s), s.shape[0]))
+
+    def count(self) -> int:
+        n = C.c_longlong()
+        _check(lib.lbmletkf_trainer_count(self.h, C.byref(n)))
+        return n.value
+
+    def get_samples(self, first: int = 0, n: int | None = None) -> np.ndarray:
+        n = self.count() - first if n is None else n
+        out = np.empty((n, CLOSURE_SAMPLE), np.float32)
+        _check(lib.lbmletkf_trainer_get_samples(self.h, first, n, _ptr(out)))
+        return out
+
+    def set_params(self, params: np.ndarray):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        if p.size != CLOSURE_NPARAM:
+            raise ValueError(f"expected {CLOSURE_NPARAM} parameters")
+        _check(lib.lbmletkf_trainer_set_params(self.h, _ptr(p)))
+
+    def get_params(self) -> np.ndarray:
+        p = np.empty(CLOSURE_NPARAM, np.float32)
+        _check(lib.lbmletkf_trainer_get_params(self.h, _ptr(p)))
+        return p
+
+    def loss_grad(self, first: int = 0, n: int | None = None, w_norm: float = 0.0):
+        """(mean loss, gradient) over samples [first, first+n)."""
+        n = self.count() - first if n is None else n
+        loss = C.c_double()
+        g = np.empty(CLOSURE_NPARAM, np.float32)
+        _check(lib.lbmletkf_trainer_loss_grad(self.h, first, n, w_norm, C.byref(loss), _ptr(g)))
+        return loss.value, g
+
+    def train(self, batch: int, steps: int, lr: float, w_norm: float = 0.0) -> np.ndarray:
+        """Adam steps; returns the loss of each minibatch before its update."""
+        hist = np.empty(max(steps, 1), np.float32)
+        _check(lib.lbmletkf_trainer_train(self.h, batch, steps, lr, w_norm, _ptr(hist)))
+        return hist[:steps]
 
 
 def update(ctx: Context, n_steps: int = 1):

@@ -605,3 +605,216 @@ void orc_ensemble_stats(const orc_config* cfg, int M, const double* f, const dou
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// learned eddy-viscosity closure (a north-star extension, no reference code)
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int CL_IN = 18, CL_H = 64;
+constexpr int CL_W1 = 0, CL_B1 = CL_W1 + CL_H * CL_IN, CL_W2 = CL_B1 + CL_H, CL_B2 = CL_W2 + CL_H * CL_H,
+              CL_W3 = CL_B2 + CL_H, CL_B3 = CL_W3 + CL_H, CL_N = CL_B3 + 1;
+static_assert(CL_N == ORC_CLOSURE_NPARAM, "parameter count");
+
+// round to the nearest bf16 (ties to even), as the GPU's conversion does
+double bf16r(double x) {
+    float f = float(x);
+    uint32_t b;
+    std::memcpy(&b, &f, 4);
+    if ((b & 0x7f800000u) == 0x7f800000u) return x;  // inf / nan
+    const uint32_t lsb = (b >> 16) & 1u;
+    b += 0x7fffu + lsb;
+    b &= 0xffff0000u;
+    std::memcpy(&f, &b, 4);
+    return double(f);
+}
+
+const double GELU_K = std::sqrt(2.0 / PI);
+inline double gelu(double x) { return 0.5 * x * (1.0 + std::tanh(GELU_K * (x + 0.044715 * x * x * x))); }
+inline double gelu_d(double x) {
+    const double t = std::tanh(GELU_K * (x + 0.044715 * x * x * x));
+    return 0.5 * (1.0 + t) + 0.5 * x * (1.0 - t * t) * GELU_K * (1.0 + 3.0 * 0.044715 * x * x);
+}
+inline double softplus(double z) { return std::max(z, 0.0) + std::log1p(std::exp(-std::fabs(z))); }
+inline double sigmoid(double z) { return 1.0 / (1.0 + std::exp(-z)); }
+
+struct Fwd {
+    double p1[CL_H], h1[CL_H], p2[CL_H], h2[CL_H], z;
+};
+
+void mlp_forward(const double* P, const double* x_in, int bf16, Fwd& F) {
+    double x[CL_IN];
+    for (int i = 0; i < CL_IN; ++i) x[i] = bf16 ? bf16r(x_in[i]) : x_in[i];
+    for (int o = 0; o < CL_H; ++o) {
+        double s = 0;
+        for (int i = 0; i < CL_IN; ++i) s += (bf16 ? bf16r(P[CL_W1 + o * CL_IN + i]) : P[CL_W1 + o * CL_IN + i]) * x[i];
+        F.p1[o] = s + P[CL_B1 + o];
+        F.h1[o] = gelu(F.p1[o]);
+    }
+    for (int o = 0; o < CL_H; ++o) {
+        double s = 0;
+        for (int i = 0; i < CL_H; ++i) {
+            const double w = bf16 ? bf16r(P[CL_W2 + o * CL_H + i]) : P[CL_W2 + o * CL_H + i];
+            s += w * (bf16 ? bf16r(F.h1[i]) : F.h1[i]);
+        }
+        F.p2[o] = s + P[CL_B2 + o];
+        F.h2[o] = gelu(F.p2[o]);
+    }
+    double z = P[CL_B3];
+    for (int i = 0; i < CL_H; ++i) z += P[CL_W3 + i] * F.h2[i];
+    F.z = z;
+}
+
+// strain (a, b) of node (i, j) by the central differences of src/lbm.cu:155-158
+inline void strain_ab(int nx, int ny, double dx, const double* u, const double* v, int i, int j, double& a,
+                      double& b) {
+    const long long e = pmod(i + 1, nx) + (long long)nx * j, w = pmod(i - 1, nx) + (long long)nx * j;
+    const long long n = i + (long long)nx * pmod(j + 1, ny), s = i + (long long)nx * pmod(j - 1, ny);
+    const double ux = (u[e] - u[w]) / (2 * dx), uy = (u[n] - u[s]) / (2 * dx);
+    const double vx = (v[e] - v[w]) / (2 * dx), vy = (v[n] - v[s]) / (2 * dx);
+    a = 0.5 * (ux - vy);
+    b = 0.5 * (uy + vx);
+}
+
+}  // namespace
+
+void orc_closure_features(int nx, int ny, double dx, double strain_scale, const double* u, const double* v,
+                          double* feat, double* ab) {
+#pragma omp parallel for
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            const long long n = i + (long long)nx * j;
+            for (int dj = -1; dj <= 1; ++dj)
+                for (int di = -1; di <= 1; ++di) {
+                    double a, b;
+                    strain_ab(nx, ny, dx, u, v, pmod(i + di, nx), pmod(j + dj, ny), a, b);
+                    const int s = (dj + 1) * 3 + (di + 1);
+                    feat[n * CL_IN + 2 * s] = a / strain_scale;
+                    feat[n * CL_IN + 2 * s + 1] = b / strain_scale;
+                    if (di == 0 && dj == 0 && ab) {
+                        ab[2 * n] = a;
+                        ab[2 * n + 1] = b;
+                    }
+                }
+        }
+}
+
+void orc_closure_nut(const double* params, const double* feat, long long n, double nu_scale, int bf16, double* nut) {
+#pragma omp parallel for
+    for (long long k = 0; k < n; ++k) {
+        Fwd F;
+        mlp_forward(params, feat + k * CL_IN, bf16, F);
+        nut[k] = nu_scale * softplus(F.z);
+    }
+}
+
+double orc_closure_loss_grad(const double* P, const double* samples, long long n, double nu_scale, double w_norm,
+                             double* grad) {
+    std::vector<double> g(CL_N, 0.0);
+    long double loss = 0;
+    for (long long k = 0; k < n; ++k) {
+        const double* smp = samples + k * ORC_CLOSURE_SAMPLE;
+        Fwd F;
+        mlp_forward(P, smp, 0, F);
+        const double a = smp[18], b = smp[19], t1 = smp[20], t2 = smp[21];
+        const double nu = nu_scale * softplus(F.z);
+        const double r1 = 2 * nu * a + t1, r2 = 2 * nu * b + t2;
+        loss += w_norm * (r1 * r1 + r2 * r2);
+        const double dnu = w_norm * (4 * r1 * a + 4 * r2 * b);
+        const double dz = dnu * nu_scale * sigmoid(F.z);
+        double dp2[CL_H], dh1[CL_H];
+        for (int i = 0; i < CL_H; ++i) {
+            g[CL_W3 + i] += dz * F.h2[i];
+            dp2[i] = dz * P[CL_W3 + i] * gelu_d(F.p2[i]);
+        }
+        g[CL_B3] += dz;
+        for (int i = 0; i < CL_H; ++i) dh1[i] = 0;
+        for (int o = 0; o < CL_H; ++o) {
+            g[CL_B2 + o] += dp2[o];
+            for (int i = 0; i < CL_H; ++i) {
+                g[CL_W2 + o * CL_H + i] += dp2[o] * F.h1[i];
+                dh1[i] += P[CL_W2 + o * CL_H + i] * dp2[o];
+            }
+        }
+        for (int o = 0; o < CL_H; ++o) {
+            const double dp1 = dh1[o] * gelu_d(F.p1[o]);
+            g[CL_B1 + o] += dp1;
+            for (int i = 0; i < CL_IN; ++i) g[CL_W1 + o * CL_IN + i] += dp1 * smp[i];
+        }
+    }
+    for (int i = 0; i < CL_N; ++i) grad[i] = g[i] / double(n);
+    return double(loss / n);
+}
+
+void orc_coarse_grain(int nxf, int nyf, int F, const double* rho, const double* u, const double* v,
+                      double dx_coarse, double strain_scale, double* samples, double* coarse) {
+    const int nx = nxf / F, ny = nyf / F;
+    const long long N = (long long)nx * ny;
+    std::vector<double> uu(N), uv(N), vv(N);
+    double* rc = coarse;
+    double* uc = coarse + N;
+    double* vc = coarse + 2 * N;
+    for (int J = 0; J < ny; ++J)
+        for (int I = 0; I < nx; ++I) {
+            double sr = 0, su = 0, sv = 0, suu = 0, suv = 0, svv = 0;
+            for (int b = 0; b < F; ++b)
+                for (int a = 0; a < F; ++a) {
+                    const long long k = (long long)(F * I + a) + (long long)nxf * (F * J + b);
+                    sr += rho[k];
+                    su += u[k];
+                    sv += v[k];
+                    suu += u[k] * u[k];
+                    suv += u[k] * v[k];
+                    svv += v[k] * v[k];
+                }
+            const double inv = 1.0 / (F * F);
+            const long long n = I + (long long)nx * J;
+            rc[n] = sr * inv;
+            uc[n] = su * inv;
+            vc[n] = sv * inv;
+            uu[n] = suu * inv;
+            uv[n] = suv * inv;
+            vv[n] = svv * inv;
+        }
+    std::vector<double> feat(N * CL_IN), ab(2 * N);
+    orc_closure_features(nx, ny, dx_coarse, strain_scale, uc, vc, feat.data(), ab.data());
+    for (long long n = 0; n < N; ++n) {
+        double* s = samples + n * ORC_CLOSURE_SAMPLE;
+        for (int i = 0; i < CL_IN; ++i) s[i] = feat[n * CL_IN + i];
+        s[18] = ab[2 * n];
+        s[19] = ab[2 * n + 1];
+        const double t11 = uu[n] - uc[n] * uc[n], t12 = uv[n] - uc[n] * vc[n], t22 = vv[n] - vc[n] * vc[n];
+        s[20] = 0.5 * (t11 - t22);
+        s[21] = t12;
+    }
+}
+
+void orc_lbm_step_closure(const orc_config* cfg, int M, const double* fin, double* fout, long long step,
+                          const double* params, int bf16) {
+    const orc_config& c = *cfg;
+    const long long N = (long long)c.nx * c.ny;
+    std::vector<double> Fx(N, 0.0), Fy(N, 0.0);
+    if (c.force_kind == 1) orc_force_field(cfg, step, Fx.data(), Fy.data());
+    std::vector<double> r(N), u(N), v(N), nus(N), nut(N), feat(N * CL_IN);
+    for (int k = 0; k < M; ++k) {
+        const double* f = fin + (long long)k * Q * N;
+        double* fn = fout + (long long)k * Q * N;
+        orc_stream(cfg, f, fn);
+        orc_macro(cfg, fn, r.data(), u.data(), v.data());
+        orc_les(cfg, u.data(), v.data(), nus.data());
+        orc_closure_features(c.nx, c.ny, c.dx, c.closure_strain_scale, u.data(), v.data(), feat.data(), nullptr);
+        orc_closure_nut(params, feat.data(), N, c.closure_nu_scale, bf16, nut.data());
+#pragma omp parallel for
+        for (long long n = 0; n < N; ++n) {
+            const double nt = (std::isfinite(nut[n]) && nut[n] >= 0 && nut[n] <= c.closure_nu_max) ? nut[n] : nus[n];
+            const double tau = 0.5 + 3.0 * (c.nu + nt) / c.c / c.c / c.dt;
+            const double omega = 1.0 / tau;
+            for (int q = 0; q < Q; ++q) {
+                double fc = fn[q * N + n];
+                fc -= omega * (fc - feq_q(c, r[n], u[n], v[n], q));
+                fc += force_q(c, Fx[n], Fy[n], q, r[n], u[n], v[n]);
+                fn[q * N + n] = fc;
+            }
+        }
+    }
+}

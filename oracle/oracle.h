@@ -37,6 +37,8 @@ typedef struct orc_config {
     double jacobi_tol;
     int    jacobi_max_sweeps;
     int    gram_passes, jacobi_fp64_sweeps;  /* GPU precision knobs; unused by the oracle */
+    int    closure_kind;
+    double closure_strain_scale, closure_nu_scale, closure_nu_max;
 } orc_config;
 
 int  orc_sizeof_config(void);
@@ -87,6 +89,36 @@ int  orc_letkf_analysis(const orc_config* cfg, int M, double* f, const double* y
 /* ensemble_stat/main.cpp:108-177 with N = nx*ny: out = {rmse_uv, spread_uv, rmse_rho, spread_rho} */
 void orc_ensemble_stats(const orc_config* cfg, int M, const double* f, const double* trho,
                         const double* tu, const double* tv, double out[4]);
+
+/* ---- learned eddy-viscosity closure (a north-star extension; no reference code) ----
+ * features at node n: for the 3x3 stencil s = (dj+1)*3 + (di+1), the deviatoric
+ * strain a = (ux - vy)/2 and shear b = (uy + vx)/2 at node (i+di, j+dj) from the
+ * central differences of src/lbm.cu:155-158, divided by strain_scale:
+ * feat[n][2s] = a, feat[n][2s+1] = b.  ab[n] = (a, b) at the node itself, unscaled.
+ * MLP 18 -> 64 -> 64 -> 1: h1 = gelu(W1 x + b1), h2 = gelu(W2 h1 + b2),
+ * z = w3 . h2 + b3, nu_t = nu_scale * softplus(z); gelu(x) = x/2 (1 + tanh(k (x +
+ * 0.044715 x^3))), k = sqrt(2/pi).  params: W1[64][18], b1[64], W2[64][64], b2[64],
+ * w3[64], b3 (5441 values).  bf16 != 0 rounds x, W1, W2 and h1 to bf16 (the fused
+ * kernel's MFMA operands).
+ * A sample is 22 values: 18 features, a, b (unscaled), t1 = (tau11 - tau22)/2,
+ * t2 = tau12 (the filtered SGS stress); loss = mean w_norm ((2 nu a + t1)^2 +
+ * (2 nu b + t2)^2), the squared misfit of tau_dev = -2 nu_t S_dev. */
+#define ORC_CLOSURE_NPARAM 5441
+#define ORC_CLOSURE_SAMPLE 22
+void   orc_closure_features(int nx, int ny, double dx, double strain_scale, const double* u, const double* v,
+                            double* feat, double* ab);
+void   orc_closure_nut(const double* params, const double* feat, long long n, double nu_scale, int bf16,
+                       double* nut);
+double orc_closure_loss_grad(const double* params, const double* samples, long long n, double nu_scale,
+                             double w_norm, double* grad);
+/* box filter by F of fine fields [nyf][nxf] -> coarse [3][ny][nx] (rho, u, v) and
+ * samples [ny*nx][22]; coarse dx = dx_coarse */
+void   orc_coarse_grain(int nxf, int nyf, int F, const double* rho, const double* u, const double* v,
+                        double dx_coarse, double strain_scale, double* samples, double* coarse);
+/* one LBM step (as orc_lbm_step) with nu_t from the MLP where it is finite and
+ * within [0, closure_nu_max], Smagorinsky elsewhere */
+void   orc_lbm_step_closure(const orc_config* cfg, int M, const double* fin, double* fout, long long step,
+                            const double* params, int bf16);
 
 #ifdef __cplusplus
 }

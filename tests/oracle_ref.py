@@ -41,7 +41,14 @@ _sig = {
     "orc_letkf_transform": ([I, I, P, P, P, D, D, I, P, P, P], I),
     "orc_letkf_analysis": ([P, I, P, P, I, I, P, P, P, I], I),
     "orc_ensemble_stats": ([P, I, P, P, P, P, P], None),
+    "orc_closure_features": ([I, I, D, D, P, P, P, P], None),
+    "orc_closure_nut": ([P, P, C.c_longlong, D, I, P], None),
+    "orc_closure_loss_grad": ([P, P, C.c_longlong, D, D, P], D),
+    "orc_coarse_grain": ([I, I, I, P, P, P, D, D, P, P], None),
+    "orc_lbm_step_closure": ([P, I, P, P, C.c_longlong, P, I], None),
 }
+CLOSURE_NPARAM = 5441
+CLOSURE_SAMPLE = 22
 for _n, (_a, _r) in _sig.items():
     _f = getattr(_o, _n)
     _f.argtypes = _a
@@ -211,3 +218,44 @@ def ensemble_stats(cfg, f, trho, tu, tv):
     _o.orc_ensemble_stats(_c(cfg), fin.shape[0], _p(fin), _p(_f64(trho)), _p(_f64(tu)),
                           _p(_f64(tv)), _p(out))
     return out
+
+
+# ---- learned eddy-viscosity closure ------------------------------------------
+def closure_features(nx, ny, dx, strain_scale, u, v):
+    """(feat [ny*nx][18], ab [ny*nx][2])"""
+    feat = np.empty((nx * ny, 18))
+    ab = np.empty((nx * ny, 2))
+    _o.orc_closure_features(nx, ny, dx, strain_scale, _p(_f64(u)), _p(_f64(v)), _p(feat), _p(ab))
+    return feat, ab
+
+
+def closure_nut(params, feat, nu_scale, bf16=False):
+    feat = _f64(feat)
+    nut = np.empty(feat.shape[0])
+    _o.orc_closure_nut(_p(_f64(params)), _p(feat), feat.shape[0], nu_scale, int(bf16), _p(nut))
+    return nut
+
+
+def closure_loss_grad(params, samples, nu_scale, w_norm=1.0):
+    samples = _f64(samples)
+    grad = np.empty(CLOSURE_NPARAM)
+    loss = _o.orc_closure_loss_grad(_p(_f64(params)), _p(samples), samples.shape[0], nu_scale, w_norm, _p(grad))
+    return loss, grad
+
+
+def coarse_grain(rho, u, v, F, dx_coarse, strain_scale):
+    """fine [nyf][nxf] fields -> (samples [n][22], coarse [3][ny][nx])"""
+    nyf, nxf = rho.shape
+    ny, nx = nyf // F, nxf // F
+    samples = np.empty((nx * ny, CLOSURE_SAMPLE))
+    coarse = np.empty((3, ny, nx))
+    _o.orc_coarse_grain(nxf, nyf, F, _p(_f64(rho)), _p(_f64(u)), _p(_f64(v)), dx_coarse, strain_scale,
+                        _p(samples), _p(coarse))
+    return samples, coarse
+
+
+def lbm_step_closure(cfg, f, step: int, params, bf16=False):
+    fin = _f64(f)
+    fout = np.empty_like(fin)
+    _o.orc_lbm_step_closure(_c(cfg), fin.shape[0], _p(fin), _p(fout), step, _p(_f64(params)), int(bf16))
+    return fout

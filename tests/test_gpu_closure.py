@@ -1,0 +1,277 @@
+"""GPU tests of the learned eddy-viscosity closure (run on an MI355X).
+
+* the trainer's forward/backward kernels against the FP64 oracle's loss and
+  gradient on the same samples and parameters (FP32 accumulation: relative L2
+  error of the gradient below 1e-4);
+* training lowers the loss and is deterministic (fixed-order reductions);
+* the coarse-graining kernels against the oracle's box filter;
+* the fused MLP step: nu_t against the oracle's bf16-emulating network on the
+  streamed velocity, and the populations against the oracle's step;
+* the fallback to Smagorinsky reproduces the Smagorinsky kernel;
+* what a trained network gains over Smagorinsky, a priori (subgrid-stress error
+  on a held-out fine run) and a posteriori (coarse runs against the filtered fine
+  truth).
+"""
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, small_cfg
+
+pytestmark = pytest.mark.gpu
+NP, NS = 5441, 22
+W1, B1, W2, B2, W3, B3 = 0, 1152, 1216, 5312, 5376, 5440
+
+
+def _params(rng, scale=0.5, b3=0.2):
+    p = np.zeros(NP)
+    p[W1:B1] = rng.standard_normal(B1 - W1) * scale / math.sqrt(18)
+    p[B1:W2] = rng.standard_normal(64) * 0.1
+    p[W2:B2] = rng.standard_normal(B2 - W2) * scale / math.sqrt(64)
+    p[B2:W3] = rng.standard_normal(64) * 0.1
+    p[W3:B3] = rng.standard_normal(64) * scale / math.sqrt(64)
+    p[B3] = b3
+    return p.astype(np.float32)
+
+
+def _samples(rng, n, s0, nu0):
+    s = np.empty((n, NS), np.float32)
+    s[:, :18] = rng.standard_normal((n, 18))
+    s[:, 18:20] = s[:, 8:10] * s0          # centre strain, unscaled
+    s[:, 20:22] = -2 * nu0 * s[:, 18:20] * (1 + 0.5 * rng.standard_normal((n, 1))) \
+        + 0.3 * nu0 * s0 * rng.standard_normal((n, 2))
+    return s
+
+
+def _trainer_module():
+    sys.path.insert(0, os.path.join(ROOT, "lbm2d-letkf_amd"))
+    import closure_train
+
+    return closure_train
+
+
+def test_gradient_matches_oracle(lb, orc):
+    cfg = lb.preset("baseline")
+    s0, nu0 = cfg.closure_strain_scale, cfg.closure_nu_scale
+    rng = np.random.default_rng(3)
+    p, smp = _params(rng), _samples(rng, 777, s0, nu0)
+    wn = 1.0 / float(np.mean(smp[:, 20].astype(np.float64) ** 2 + smp[:, 21].astype(np.float64) ** 2))
+    print(f"case: closure gradient n=777 w_norm={wn:.3e}", flush=True)
+    with lb.Trainer(cfg, 1000) as tr:
+        tr.set_samples(smp)
+        tr.set_params(p)
+        loss, g = tr.loss_grad(w_norm=wn)
+        loss_o, g_o = orc.closure_loss_grad(p, smp, nu0, wn)
+        print(f"  loss gpu {loss:.8e} oracle {loss_o:.8e}", flush=True)
+        assert loss == pytest.approx(loss_o, rel=1e-5)
+        for name, a, b in (("W1", W1, B1), ("b1", B1, W2), ("W2", W2, B2), ("b2", B2, W3), ("w3", W3, B3),
+                           ("b3", B3, NP)):
+            err = np.linalg.norm(g[a:b] - g_o[a:b]) / np.linalg.norm(g_o[a:b])
+            print(f"  {name}: rel L2 {err:.2e}", flush=True)
+            assert err < 1e-4, name
+        # a sub-range through the same entry point
+        loss2, g2 = tr.loss_grad(100, 300, w_norm=wn)
+        loss2_o, g2_o = orc.closure_loss_grad(p, smp[100:400], nu0, wn)
+        assert loss2 == pytest.approx(loss2_o, rel=1e-5)
+        assert np.linalg.norm(g2 - g2_o) / np.linalg.norm(g2_o) < 1e-4
+
+
+def test_training_lowers_loss_and_is_deterministic(lb, orc):
+    cfg = lb.preset("baseline")
+    s0, nu0 = cfg.closure_strain_scale, cfg.closure_nu_scale
+    rng = np.random.default_rng(5)
+    n = 8192
+    smp = _samples(rng, n, s0, nu0)
+    # targets of a teacher network of the same shape: a learnable nu_t field
+    teacher = _params(rng, scale=1.0, b3=0.5)
+    nut = orc.closure_nut(teacher, smp[:, :18], nu0)
+    smp[:, 20:22] = (-2 * nut[:, None] * smp[:, 18:20]).astype(np.float32)
+    print("case: closure training n=8192 batch 512 steps 600", flush=True)
+    out = []
+    for rep in range(2):
+        with lb.Trainer(cfg, n) as tr:
+            tr.set_samples(smp)
+            tr.set_params(lb.closure_init_params(11))
+            l0, _ = tr.loss_grad()
+            hist = tr.train(512, 600, 3e-3)
+            l1, _ = tr.loss_grad()
+            out.append((tr.get_params(), hist, l0, l1))
+    p, hist, l0, l1 = out[0]
+    print(f"  full loss {l0:.4f} -> {l1:.4f}; minibatch {hist[:20].mean():.4f} -> {hist[-20:].mean():.4f}",
+          flush=True)
+    assert np.isfinite(hist).all()
+    assert l1 < 0.1 * l0
+    np.testing.assert_array_equal(out[1][0], p)
+    np.testing.assert_array_equal(out[1][1], hist)
+
+
+def test_coarse_samples_match_oracle(lb, orc):
+    coarse = small_cfg(lb, "baseline", n=64)
+    ct = _trainer_module()
+    fine = ct.fine_config(coarse, 4)
+    print("case: coarse-grain 256x256 -> 64x64", flush=True)
+    with lb.Context(fine, 2) as ctx, lb.Trainer(coarse, 2 * 64 * 64) as tr:
+        ctx.init_member(0, 21)
+        ctx.init_member(1, 22)
+        ctx.forecast(40)
+        tr.add_samples(ctx, 1, 4)
+        tr.add_samples(ctx, 0, 4)
+        assert tr.count() == 2 * 4096
+        got = tr.get_samples()
+        for k, m in enumerate((1, 0)):
+            r, u, v = ctx.get_macro(m)
+            smp, cf = orc.coarse_grain(r, u, v, 4, coarse.dx, coarse.closure_strain_scale)
+            g = got[k * 4096:(k + 1) * 4096]
+            fs = np.abs(smp[:, :20]).max(axis=0)
+            np.testing.assert_allclose(g[:, :20], smp[:, :20], rtol=0, atol=1e-5 * fs.max())
+            ts = np.abs(smp[:, 20:]).max()
+            np.testing.assert_allclose(g[:, 20:], smp[:, 20:], rtol=0, atol=2e-5 * ts)
+            fr, fu, fv = ctx.filtered_macro(m, 4)
+            np.testing.assert_allclose(fr, cf[0], rtol=1e-6)
+            np.testing.assert_allclose(fu, cf[1], rtol=0, atol=1e-7 * np.abs(cf[1]).max())
+            np.testing.assert_allclose(fv, cf[2], rtol=0, atol=1e-7 * np.abs(cf[2]).max())
+
+
+def _oracle_nut(orc, cfg, f, params):
+    fn = orc.stream(cfg, f)
+    _, u, v = orc.macro(cfg, fn)
+    feat, _ = orc.closure_features(cfg.nx, cfg.ny, cfg.dx, cfg.closure_strain_scale, u, v)
+    nut = orc.closure_nut(params, feat, cfg.closure_nu_scale, bf16=True).reshape(cfg.ny, cfg.nx)
+    nus = orc.les(cfg, u, v)
+    ok = np.isfinite(nut) & (nut >= 0) & (nut <= cfg.closure_nu_max)
+    return np.where(ok, nut, nus), ok
+
+
+@pytest.mark.parametrize("kind", ["baseline", "ref_forced"])
+def test_mlp_step_matches_oracle(lb, orc, kind):
+    cfg = small_cfg(lb, "baseline" if kind == "baseline" else "ref", n=64)
+    cfg.closure_kind = lb.CLOSURE_MLP
+    rng = np.random.default_rng(9)
+    p = _params(rng, scale=1.0, b3=1.0)
+    M = 2
+    print(f"case: mlp step {kind} 64x64 M={M}", flush=True)
+    with lb.Context(cfg, M) as ctx:
+        for k in range(M):
+            ctx.init_member(k, 300 + k)
+        ctx.set_closure(p)
+        f = ctx.get_all_states().astype(np.float64)
+        for m in range(M):
+            nut = ctx.closure_nut(m)
+            ref, ok = _oracle_nut(orc, cfg, f[m], p)
+            rel = np.abs(nut - ref) / np.abs(ref)
+            print(f"  member {m}: nu_t max rel {rel.max():.2e} mean {rel.mean():.2e}, "
+                  f"range {ref.min():.3e}..{ref.max():.3e}, network nodes {ok.mean():.3f}", flush=True)
+            assert ok.mean() > 0.99
+            assert rel.max() < 2e-2 and rel.mean() < 1e-3
+        for s in range(4):
+            ctx.forecast(1)
+            f = orc.lbm_step_closure(cfg, f, s, p, bf16=True)
+            err = np.linalg.norm(ctx.get_all_states() - f) / np.linalg.norm(f)
+            print(f"  step {s + 1}: rel L2 {err:.2e}", flush=True)
+            assert err < 1e-5
+
+
+def test_fallback_reproduces_smagorinsky(lb):
+    cfg = small_cfg(lb, "ref", n=64)
+    cfg.variant = lb.VARIANT_MIXED
+    M = 2
+    print("case: mlp fallback 64x64 M=2", flush=True)
+    with lb.Context(cfg, M) as ref:
+        for k in range(M):
+            ref.init_member(k, 40 + k)
+        f0 = ref.get_all_states()
+        ref.forecast(5)
+        want = ref.get_all_states()
+    mcfg = cfg.copy()
+    mcfg.closure_kind = lb.CLOSURE_MLP
+    for b3 in (1e4, float("nan")):  # nu_t far above nu_max, and not finite
+        p = np.zeros(NP, np.float32)
+        p[B3] = b3
+        with lb.Context(mcfg, M) as ctx:
+            ctx.set_all_states(f0)
+            ctx.set_closure(p)
+            ctx.forecast(5)
+            got = ctx.get_all_states()
+        d = np.abs(got - want).max()
+        print(f"  b3={b3}: max |diff| {d:.3e}", flush=True)
+        assert d <= 2e-7 * np.abs(want).max()
+
+
+def test_closure_errors(lb):
+    cfg = small_cfg(lb, "baseline", n=64)
+    with lb.Context(cfg, 1) as ctx:
+        with pytest.raises(lb.LbmLetkfError) as e:
+            ctx.set_closure(np.zeros(NP, np.float32))
+        assert e.value.status == lb.ERR_STATE
+    cfg.closure_kind = lb.CLOSURE_MLP
+    with lb.Context(cfg, 1) as ctx:
+        with pytest.raises(lb.LbmLetkfError) as e:
+            ctx.forecast(1)
+        assert e.value.status == lb.ERR_STATE
+    bad = small_cfg(lb, "baseline", n=48)
+    bad.closure_kind = lb.CLOSURE_MLP
+    with pytest.raises(lb.LbmLetkfError):
+        lb.config_check(bad, 1)
+
+
+def _smag_loss(smp, cs, dx, w):
+    a, b, t1, t2 = (smp[:, k].astype(np.float64) for k in (18, 19, 20, 21))
+    nu = (cs * dx) ** 2 * 2 * np.sqrt(a * a + b * b)  # |S| of a divergence-free strain
+    return float(np.mean(w * ((2 * nu * a + t1) ** 2 + (2 * nu * b + t2) ** 2)))
+
+
+def _best_cs2(smp, dx):
+    a, b, t1, t2 = (smp[:, k].astype(np.float64) for k in (18, 19, 20, 21))
+    g = 2 * dx * dx * 2 * np.sqrt(a * a + b * b)  # nu = cs^2 g
+    # minimise sum (2 cs^2 g a + t1)^2 + (2 cs^2 g b + t2)^2 over cs^2
+    return float(-np.sum(2 * g * (a * t1 + b * t2)) / np.sum(4 * g * g * (a * a + b * b)))
+
+
+def test_trained_closure_gain_over_smagorinsky(lb):
+    coarse = lb.preset("baseline")  # 256^2, lattice units, Cs = 0.1
+    ct = _trainer_module()
+    n = coarse.nx * coarse.ny
+    print("case: closure training on 1024^2 -> 256^2 snapshots", flush=True)
+    p, hist = ct.train_closure(coarse, factor=4, n_snap=12, every=25, spinup=150, seed=7, batch=4096,
+                               steps=1500, lr=2e-3)
+    print(f"  training loss {hist[:50].mean():.4f} -> {hist[-50:].mean():.4f}", flush=True)
+    # a priori, on a held-out fine run
+    with lb.Trainer(coarse, 2 * n) as ev:
+        ct.collect(ev, coarse, factor=4, n_snap=2, every=25, spinup=300, seed=99)
+        smp = ev.get_samples()
+        w = 1.0 / float(np.mean(smp[:, 20].astype(np.float64) ** 2 + smp[:, 21].astype(np.float64) ** 2))
+        ev.set_params(p)
+        l_mlp, _ = ev.loss_grad(w_norm=w)
+    l_smag = _smag_loss(smp, coarse.cs_smag, coarse.dx, w)
+    cs2 = max(_best_cs2(smp, coarse.dx), 0.0)
+    l_best = _smag_loss(smp, math.sqrt(cs2), coarse.dx, w)
+    print(f"  a priori normalised stress error: none 1.000  Smagorinsky(Cs={coarse.cs_smag}) {l_smag:.4f}  "
+          f"best-fit Cs={math.sqrt(cs2):.3f} {l_best:.4f}  MLP {l_mlp:.4f}", flush=True)
+    assert l_mlp < l_best <= l_smag + 1e-12
+    # a posteriori: coarse runs from the filtered fine state against the filtered truth
+    fine_cfg = ct.fine_config(coarse, 4)
+    T = 100
+    with lb.Context(fine_cfg, 1) as fine:
+        fine.init_member(0, 99)
+        fine.forecast(300 * 4)
+        r0, u0, v0 = fine.filtered_macro(0, 4)
+        fine.forecast(T * 4)
+        rt, ut, vt = fine.filtered_macro(0, 4)
+    res = {}
+    for name, kind in (("smagorinsky", lb.CLOSURE_SMAGORINSKY), ("mlp", lb.CLOSURE_MLP)):
+        cfg = coarse.copy()
+        cfg.closure_kind = kind
+        with lb.Context(cfg, 1) as ctx:
+            ctx.set_macro(0, r0, u0, v0)
+            if kind == lb.CLOSURE_MLP:
+                ctx.set_closure(p)
+            ctx.forecast(T)
+            _, u, v = ctx.get_macro(0)
+        res[name] = float(np.sqrt(np.mean((u - ut) ** 2 + (v - vt) ** 2)))
+    base = float(np.sqrt(np.mean((u0 - ut) ** 2 + (v0 - vt) ** 2)))
+    print(f"  a posteriori velocity RMSE vs filtered truth after {T} steps: persistence {base:.4e}  "
+          f"Smagorinsky {res['smagorinsky']:.4e}  MLP {res['mlp']:.4e}", flush=True)
+    assert np.isfinite(res["mlp"]) and res["mlp"] < base
