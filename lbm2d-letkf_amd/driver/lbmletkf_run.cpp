@@ -31,6 +31,9 @@
 //   --no-nan                keep values at unobserved nodes (default: NaN, OBS_XYPRUNE_NAN)
 //   --stats                 also write the ensemble mean/std files (ENSEMBLE_STAT)
 //   --timing                print the phase timers at the end
+//   --savemat [g0:count]    at each analysis write the LETKF matrices of gridpoints
+//                           [g0, g0+count) (default all) to <prefix>/letkf/mat/ens0
+//                           (src/util/letkf_solver.h:244-345, src/letkf.cu:277-306)
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -42,6 +45,7 @@
 
 #include "../../include/lbmletkf.h"
 #include "datio.hpp"
+#include "matio.hpp"
 
 namespace {
 
@@ -70,6 +74,8 @@ struct Options {
     bool nan_unobserved = true;
     bool stats = false;
     bool timing = false;
+    bool savemat = false;
+    long long savemat_g0 = 0, savemat_n = -1;
 };
 
 [[noreturn]] void usage(const char* msg) {
@@ -78,7 +84,7 @@ struct Options {
                          "       [--ens-offset K] [--prefix DIR] [--outputs N] [--iiter n] [--ioprune n]\n"
                          "       [--daprune n] [--spinup n] [--nx n] [--variant plain|mixed] [--obs-stride p]\n"
                          "       [--loc-scale L] [--loc-cutoff d] [--inflation b] [--obs-seed S] [--no-nan]\n"
-                         "       [--stats] [--timing]\n");
+                         "       [--stats] [--timing] [--savemat [g0:count]]\n");
     std::exit(2);
 }
 
@@ -117,6 +123,16 @@ Options parse(int argc, char** argv) {
         else if (k == "--no-nan") o.nan_unobserved = false;
         else if (k == "--stats") o.stats = true;
         else if (k == "--timing") o.timing = true;
+        else if (k == "--savemat") {
+            o.savemat = true;
+            if (a + 1 < argc && argv[a + 1][0] != '-') {
+                const std::string v = argv[++a];
+                const size_t c = v.find(':');
+                if (c == std::string::npos) usage("--savemat range must be g0:count");
+                o.savemat_g0 = std::stoll(v.substr(0, c));
+                o.savemat_n = std::stoll(v.substr(c + 1));
+            }
+        }
         else usage(("unknown option " + k).c_str());
     }
     if (o.members < 1 || o.outputs < 0 || o.ioprune < 1 || o.daprune < 1 || o.spinup < 0)
@@ -215,6 +231,68 @@ struct EnsembleStat {
     }
 };
 
+// Member states of gridpoints [g0, g0+n) as the reference's X matrices: per
+// gridpoint 9 x M column-major (row = population q, column = member).
+std::vector<float> state_matrices(const Ctx& c, int M, size_t nodes, long long g0, long long n) {
+    std::vector<float> X(size_t(9) * M * size_t(n)), f(size_t(9) * nodes);
+    for (int m = 0; m < M; ++m) {
+        c.ck(lbmletkf_get_state(c.h, m, f.data()), "get_state");
+        for (long long b = 0; b < n; ++b)
+            for (int q = 0; q < 9; ++q) X[q + 9 * (m + size_t(M) * b)] = f[size_t(q) * nodes + size_t(g0 + b)];
+    }
+    return X;
+}
+
+// The reference's solve_xa diagnostics from the transform dump and the states
+// before and after the analysis: _Xf, _dXf, _Pa = V D^-1 V^T = W W / (M-1),
+// _dWa = W, _Xa, _Wa = W + w 1^T, _AI = Xa - Xf, _Z = Wa - I, _Lambda = diag(d),
+// _Wma = w 1^T.  W, w and d are the FP64 results of the analysis kernels; the
+// eigenvalues are in ascending order.
+void save_matrices(const std::string& dir, long long t, int M, long long n, const std::vector<float>& Xf,
+                   const std::vector<float>& Xa, const std::vector<double>& W, const std::vector<double>& w,
+                   const std::vector<double>& d) {
+    using lbmletkf_io::save_matrix;
+    const size_t MM = size_t(M) * M, N = size_t(n);
+    std::vector<float> dX(Xf.size()), AI(Xf.size());
+    for (size_t b = 0; b < N; ++b)
+        for (int q = 0; q < 9; ++q) {
+            double mean = 0;
+            for (int m = 0; m < M; ++m) mean += Xf[q + 9 * (m + M * b)];
+            mean /= M;
+            for (int m = 0; m < M; ++m) {
+                const size_t k = q + 9 * (m + M * b);
+                dX[k] = float(Xf[k] - mean);
+                AI[k] = Xa[k] - Xf[k];
+            }
+        }
+    std::vector<double> dWa(MM * N), Wa(MM * N), Z(MM * N), Lam(MM * N, 0.0), Wma(MM * N), Pa(MM * N);
+    for (size_t b = 0; b < N; ++b) {
+        const double* Wb = W.data() + MM * b;  // row-major W[i*M + j]
+        for (int c = 0; c < M; ++c)
+            for (int r = 0; r < M; ++r) {
+                const size_t k = r + size_t(M) * c + MM * b;
+                dWa[k] = Wb[size_t(r) * M + c];
+                Wa[k] = dWa[k] + w[size_t(M) * b + r];
+                Z[k] = Wa[k] - (r == c ? 1.0 : 0.0);
+                Wma[k] = w[size_t(M) * b + r];
+                double s = 0;
+                for (int k2 = 0; k2 < M; ++k2) s += Wb[size_t(r) * M + k2] * Wb[size_t(k2) * M + c];
+                Pa[k] = s / (M - 1);
+            }
+        for (int r = 0; r < M; ++r) Lam[r + size_t(M) * r + MM * b] = d[size_t(M) * b + r];
+    }
+    save_matrix(dir, "_Xf", 9, M, N, t, Xf);
+    save_matrix(dir, "_dXf", 9, M, N, t, dX);
+    save_matrix(dir, "_Pa", M, M, N, t, Pa);
+    save_matrix(dir, "_dWa", M, M, N, t, dWa);
+    save_matrix(dir, "_Xa", 9, M, N, t, Xa);
+    save_matrix(dir, "_Wa", M, M, N, t, Wa);
+    save_matrix(dir, "_AI", 9, M, N, t, AI);
+    save_matrix(dir, "_Z", M, M, N, t, Z);
+    save_matrix(dir, "_Lambda", M, M, N, t, Lam);
+    save_matrix(dir, "_Wma", M, M, N, t, Wma);
+}
+
 int run(const Options& o) {
     lbmletkf_config cfg;
     if (lbmletkf_config_preset(o.preset.c_str(), &cfg)) throw std::runtime_error(lbmletkf_last_error(nullptr));
@@ -254,6 +332,13 @@ int run(const Options& o) {
     int n_obs = 0, n_sites = 0;
     if (o.mode == "da_letkf") c.ck(lbmletkf_obs_count(c.h, &n_sites, &n_obs), "obs_count");
     std::vector<float> yo(size_t(std::max(n_obs, 1)));
+    const long long mat_g0 = o.savemat_g0, mat_n = o.savemat_n < 0 ? (long long)n - o.savemat_g0 : o.savemat_n;
+    const std::string mat_dir = o.prefix + "/letkf/mat/ens0";
+    if (o.savemat && o.mode == "da_letkf") {
+        if (mat_g0 < 0 || mat_n < 1 || mat_g0 + mat_n > (long long)n) usage("--savemat range out of the grid");
+        c.ck(lbmletkf_set_dump(c.h, int(mat_g0), int(mat_n)), "set_dump");
+        make_dirs(mat_dir);
+    }
 
     for (int tt = 0; tt <= o.outputs; ++tt) {
         const long long t = (long long)tt * iiter;
@@ -264,8 +349,17 @@ int run(const Options& o) {
             read_field(step_file(od, "u", step), obs.u.data(), n);
             read_field(step_file(od, "v", step), obs.v.data(), n);
             c.ck(lbmletkf_obs_from_fields(c.h, obs.r.data(), obs.u.data(), obs.v.data(), yo.data()), "obs_from_fields");
+            std::vector<float> Xf;
+            if (o.savemat) Xf = state_matrices(c, M, n, mat_g0, mat_n);
             c.ck(lbmletkf_analysis(c.h, yo.data(), 0), "analysis");
             std::printf("t=%lld: analysis with %d observations\n", t, n_obs);
+            if (o.savemat) {
+                const size_t nm = size_t(mat_n);
+                std::vector<double> W(nm * M * M), w(nm * M), d(nm * M);
+                c.ck(lbmletkf_get_dump(c.h, W.data(), w.data(), d.data()), "get_dump");
+                save_matrices(mat_dir, t, M, mat_n, Xf, state_matrices(c, M, n, mat_g0, mat_n), W, w, d);
+                std::printf("t=%lld: LETKF matrices of %lld gridpoints -> %s\n", t, mat_n, mat_dir.c_str());
+            }
         }
         if (t % ((long long)iiter * o.ioprune) == 0) {
             EnsembleStat es(o.stats ? n : 0);

@@ -83,3 +83,74 @@ def test_driver_rejects_bad_arguments(tmp_path):
     p = subprocess.run([RUN, "da_letkf", "--prefix", str(tmp_path / "none"), "--members", "4"] + COMMON,
                        capture_output=True, text=True, timeout=120)
     assert p.returncode == 1 and "could not open file" in p.stderr
+
+
+def _read_matrix(path, rows, cols, batch):
+    """A matrix file of the driver: HDF5 dataset "data" (dims rows, cols, batch) or
+    the raw .bin twin; returns [batch][cols][rows] (column-major per matrix)."""
+    import ctypes as C
+
+    is_f = ".float_" in os.path.basename(path)
+    if path.endswith(".bin"):
+        a = np.fromfile(path, dtype=np.float32 if is_f else np.float64)
+    else:
+        hl = C.CDLL("/opt/conda/lib/libhdf5_hl.so")
+        h5 = C.CDLL("/opt/conda/lib/libhdf5.so")
+        h5.H5Fopen.restype = C.c_int64
+        h5.H5Fopen.argtypes = [C.c_char_p, C.c_uint, C.c_int64]
+        h5.H5Fclose.argtypes = [C.c_int64]
+        fid = h5.H5Fopen(path.encode(), 0, 0)
+        assert fid > 0, path
+        dims = (C.c_ulonglong * 3)()
+        cls, tsize = C.c_int(), C.c_size_t()
+        hl.H5LTget_dataset_info.argtypes = [C.c_int64, C.c_char_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        assert hl.H5LTget_dataset_info(fid, b"data", dims, C.byref(cls), C.byref(tsize)) >= 0
+        assert list(dims) == [rows, cols, batch], list(dims)
+        a = np.empty(rows * cols * batch, np.float32 if is_f else np.float64)
+        fn = hl.H5LTread_dataset_float if is_f else hl.H5LTread_dataset_double
+        fn.argtypes = [C.c_int64, C.c_char_p, C.c_void_p]
+        assert fn(fid, b"data", a.ctypes.data_as(C.c_void_p)) >= 0
+        h5.H5Fclose(fid)
+    return a.reshape(batch, cols, rows)
+
+
+def test_savemat_letkf_matrices(tmp_path):
+    io = str(tmp_path / "io")
+    short = [a for a in COMMON]
+    short[short.index("--outputs") + 1] = "5"
+    _run([RUN, "observe", "--prefix", io] + short)
+    M, g0, n, t = 8, 100, 300, 50
+    _run([RUN, "da_letkf", "--prefix", io, "--members", str(M), "--savemat", f"{g0}:{n}"] + short)
+    d = os.path.join(io, "letkf", "mat", "ens0")
+    names = sorted(os.listdir(d))
+    print(names)
+    ext = names[0].rsplit(".", 1)[1]
+    assert ext in ("h5", "bin") and len(names) == 10
+
+    def rd(label, rows, typ):
+        return _read_matrix(os.path.join(d, f"{label}.{typ}_{rows}_{M}_{n}.{t}.{ext}"), rows, M, n)
+
+    Xf, dXf, Xa, AI = (rd(k, 9, "float").astype(np.float64) for k in ("_Xf", "_dXf", "_Xa", "_AI"))
+    dWa, Wa, Z, Lam, Wma, Pa = (rd(k, M, "double") for k in ("_dWa", "_Wa", "_Z", "_Lambda", "_Wma", "_Pa"))
+    # arrays are [batch][col][row]; transpose to [batch][row][col]
+    T = lambda a: np.swapaxes(a, 1, 2)  # noqa: E731
+    Xf, dXf, Xa, AI, dWa, Wa, Z, Lam, Wma, Pa = map(T, (Xf, dXf, Xa, AI, dWa, Wa, Z, Lam, Wma, Pa))
+    np.testing.assert_allclose(dXf, Xf - Xf.mean(axis=2, keepdims=True), atol=1e-7)
+    np.testing.assert_allclose(AI, Xa - Xf, atol=1e-7)
+    np.testing.assert_allclose(Z, Wa - np.eye(M), atol=1e-15)
+    np.testing.assert_allclose(Wa, dWa + Wma, atol=1e-15)
+    np.testing.assert_allclose(dWa, np.swapaxes(dWa, 1, 2), atol=1e-12)
+    # the LETKF update the kernels applied: Xa = xbar 1^T + dX Wa
+    xa = Xf.mean(axis=2, keepdims=True) + dXf @ Wa
+    np.testing.assert_allclose(Xa, xa, rtol=0, atol=2e-6 * np.abs(Xa).max())
+    # W 1 = sqrt(beta) 1 (the perturbations keep a zero mean); eig(Pa) = 1 / eig(C)
+    beta = 1.1
+    # (dY 1 = 0 up to the FP32 rounding of the perturbations and the split-bf16 Gram)
+    print("max |W 1 - sqrt(beta)|:", np.abs(dWa.sum(axis=2) - np.sqrt(beta)).max())
+    np.testing.assert_allclose(dWa.sum(axis=2), np.sqrt(beta), rtol=1e-4)
+    ev = np.diagonal(Lam, axis1=1, axis2=2)
+    assert (np.diff(ev, axis=1) >= 0).all()
+    np.testing.assert_allclose(np.sort(1 / np.linalg.eigvalsh(Pa), axis=1), ev, rtol=1e-8)
+    # the states of the matrices are those of the run's own output at that step
+    u_calc = _read(io, "calc", 1, "u", 5)
+    assert np.isfinite(u_calc).all()
