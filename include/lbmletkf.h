@@ -222,6 +222,23 @@ int lbmletkf_profile_enable(lbmletkf_ctx* ctx, int on);
 int lbmletkf_profile_read(lbmletkf_ctx* ctx, const char* phase, double* seconds,
                           long long* launches, long long* work);
 
+/* Newtonian nudging of every member towards observed fields, the reference's
+ * DA_NUDGING analysis (src/nudging.cu:40-142, da_w_itp): the observation lattice of
+ * stride xyprune is interpolated bilinearly (periodic) to every node and
+ * f_q := alpha feq_q(rho_o, u_o, v_o) + (1 - alpha) f_q.  rho, u, v: [ny][nx] fields
+ * (device or host) of which only the lattice nodes are read (xyprune = 1: every
+ * node); rho may be NULL (observations of u, v only: rho_o is the member's own
+ * density).  quadra (DA_QUADRA): odd leaves i >= nx/2 alone, >= 2 leaves j >= ny/2
+ * alone. */
+int lbmletkf_nudge(lbmletkf_ctx* ctx, const float* rho, const float* u, const float* v, int on_device,
+                   double alpha, int xyprune, int quadra);
+
+/* Perturbation of the Lyapunov runs (src/data.cu:426-456, src/main.cu:73-76):
+ * random velocities u_ref N(0,1) per node from std::mt19937(ensemble_id + 132333)
+ * and std::normal_distribution<float> (u then v, rows outer), and
+ * f_q := epsilon feq_q(rho, u_r, v_r) + (1 - epsilon) f_q. */
+int lbmletkf_perturb(lbmletkf_ctx* ctx, int member, unsigned long long ensemble_id, double epsilon);
+
 /* ---- learned eddy-viscosity closure ------------------------------------------
  * No counterpart in the reference (its LES model is Smagorinsky only,
  * src/lbm.cu:24-65): a per-node network 18 -> 64 -> 64 -> 1 (tanh-form GELU,

@@ -724,6 +724,53 @@ int lbmletkf_set_macro(lbmletkf_ctx* ctx, int member, const float* rho, const fl
     return LBMLETKF_OK;
 }
 
+int lbmletkf_nudge(lbmletkf_ctx* ctx, const float* rho, const float* u, const float* v, int on_device,
+                   double alpha, int xyprune, int quadra) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    CHECK_CTX(c);
+    if (!u || !v) return fail(c, LBMLETKF_ERR_ARG, "u and v are required");
+    if (!std::isfinite(alpha)) return fail(c, LBMLETKF_ERR_ARG, "alpha must be finite");
+    if (xyprune < 1 || xyprune > c->nx || xyprune > c->ny) return fail(c, LBMLETKF_ERR_ARG, "xyprune out of range");
+    if (quadra < 0 || quadra > 3) return fail(c, LBMLETKF_ERR_ARG, "quadra must be 0..3");
+    const float *dr = rho, *du = u, *dv = v;
+    if (!on_device) {
+        if (int st = ensure_scratch(c)) return st;
+        const size_t n = size_t(c->nodes);
+        if (rho) HIP_TRY(c, hipMemcpyAsync(c->d_truth, rho, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(c->d_truth + n, u, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(c->d_truth + 2 * n, v, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+        dr = rho ? c->d_truth : nullptr;
+        du = c->d_truth + n;
+        dv = c->d_truth + 2 * n;
+    }
+    HIP_TRY(c, launch_nudge(*c, dr, du, dv, float(alpha), xyprune, quadra % 2 == 1, quadra >= 2, rho != nullptr));
+    if (!on_device) HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return LBMLETKF_OK;
+}
+
+int lbmletkf_perturb(lbmletkf_ctx* ctx, int member, unsigned long long ensemble_id, double epsilon) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    CHECK_CTX(c);
+    CHECK_MEMBER(c, member);
+    if (!std::isfinite(epsilon)) return fail(c, LBMLETKF_ERR_ARG, "epsilon must be finite");
+    if (int st = ensure_scratch(c)) return st;
+    // src/data.cu:432-443: std::mt19937(id + 132333), normal_distribution<float>(0, 1),
+    // u then v per node, rows j outer, scaled by u_ref
+    std::mt19937 eng(uint32_t(ensemble_id + 132333ull));
+    std::normal_distribution<float> nd(0.f, 1.f);
+    const size_t n = size_t(c->nodes);
+    std::vector<float> uv(2 * n);
+    const float u_ref = float(c->cfg.u_ref);
+    for (size_t ij = 0; ij < n; ++ij) {
+        uv[ij] = u_ref * nd(eng);
+        uv[n + ij] = u_ref * nd(eng);
+    }
+    HIP_TRY(c, hipMemcpyAsync(c->d_truth, uv.data(), sizeof(float) * 2 * n, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, launch_perturb(*c, member, c->d_truth, c->d_truth + n, float(epsilon)));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return LBMLETKF_OK;
+}
+
 int lbmletkf_init_member(lbmletkf_ctx* ctx, int member, unsigned long long seed) {
     Ctx* c = reinterpret_cast<Ctx*>(ctx);
     CHECK_CTX(c);

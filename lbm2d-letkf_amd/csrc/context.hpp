@@ -154,6 +154,9 @@ hipError_t launch_lbm_fused(Ctx& c, const float* fin, float* fout);
 // tiles' nodes, indexed by node within the member
 hipError_t launch_lbm_mlp(Ctx& c, const float* fin, float* fout, int tile0, int n_tiles, float* nut_out);
 hipError_t launch_lbm_plain_member(Ctx& c, const float* fin, float* fout, int member);
+hipError_t launch_nudge(Ctx& c, const float* ro, const float* uo, const float* vo, float alpha, int xyprune,
+                        int skip_east, int skip_north, int use_obs_rho);
+hipError_t launch_perturb(Ctx& c, int member, const float* ur, const float* vr, float eps);
 hipError_t launch_set_feq(Ctx& c, int member, const float* d_rho, const float* d_u, const float* d_v);
 hipError_t launch_macro(Ctx& c, const float* f_member, float* rho, float* u, float* v);
 hipError_t launch_ensemble_stats(Ctx& c, const float* truth_rho, const float* truth_u,

@@ -230,6 +230,82 @@ __global__ void k_set_feq(float* __restrict__ f, const float* __restrict__ rho,
     for (int q = 0; q < Q; ++q) f[q * p.nodes + node] = feq[q];
 }
 
+// feq_q(rho, u, v) of every q by the formula (src/lbm.h:31-39), as the reference's
+// nudging and perturbation evaluate it per population
+__device__ __forceinline__ float feq_formula(float rho, float u, float v, int q, const LbmParams& p) {
+    const int ex = q % 3 - 1, ey = q / 3 - 1;
+    const float wq = (ex == 0 && ey == 0) ? W0 : ((ex == 0 || ey == 0) ? W1 : W2);
+    const float uc = (u * float(ex) + v * float(ey)) * p.inv_c;
+    const float uu = (u * u + v * v) * p.inv_c2;
+    return rho * wq * (1.f + 3.f * uc + 4.5f * uc * uc - 1.5f * uu);
+}
+
+// Newtonian nudging of every member towards the observed fields
+// (src/nudging.cu:40-142 with da_w_itp): the observation lattice (stride xyprune)
+// is interpolated bilinearly to each node (directly read when xyprune = 1), and
+// f_q := alpha feq_q(rho_o, u_o, v_o) + (1 - alpha) f_q.  skip_east / skip_north
+// leave the nodes with i >= nx/2 / j >= ny/2 untouched (DA_QUADRA); without an
+// observed density (use_obs_rho = 0) rho_o is the member's own density.
+__global__ void k_nudge(float* __restrict__ f, const float* __restrict__ ro, const float* __restrict__ uo,
+                        const float* __restrict__ vo, LbmParams p, float alpha, int xyprune, int skip_east,
+                        int skip_north, int use_obs_rho) {
+    const long long node = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= p.nodes) return;
+    const int nx = p.nx, ny = p.ny;
+    const int i = int(node % nx), j = int(node / nx);
+    if ((skip_east && i >= nx / 2) || (skip_north && j >= ny / 2)) return;
+    float rob, uob, vob;
+    if (xyprune == 1) {
+        rob = use_obs_rho ? ro[node] : 0.f;
+        uob = uo[node];
+        vob = vo[node];
+    } else {
+        const int iw = (i / xyprune) * xyprune, js = (j / xyprune) * xyprune;
+        const int ie = (iw + xyprune) % nx, jn = (js + xyprune) % ny;
+        const long long k0 = (long long)js * nx + iw, ke = (long long)js * nx + ie;
+        const long long kn = (long long)jn * nx + iw, ken = (long long)jn * nx + ie;
+        const double ir = double(float(i - iw) / float(xyprune)), jr = double(float(j - js) / float(xyprune));
+        const double c0 = (1. - ir) * (1. - jr), ce = ir * (1. - jr), cn = (1. - ir) * jr, cen = ir * jr;
+        uob = float(c0 * uo[k0] + ce * uo[ke] + cn * uo[kn] + cen * uo[ken]);
+        vob = float(c0 * vo[k0] + ce * vo[ke] + cn * vo[kn] + cen * vo[ken]);
+        rob = use_obs_rho ? float(c0 * ro[k0] + ce * ro[ke] + cn * ro[kn] + cen * ro[ken]) : 0.f;
+    }
+    for (int m = 0; m < p.n_members; ++m) {
+        float* fm = f + (long long)m * p.member_stride;
+        float fc[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) fc[q] = fm[q * p.nodes + node];
+        float r = rob;
+        if (!use_obs_rho) {
+            r = 0.f;
+#pragma unroll
+            for (int q = 0; q < Q; ++q) r += fc[q];
+        }
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+            fm[q * p.nodes + node] = alpha * feq_formula(r, uob, vob, q, p) + (1.f - alpha) * fc[q];
+    }
+}
+
+// Perturbation of the Lyapunov runs (src/data.cu:426-456): with random velocities
+// (ur, vr) per node, f_q := eps feq_q(rho, ur, vr) + (1 - eps) f_q, rho the node's
+// density.
+__global__ void k_perturb(float* __restrict__ f, const float* __restrict__ ur, const float* __restrict__ vr,
+                          LbmParams p, float eps) {
+    const long long node = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= p.nodes) return;
+    float fc[Q];
+    float r = 0.f;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        fc[q] = f[q * p.nodes + node];
+        r += fc[q];
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+        f[q * p.nodes + node] = eps * feq_formula(r, ur[node], vr[node], q, p) + (1.f - eps) * fc[q];
+}
+
 // rho, u, v of the stored populations (no streaming), src/letkf.cu:336-357
 __global__ void k_macro(const float* __restrict__ f, float* __restrict__ rho,
                         float* __restrict__ u, float* __restrict__ v, LbmParams p) {
@@ -338,6 +414,21 @@ hipError_t launch_set_feq(Ctx& c, int member, const float* d_rho, const float* d
     const int nt = 256;
     k_set_feq<<<node_blocks(c.nodes, nt), nt, 0, c.stream>>>(
         c.f[c.cur] + (long long)member * c.lp.member_stride, d_rho, d_u, d_v, c.lp);
+    return hipGetLastError();
+}
+
+hipError_t launch_nudge(Ctx& c, const float* ro, const float* uo, const float* vo, float alpha, int xyprune,
+                        int skip_east, int skip_north, int use_obs_rho) {
+    const int nt = 256;
+    k_nudge<<<node_blocks(c.nodes, nt), nt, 0, c.stream>>>(c.f[c.cur], ro, uo, vo, c.lp, alpha, xyprune, skip_east,
+                                                          skip_north, use_obs_rho);
+    return hipGetLastError();
+}
+
+hipError_t launch_perturb(Ctx& c, int member, const float* ur, const float* vr, float eps) {
+    const int nt = 256;
+    k_perturb<<<node_blocks(c.nodes, nt), nt, 0, c.stream>>>(c.f[c.cur] + (long long)member * c.lp.member_stride,
+                                                            ur, vr, c.lp, eps);
     return hipGetLastError();
 }
 

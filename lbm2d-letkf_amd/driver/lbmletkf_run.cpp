@@ -5,6 +5,8 @@
 //   lbmletkf_run observe  [options]   nature run + noisy observations io/observed/ens0000/*
 //   lbmletkf_run da_letkf [options]   ensemble with LETKF analyses read from io/observed
 //   lbmletkf_run free     [options]   the same ensemble without analyses (DA_DUMMY)
+//   lbmletkf_run da_nudging [options] ensemble nudged towards io/observed (DA_NUDGING)
+//   lbmletkf_run lyapunov [options]   ensemble perturbed after the spin-up (LYAPNOV)
 //
 // Main loop, as the reference's: for tt = 0 .. outputs, t = tt*iiter:
 //   analysis if t > 0 and (t/iiter) % daprune == 0      (src/letkf.cu:120-123)
@@ -31,6 +33,9 @@
 //   --no-nan                keep values at unobserved nodes (default: NaN, OBS_XYPRUNE_NAN)
 //   --stats                 also write the ensemble mean/std files (ENSEMBLE_STAT)
 //   --timing                print the phase timers at the end
+//   --nud-rate a            nudging rate alpha (DA_NUD_RATE, default 0.1)
+//   --quadra q              nudged quadrants (DA_QUADRA, default 0 = all)
+//   --ly-epsilon e          perturbation amplitude of lyapunov runs (LY_EPSILON, default 1e-6)
 //   --savemat [g0:count]    at each analysis write the LETKF matrices of gridpoints
 //                           [g0, g0+count) (default all) to <prefix>/letkf/mat/ens0
 //                           (src/util/letkf_solver.h:244-345, src/letkf.cu:277-306)
@@ -74,13 +79,17 @@ struct Options {
     bool nan_unobserved = true;
     bool stats = false;
     bool timing = false;
+    double nud_rate = 0.1;
+    int quadra = 0;
+    double ly_epsilon = 1e-6;
     bool savemat = false;
     long long savemat_g0 = 0, savemat_n = -1;
 };
 
 [[noreturn]] void usage(const char* msg) {
     std::fprintf(stderr, "lbmletkf_run: %s\n", msg);
-    std::fprintf(stderr, "usage: lbmletkf_run nature|observe|da_letkf|free [--preset ref|baseline] [--members M]\n"
+    std::fprintf(stderr, "usage: lbmletkf_run nature|observe|da_letkf|free|da_nudging|lyapunov [--preset ref|baseline]\n"
+                         "       [--members M] [--nud-rate a] [--quadra q] [--ly-epsilon e]\n"
                          "       [--ens-offset K] [--prefix DIR] [--outputs N] [--iiter n] [--ioprune n]\n"
                          "       [--daprune n] [--spinup n] [--nx n] [--variant plain|mixed] [--obs-stride p]\n"
                          "       [--loc-scale L] [--loc-cutoff d] [--inflation b] [--obs-seed S] [--no-nan]\n"
@@ -92,7 +101,8 @@ Options parse(int argc, char** argv) {
     if (argc < 2) usage("missing mode");
     Options o;
     o.mode = argv[1];
-    if (o.mode != "nature" && o.mode != "observe" && o.mode != "da_letkf" && o.mode != "free")
+    if (o.mode != "nature" && o.mode != "observe" && o.mode != "da_letkf" && o.mode != "free" &&
+        o.mode != "da_nudging" && o.mode != "lyapunov")
         usage(("unknown mode " + o.mode).c_str());
     for (int a = 2; a < argc; ++a) {
         const std::string k = argv[a];
@@ -123,6 +133,9 @@ Options parse(int argc, char** argv) {
         else if (k == "--no-nan") o.nan_unobserved = false;
         else if (k == "--stats") o.stats = true;
         else if (k == "--timing") o.timing = true;
+        else if (k == "--nud-rate") o.nud_rate = std::stod(val());
+        else if (k == "--quadra") o.quadra = std::stoi(val());
+        else if (k == "--ly-epsilon") o.ly_epsilon = std::stod(val());
         else if (k == "--savemat") {
             o.savemat = true;
             if (a + 1 < argc && argv[a + 1][0] != '-') {
@@ -304,7 +317,8 @@ int run(const Options& o) {
     if (o.inflation > 0) cfg.inflation = o.inflation;
     const int iiter = o.iiter > 0 ? o.iiter : std::max(1, cfg.da_interval / o.daprune);
     cfg.da_interval = iiter * o.daprune;
-    const bool ensemble = o.mode == "da_letkf" || o.mode == "free";
+    const bool ensemble = o.mode == "da_letkf" || o.mode == "free" || o.mode == "da_nudging" || o.mode == "lyapunov";
+    const bool assimilate = o.mode == "da_letkf" || o.mode == "da_nudging";
     const int M = ensemble ? o.members : 1;
     const int first_id = ensemble ? o.ens_offset : 0;
     if (int st = lbmletkf_config_check(&cfg, M)) {
@@ -324,6 +338,8 @@ int run(const Options& o) {
         c.ck(lbmletkf_forecast(c.h, int(o.spinup)), "forecast");
         c.ck(lbmletkf_set_step(c.h, 0), "set_step");
     }
+    if (o.mode == "lyapunov")  // src/main.cu:73-76
+        for (int m = 0; m < M; ++m) c.ck(lbmletkf_perturb(c.h, m, (unsigned long long)(first_id + m), o.ly_epsilon), "perturb");
     if (o.timing) c.ck(lbmletkf_profile_enable(c.h, 1), "profile_enable");
 
     std::mt19937 eng(o.obs_seed);
@@ -343,11 +359,20 @@ int run(const Options& o) {
     for (int tt = 0; tt <= o.outputs; ++tt) {
         const long long t = (long long)tt * iiter;
         const long long step = t / iiter;
-        if (o.mode == "da_letkf" && t > 0 && step % o.daprune == 0) {
+        if (assimilate && t > 0 && step % o.daprune == 0) {
             const std::string od = ens_dir(o.prefix, "observed", 0);
             read_field(step_file(od, "rho", step), obs.r.data(), n);
             read_field(step_file(od, "u", step), obs.u.data(), n);
             read_field(step_file(od, "v", step), obs.v.data(), n);
+        }
+        if (o.mode == "da_nudging" && t > 0 && step % o.daprune == 0) {
+            // src/nudging.cu:40-142 with DA_OBS_RUV: rho, u, v from the observation lattice
+            const bool obs_rho = (cfg.obs_vars & LBMLETKF_OBS_RHO) != 0;
+            c.ck(lbmletkf_nudge(c.h, obs_rho ? obs.r.data() : nullptr, obs.u.data(), obs.v.data(), 0, o.nud_rate,
+                                cfg.obs_stride, o.quadra), "nudge");
+            std::printf("t=%lld: nudging, rate %g\n", t, o.nud_rate);
+        }
+        if (o.mode == "da_letkf" && t > 0 && step % o.daprune == 0) {
             c.ck(lbmletkf_obs_from_fields(c.h, obs.r.data(), obs.u.data(), obs.v.data(), yo.data()), "obs_from_fields");
             std::vector<float> Xf;
             if (o.savemat) Xf = state_matrices(c, M, n, mat_g0, mat_n);

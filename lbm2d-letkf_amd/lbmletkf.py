@@ -103,6 +103,8 @@ def _load():
         "lbmletkf_profile_enable": ([P, I], I),
         "lbmletkf_profile_read": ([P, C.c_char_p, C.POINTER(D), C.POINTER(C.c_longlong),
                                    C.POINTER(C.c_longlong)], I),
+        "lbmletkf_nudge": ([P, P, P, P, I, D, I, I], I),
+        "lbmletkf_perturb": ([P, I, C.c_ulonglong, D], I),
         "lbmletkf_closure_init_params": ([C.c_ulonglong, P], I),
         "lbmletkf_set_closure": ([P, P], I),
         "lbmletkf_closure_nut": ([P, I, P], I),
@@ -304,6 +306,17 @@ class Context:
         sec, n, w = C.c_double(), C.c_longlong(), C.c_longlong()
         self._ck(lib.lbmletkf_profile_read(self.h, phase.encode(), C.byref(sec), C.byref(n), C.byref(w)))
         return sec.value, n.value, w.value
+
+    def nudge(self, u, v, rho=None, alpha: float = 0.1, xyprune: int = 1, quadra: int = 0):
+        """Newtonian nudging of every member towards observed (rho,) u, v fields."""
+        a = [None if x is None else np.ascontiguousarray(x, dtype=np.float32).reshape(self.ny, self.nx)
+             for x in (rho, u, v)]
+        ptrs = [None if x is None else _ptr(x) for x in a]
+        self._ck(lib.lbmletkf_nudge(self.h, ptrs[0], ptrs[1], ptrs[2], 0, alpha, xyprune, quadra))
+
+    def perturb(self, member: int, ensemble_id: int, epsilon: float):
+        """The Lyapunov runs' random-velocity perturbation of one member."""
+        self._ck(lib.lbmletkf_perturb(self.h, member, ensemble_id, epsilon))
 
     def set_closure(self, params: np.ndarray):
         """Load the learned closure (closure_kind = MLP contexts)."""

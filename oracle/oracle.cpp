@@ -818,3 +818,67 @@ void orc_lbm_step_closure(const orc_config* cfg, int M, const double* fin, doubl
         }
     }
 }
+
+// ---- nudging and the Lyapunov perturbation ---------------------------------------
+void orc_nudge(const orc_config* cfg, int M, double* f, const double* rho, const double* u, const double* v,
+               double alpha, int xyprune, int quadra) {
+    const orc_config& c = *cfg;
+    const int nx = c.nx, ny = c.ny;
+    const long long N = (long long)nx * ny;
+    const bool skip_east = quadra % 2 == 1, skip_north = quadra >= 2;  // src/nudging.cu:26-27
+#pragma omp parallel for
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            if ((skip_east && i >= nx / 2) || (skip_north && j >= ny / 2)) continue;
+            const long long ij = i + (long long)nx * j;
+            double ro = 0, uo, vo;
+            if (xyprune == 1) {
+                uo = u[ij];
+                vo = v[ij];
+                if (rho) ro = rho[ij];
+            } else {  // src/nudging.cu:100-126: bilinear from the lattice corners
+                const int iw = (i / xyprune) * xyprune, js = (j / xyprune) * xyprune;
+                const int ie = pmod(iw + xyprune, nx), jn = pmod(js + xyprune, ny);
+                const double ir = double(i - iw) / xyprune, jr = double(j - js) / xyprune;
+                auto itp = [&](const double* a) {
+                    return (1 - ir) * (1 - jr) * a[iw + (long long)nx * js] + ir * (1 - jr) * a[ie + (long long)nx * js] +
+                           (1 - ir) * jr * a[iw + (long long)nx * jn] + ir * jr * a[ie + (long long)nx * jn];
+                };
+                uo = itp(u);
+                vo = itp(v);
+                if (rho) ro = itp(rho);
+            }
+            for (int m = 0; m < M; ++m) {
+                double* fm = f + (long long)m * Q * N;
+                double r = ro;
+                if (!rho) {  // src/nudging.cu:128-131
+                    r = 0;
+                    for (int q = 0; q < Q; ++q) r += fm[q * N + ij];
+                }
+                for (int q = 0; q < Q; ++q)  // src/nudging.cu:133-138
+                    fm[q * N + ij] = alpha * feq_q(c, r, uo, vo, q) + (1 - alpha) * fm[q * N + ij];
+            }
+        }
+}
+
+void orc_perturb_draws(const orc_config* cfg, unsigned long long ensemble_id, double* ur, double* vr) {
+    std::mt19937 eng(uint32_t(ensemble_id + 132333ull));
+    std::normal_distribution<float> nd(0.f, 1.f);
+    const long long N = (long long)cfg->nx * cfg->ny;
+    const float u_ref = float(cfg->u_ref);
+    for (long long ij = 0; ij < N; ++ij) {
+        ur[ij] = u_ref * nd(eng);
+        vr[ij] = u_ref * nd(eng);
+    }
+}
+
+void orc_perturb(const orc_config* cfg, double* f, const double* ur, const double* vr, double eps) {
+    const orc_config& c = *cfg;
+    const long long N = (long long)c.nx * c.ny;
+#pragma omp parallel for
+    for (long long ij = 0; ij < N; ++ij) {
+        double r = 0;
+        for (int q = 0; q < Q; ++q) r += f[q * N + ij];
+        for (int q = 0; q < Q; ++q) f[q * N + ij] = eps * feq_q(c, r, ur[ij], vr[ij], q) + (1 - eps) * f[q * N + ij];
+    }
+}

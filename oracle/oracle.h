@@ -120,6 +120,17 @@ void   orc_coarse_grain(int nxf, int nyf, int F, const double* rho, const double
 void   orc_lbm_step_closure(const orc_config* cfg, int M, const double* fin, double* fout, long long step,
                             const double* params, int bf16);
 
+/* ---- nudging and the Lyapunov perturbation ---------------------------------- */
+/* src/nudging.cu:40-142 (da_w_itp): M members f [M][9][ny][nx] updated in place;
+ * rho may be NULL (rho_o = the member's own density) */
+void   orc_nudge(const orc_config* cfg, int M, double* f, const double* rho, const double* u, const double* v,
+                 double alpha, int xyprune, int quadra);
+/* src/data.cu:432-443: the random velocities of ensemble id (std::mt19937(id + 132333),
+ * std::normal_distribution<float>, u then v per node, rows outer, times u_ref) */
+void   orc_perturb_draws(const orc_config* cfg, unsigned long long ensemble_id, double* ur, double* vr);
+/* src/data.cu:445-453: f [9][ny][nx] := eps feq(rho, ur, vr) + (1 - eps) f */
+void   orc_perturb(const orc_config* cfg, double* f, const double* ur, const double* vr, double eps);
+
 #ifdef __cplusplus
 }
 #endif

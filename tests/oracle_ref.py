@@ -46,6 +46,9 @@ _sig = {
     "orc_closure_loss_grad": ([P, P, C.c_longlong, D, D, P], D),
     "orc_coarse_grain": ([I, I, I, P, P, P, D, D, P, P], None),
     "orc_lbm_step_closure": ([P, I, P, P, C.c_longlong, P, I], None),
+    "orc_nudge": ([P, I, P, P, P, P, D, I, I], None),
+    "orc_perturb_draws": ([P, C.c_ulonglong, P, P], None),
+    "orc_perturb": ([P, P, P, P, D], None),
 }
 CLOSURE_NPARAM = 5441
 CLOSURE_SAMPLE = 22
@@ -259,3 +262,25 @@ def lbm_step_closure(cfg, f, step: int, params, bf16=False):
     fout = np.empty_like(fin)
     _o.orc_lbm_step_closure(_c(cfg), fin.shape[0], _p(fin), _p(fout), step, _p(_f64(params)), int(bf16))
     return fout
+
+
+# ---- nudging and the Lyapunov perturbation -----------------------------------------
+def nudge(cfg, f, u, v, rho=None, alpha=0.1, xyprune=1, quadra=0):
+    out = _f64(f).copy()
+    M = out.shape[0]
+    _o.orc_nudge(_c(cfg), M, _p(out), None if rho is None else _p(_f64(rho)), _p(_f64(u)), _p(_f64(v)),
+                 alpha, xyprune, quadra)
+    return out
+
+
+def perturb_draws(cfg, ensemble_id):
+    ur = np.empty((cfg.ny, cfg.nx))
+    vr = np.empty((cfg.ny, cfg.nx))
+    _o.orc_perturb_draws(_c(cfg), ensemble_id, _p(ur), _p(vr))
+    return ur, vr
+
+
+def perturb(cfg, f, ur, vr, eps):
+    out = _f64(f).copy()
+    _o.orc_perturb(_c(cfg), _p(out), _p(_f64(ur)), _p(_f64(vr)), eps)
+    return out
