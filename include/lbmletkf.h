@@ -155,6 +155,15 @@ int lbmletkf_get_step(lbmletkf_ctx* ctx, long long* step);
  * set_rR (src/letkf.cu:59-118).  Called implicitly by the first analysis. */
 int lbmletkf_setup_obs(lbmletkf_ctx* ctx);
 
+/* Explicit observation network: sites at the given nodes (i + nx*j, distinct),
+ * site s = position in the list, replacing the lattice of cfg.obs_stride (uneven
+ * site densities; a gridpoint may have no local observation, in which case its
+ * analysis is xbar + sqrt(beta) dX).  Needs the Gaussian localization; the local
+ * set of a gridpoint lists the sites within the cutoff scanning the window by
+ * rows, then columns, ascending.  No counterpart in the reference, whose network
+ * is the DA_XYPRUNE lattice (src/letkf.cu:29-56). */
+int lbmletkf_set_obs_sites(lbmletkf_ctx* ctx, int n_sites, const int* nodes);
+
 /* Number of observed sites and of scalar observations (sites x observed vars).
  * Observation vectors are ordered site-major: o = site*n_vars + var, with site
  * s = sx + sy*(nx/stride) and vars in the order u, v, rho (those in the mask). */

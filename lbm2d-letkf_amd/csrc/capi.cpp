@@ -273,25 +273,38 @@ int setup_obs_impl(Ctx* c) {
     if (c->obs_ready) return LBMLETKF_OK;
     const auto& cfg = c->cfg;
     ObsNet& o = c->obs;
-    o.stride = cfg.obs_stride;
-    o.offset = cfg.obs_offset;
-    o.n_sx = c->nx / o.stride;
-    o.n_sy = c->ny / o.stride;
-    o.n_sites = o.n_sx * o.n_sy;
     o.n_vars = 0;
     const double isu = 1.0 / (cfg.obs_sigma_u * cfg.obs_sigma_u);
     const double isr = 1.0 / (cfg.obs_sigma_rho * cfg.obs_sigma_rho);
     if (cfg.obs_vars & LBMLETKF_OBS_U) { o.var_id[o.n_vars] = 0; o.inv_sigma2[o.n_vars++] = float(isu); }
     if (cfg.obs_vars & LBMLETKF_OBS_V) { o.var_id[o.n_vars] = 1; o.inv_sigma2[o.n_vars++] = float(isu); }
     if (cfg.obs_vars & LBMLETKF_OBS_RHO) { o.var_id[o.n_vars] = 2; o.inv_sigma2[o.n_vars++] = float(isr); }
+    if (o.custom) {
+        // explicit site list: the site map lets the localization scan find them
+        o.stride = 1;
+        o.offset = 0;
+        o.n_sites = int(o.site_nodes.size());
+        o.n_sx = o.n_sites;
+        o.n_sy = 1;
+        std::vector<int> map(c->nodes, -1);
+        for (int s = 0; s < o.n_sites; ++s) map[o.site_nodes[s]] = s;
+        HIP_TRY(c, dalloc(&c->d_site_map, c->nodes));
+        HIP_TRY(c, hipMemcpy(c->d_site_map, map.data(), sizeof(int) * c->nodes, hipMemcpyHostToDevice));
+    } else {
+        o.stride = cfg.obs_stride;
+        o.offset = cfg.obs_offset;
+        o.n_sx = c->nx / o.stride;
+        o.n_sy = c->ny / o.stride;
+        o.n_sites = o.n_sx * o.n_sy;
+        o.site_nodes.resize(o.n_sites);
+        for (int sy = 0; sy < o.n_sy; ++sy)
+            for (int sx = 0; sx < o.n_sx; ++sx)
+                o.site_nodes[sx + sy * o.n_sx] = (sx * o.stride + o.offset) + (sy * o.stride + o.offset) * c->nx;
+    }
     const int n_obs = o.n_sites * o.n_vars;
-
-    std::vector<int> site_node(o.n_sites);
-    for (int sy = 0; sy < o.n_sy; ++sy)
-        for (int sx = 0; sx < o.n_sx; ++sx)
-            site_node[sx + sy * o.n_sx] = (sx * o.stride + o.offset) + (sy * o.stride + o.offset) * c->nx;
-    HIP_TRY(c, dalloc(&c->d_site_node, o.n_sites));
-    HIP_TRY(c, hipMemcpy(c->d_site_node, site_node.data(), sizeof(int) * o.n_sites, hipMemcpyHostToDevice));
+    HIP_TRY(c, dalloc(&c->d_site_node, std::max(o.n_sites, 1)));
+    if (o.n_sites)
+        HIP_TRY(c, hipMemcpy(c->d_site_node, o.site_nodes.data(), sizeof(int) * o.n_sites, hipMemcpyHostToDevice));
 
     // local sets: count on device, scan on host, fill on device
     int* d_count = nullptr;
@@ -315,18 +328,19 @@ int setup_obs_impl(Ctx* c) {
     o.n_entries = acc;
     HIP_TRY(c, dalloc(&c->d_loc_ptr, c->nodes + 1));
     HIP_TRY(c, hipMemcpy(c->d_loc_ptr, ptr.data(), sizeof(int) * (c->nodes + 1), hipMemcpyHostToDevice));
-    HIP_TRY(c, dalloc(&c->d_loc_site, acc));
-    HIP_TRY(c, dalloc(&c->d_loc_w, acc));
+    HIP_TRY(c, dalloc(&c->d_loc_site, std::max(acc, 1LL)));
+    HIP_TRY(c, dalloc(&c->d_loc_w, std::max(acc, 1LL)));
     HIP_TRY(c, launch_local_fill(*c));
 
     float coef[3] = {o.inv_sigma2[0], o.inv_sigma2[1], o.inv_sigma2[2]};
     HIP_TRY(c, dalloc(&c->d_obs_coef, 3));
     HIP_TRY(c, hipMemcpy(c->d_obs_coef, coef, sizeof(coef), hipMemcpyHostToDevice));
-    HIP_TRY(c, dalloc(&c->d_yo, n_obs));
-    HIP_TRY(c, dalloc(&c->d_hx, (size_t)n_obs * c->M));
-    HIP_TRY(c, dalloc(&c->d_hx1, n_obs));
-    HIP_TRY(c, dalloc(&c->d_dy, (size_t)n_obs * c->M));
-    HIP_TRY(c, dalloc(&c->d_dyo, n_obs));
+    const int n_obs1 = std::max(n_obs, 1);
+    HIP_TRY(c, dalloc(&c->d_yo, n_obs1));
+    HIP_TRY(c, dalloc(&c->d_hx, (size_t)n_obs1 * c->M));
+    HIP_TRY(c, dalloc(&c->d_hx1, n_obs1));
+    HIP_TRY(c, dalloc(&c->d_dy, (size_t)n_obs1 * c->M));
+    HIP_TRY(c, dalloc(&c->d_dyo, n_obs1));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     c->obs_ready = true;
     return LBMLETKF_OK;
@@ -519,7 +533,7 @@ Ctx::~Ctx() {
     void* ptrs[] = {f[0], f[1], plain_ruv, plain_nus, force_xy, d_force, d_step, d_status,
                     d_site_node, d_loc_ptr, d_loc_site, d_loc_w, d_obs_coef, d_yo, d_hx, d_hx1, d_dy,
                     d_dyo, d_gram, d_bvec, d_mix_u, d_mix_b, d_dump, d_red, d_truth, d_sweeps,
-                    d_sweeps64, d_closure};
+                    d_sweeps64, d_closure, d_site_map};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (const ProfRec& r : prof_pending) {
@@ -871,6 +885,36 @@ int lbmletkf_get_step(lbmletkf_ctx* ctx, long long* step) {
     return LBMLETKF_OK;
 }
 
+// drop the observation network and everything sized by it
+void release_obs(Ctx* c) {
+    c->drop_graphs();
+    for (void** p : {(void**)&c->d_site_node, (void**)&c->d_loc_ptr, (void**)&c->d_loc_site, (void**)&c->d_loc_w,
+                     (void**)&c->d_obs_coef, (void**)&c->d_yo, (void**)&c->d_hx, (void**)&c->d_hx1,
+                     (void**)&c->d_dy, (void**)&c->d_dyo, (void**)&c->d_site_map}) {
+        if (*p) (void)hipFree(*p);
+        *p = nullptr;
+    }
+    c->obs_ready = false;
+}
+
+int lbmletkf_set_obs_sites(lbmletkf_ctx* ctx, int n_sites, const int* nodes) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    CHECK_CTX(c);
+    if (n_sites < 0 || (n_sites > 0 && !nodes)) return fail(c, LBMLETKF_ERR_ARG, "bad site list");
+    if (c->cfg.loc_kind != LBMLETKF_LOC_GAUSS)
+        return fail(c, LBMLETKF_ERR_ARG, "an explicit site list needs the Gaussian localization");
+    std::vector<char> seen(c->nodes, 0);
+    for (int s = 0; s < n_sites; ++s) {
+        if (nodes[s] < 0 || nodes[s] >= c->nodes) return fail(c, LBMLETKF_ERR_ARG, "site node out of the grid");
+        if (seen[nodes[s]]++) return fail(c, LBMLETKF_ERR_ARG, "duplicate site node");
+    }
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    release_obs(c);
+    c->obs.custom = true;
+    c->obs.site_nodes.assign(nodes, nodes + n_sites);
+    return setup_obs_impl(c);
+}
+
 int lbmletkf_setup_obs(lbmletkf_ctx* ctx) {
     Ctx* c = reinterpret_cast<Ctx*>(ctx);
     CHECK_CTX(c);
@@ -917,8 +961,7 @@ int lbmletkf_obs_from_fields(lbmletkf_ctx* ctx, const float* rho, const float* u
         if (!fields[c->obs.var_id[a]]) return fail(c, LBMLETKF_ERR_ARG, "a field of an observed variable is NULL");
     const ObsNet& o = c->obs;
     for (int s = 0; s < o.n_sites; ++s) {
-        const int sx = s % o.n_sx, sy = s / o.n_sx;
-        const long long node = (sx * o.stride + o.offset) + (long long)(sy * o.stride + o.offset) * c->nx;
+        const long long node = o.site_nodes[s];
         for (int a = 0; a < o.n_vars; ++a) yo_host[s * o.n_vars + a] = fields[o.var_id[a]][node];
     }
     return LBMLETKF_OK;

@@ -43,6 +43,9 @@ struct ObsNet {
     int n_vars = 0;
     int var_id[3] = {0, 0, 0};          // 0=u 1=v 2=rho, in observation order
     float inv_sigma2[3] = {0, 0, 0};    // R^-1 diagonal per variable (letkf.cu:174-178)
+    // sites: a lattice (stride, offset) or, when `custom`, an explicit node list
+    bool custom = false;
+    std::vector<int> site_nodes;        // node of each site (filled by setup for both kinds)
     int max_local = 0;                  // largest local site count over gridpoints
     long long n_entries = 0;            // total local (site, weight) pairs
 };
@@ -93,6 +96,7 @@ struct Ctx {
     int* d_loc_site = nullptr;    // [n_entries]
     float* d_loc_w = nullptr;     // [n_entries] localization weight (without R^-1)
     float* d_obs_coef = nullptr;  // [3] R^-1 per variable, device copy
+    int* d_site_map = nullptr;    // custom networks: [nodes] site index of each node or -1
 
     // analysis buffers
     float* d_yo = nullptr;        // [n_obs]

@@ -24,6 +24,7 @@ namespace {
 struct LocParams {
     int nx, ny, kind, stride, offset, n_sx, n_sy, rloc_len;
     double scale, cutoff;
+    const int* site_map;  // explicit site lists: site index of each node or -1; NULL for lattices
 };
 
 __host__ __device__ inline double gaspari_cohn(double d) {
@@ -71,6 +72,20 @@ __device__ inline void enumerate_local(const LocParams& L, int i, int j, Emit em
                 if (p > 1) w = gaspari_cohn(sqrt(double(di) * di + double(dj) * dj) / cutoff);
                 if (di > n_obs_local * p || dj > n_obs_local * p) w = 0.0;
                 emit(site, float(w));
+            }
+        }
+    } else if (L.site_map) {
+        // explicit sites: scan the (2R+1)^2 window row by row, ascending (dy, dx)
+        const int R = int(floor(L.cutoff));
+        const double R2 = L.cutoff * L.cutoff;
+        const double inv2L2 = 1.0 / (2.0 * L.scale * L.scale);
+        for (int dy = -R; dy <= R; ++dy) {
+            const int row = pmod(j + dy, L.ny) * L.nx;
+            for (int dx = -R; dx <= R; ++dx) {
+                const double d2 = double(dx * dx + dy * dy);
+                if (d2 > R2) continue;
+                const int site = L.site_map[row + pmod(i + dx, L.nx)];
+                if (site >= 0) emit(site, float(exp(-d2 * inv2L2)));
             }
         }
     } else {
@@ -474,6 +489,7 @@ LocParams make_loc(const Ctx& c) {
     L.rloc_len = c.cfg.loc_rloc_len;
     L.scale = c.cfg.loc_scale;
     L.cutoff = c.cfg.loc_cutoff;
+    L.site_map = c.obs.custom ? c.d_site_map : nullptr;
     return L;
 }
 
@@ -495,6 +511,7 @@ hipError_t launch_local_fill(Ctx& c) {
 hipError_t launch_obs_moments(Ctx& c, const float* f, double* hx, int member0, int n_members) {
     const int nt = 256;
     const long long n = (long long)c.obs.n_sites * n_members;
+    if (n == 0) return hipSuccess;  // a network without sites
     k_obs_moments<<<int((n + nt - 1) / nt), nt, 0, c.stream>>>(
         f, c.d_site_node, c.obs.n_sites, member0, n_members, c.lp.member_stride, c.nodes,
         c.cfg.c, c.obs.n_vars, c.obs.var_id[0], c.obs.var_id[1], c.obs.var_id[2], hx);
@@ -503,6 +520,7 @@ hipError_t launch_obs_moments(Ctx& c, const float* f, double* hx, int member0, i
 
 hipError_t launch_obs_prepare(Ctx& c) {
     const int n_obs = c.obs.n_sites * c.obs.n_vars;
+    if (n_obs == 0) return hipSuccess;
     k_obs_prepare<<<n_obs, 64, 0, c.stream>>>(c.d_hx, c.d_yo, n_obs, c.M, c.d_dy, c.d_dyo,
                                               c.d_status);
     return hipGetLastError();
@@ -511,6 +529,7 @@ hipError_t launch_obs_prepare(Ctx& c) {
 hipError_t launch_observe(Ctx& c, const double* hx_member, float* yo, uint64_t seed,
                           uint64_t counter, int noise) {
     const int n_obs = c.obs.n_sites * c.obs.n_vars;
+    if (n_obs == 0) return hipSuccess;
     const int nt = 256;
     k_observe<<<(n_obs + nt - 1) / nt, nt, 0, c.stream>>>(
         hx_member, yo, n_obs, c.obs.n_vars, c.obs.var_id[0], c.obs.var_id[1], c.obs.var_id[2],

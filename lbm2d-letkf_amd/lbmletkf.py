@@ -104,6 +104,7 @@ def _load():
         "lbmletkf_profile_read": ([P, C.c_char_p, C.POINTER(D), C.POINTER(C.c_longlong),
                                    C.POINTER(C.c_longlong)], I),
         "lbmletkf_nudge": ([P, P, P, P, I, D, I, I], I),
+        "lbmletkf_set_obs_sites": ([P, I, P], I),
         "lbmletkf_perturb": ([P, I, C.c_ulonglong, D], I),
         "lbmletkf_closure_init_params": ([C.c_ulonglong, P], I),
         "lbmletkf_set_closure": ([P, P], I),
@@ -307,6 +308,11 @@ class Context:
         self._ck(lib.lbmletkf_profile_read(self.h, phase.encode(), C.byref(sec), C.byref(n), C.byref(w)))
         return sec.value, n.value, w.value
 
+    def set_obs_sites(self, nodes):
+        """Explicit observation network: sites at the given node indices."""
+        a = np.ascontiguousarray(nodes, dtype=np.int32).ravel()
+        self._ck(lib.lbmletkf_set_obs_sites(self.h, a.size, _ptr(a)))
+
     def nudge(self, u, v, rho=None, alpha: float = 0.1, xyprune: int = 1, quadra: int = 0):
         """Newtonian nudging of every member towards observed (rho,) u, v fields."""
         a = [None if x is None else np.ascontiguousarray(x, dtype=np.float32).reshape(self.ny, self.nx)
@@ -344,6 +350,29 @@ class Context:
 
     def synchronize(self):
         self._ck(lib.lbmletkf_synchronize(self.h))
+
+
+def uneven_sites(nx: int, ny: int, seed: int, regions: int = 4, density=(0.005, 0.04),
+                 empty_regions: int = 0) -> np.ndarray:
+    """Observation nodes at a seeded random density per region: the grid is cut
+    into regions x regions blocks, each block drawing its density uniformly from
+    `density` and its sites without replacement; the first `empty_regions` blocks
+    (row-major) get none.  Returns the sorted node indices i + nx*j."""
+    rng = np.random.default_rng(seed)
+    bx, by = nx // regions, ny // regions
+    out = []
+    for r in range(regions * regions):
+        rx, ry = r % regions, r // regions
+        x0, y0 = rx * bx, ry * by
+        x1 = nx if rx == regions - 1 else x0 + bx
+        y1 = ny if ry == regions - 1 else y0 + by
+        d = rng.uniform(*density)
+        nodes = (np.arange(y0, y1)[:, None] * nx + np.arange(x0, x1)[None, :]).ravel()
+        k = int(round(d * nodes.size))
+        pick = rng.choice(nodes, size=k, replace=False)
+        if r >= empty_regions:
+            out.append(pick)
+    return np.sort(np.concatenate(out)).astype(np.int32) if out else np.zeros(0, np.int32)
 
 
 def closure_init_params(seed: int) -> np.ndarray:

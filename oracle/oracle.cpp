@@ -80,11 +80,27 @@ struct ObsInfo {
     double rinv[3];
 };
 
+// explicit observation sites (orc_set_obs_sites); empty: the lattice of cfg
+std::vector<int> g_sites;
+bool g_custom = false;
+std::vector<int> g_site_map;  // site index of each node, built for g_map_nodes nodes
+long long g_map_nodes = -1;
+
+const std::vector<int>& site_map(long long nodes) {
+#pragma omp critical(orc_site_map)
+    if (g_map_nodes != nodes) {
+        g_site_map.assign(size_t(nodes), -1);
+        for (int s = 0; s < int(g_sites.size()); ++s) g_site_map[g_sites[s]] = s;
+        g_map_nodes = nodes;
+    }
+    return g_site_map;
+}
+
 ObsInfo obs_info(const orc_config& c) {
     ObsInfo o{};
     o.n_sx = c.nx / c.obs_stride;
     o.n_sy = c.ny / c.obs_stride;
-    o.n_sites = o.n_sx * o.n_sy;
+    o.n_sites = g_custom ? int(g_sites.size()) : o.n_sx * o.n_sy;
     const double ru = 1.0 / (c.obs_sigma_u * c.obs_sigma_u), rr = 1.0 / (c.obs_sigma_rho * c.obs_sigma_rho);
     if (c.obs_vars & 1) { o.var_id[o.n_vars] = 0; o.rinv[o.n_vars++] = ru; }
     if (c.obs_vars & 2) { o.var_id[o.n_vars] = 1; o.rinv[o.n_vars++] = ru; }
@@ -319,6 +335,12 @@ void orc_initial_velocity(const orc_config* cfg, unsigned long long seed, double
     }
 }
 
+void orc_set_obs_sites(int n, const int* nodes) {
+    g_custom = nodes != nullptr && n >= 0;
+    g_sites.assign(nodes, nodes + (g_custom ? n : 0));
+    g_map_nodes = -1;
+}
+
 void orc_obs_count(const orc_config* cfg, int* n_sites, int* n_obs) {
     const ObsInfo o = obs_info(*cfg);
     if (n_sites) *n_sites = o.n_sites;
@@ -376,6 +398,17 @@ int orc_local_obs(const orc_config* cfg, int gridpoint, int* sites, double* weig
                 if (di > n_obs_local * p || dj > n_obs_local * p) w = 0.0;  // src/letkf.cu:265
                 put(iop / p + (jop / p) * o.n_sx, w);
             }
+    } else if (g_custom) {
+        // explicit sites inside the cutoff, window rows then columns ascending
+        const int R = int(std::floor(c.loc_cutoff));
+        const std::vector<int>& map = site_map((long long)c.nx * c.ny);
+        for (int dy = -R; dy <= R; ++dy)
+            for (int dx = -R; dx <= R; ++dx) {
+                const double d2 = double(dx * dx + dy * dy);
+                if (d2 > c.loc_cutoff * c.loc_cutoff) continue;
+                const int s = map[pmod(i + dx, c.nx) + (size_t)c.nx * pmod(j + dy, c.ny)];
+                if (s >= 0) put(s, std::exp(-d2 / (2 * c.loc_scale * c.loc_scale)));
+            }
     } else {
         const int R = int(std::floor(c.loc_cutoff));
         for (int sy = (int)std::ceil(double(j - c.obs_offset - R) / p); sy * p + c.obs_offset <= j + R; ++sy)
@@ -397,7 +430,8 @@ void orc_obs_moments(const orc_config* cfg, int M, const double* f, double* hx) 
 #pragma omp parallel for
     for (int s = 0; s < o.n_sites; ++s) {
         const int sx = s % o.n_sx, sy = s / o.n_sx;
-        const long long node = (sx * c.obs_stride + c.obs_offset) + (long long)c.nx * (sy * c.obs_stride + c.obs_offset);
+        const long long node = g_custom ? (long long)g_sites[s]
+                                        : (sx * c.obs_stride + c.obs_offset) + (long long)c.nx * (sy * c.obs_stride + c.obs_offset);
         for (int k = 0; k < M; ++k) {
             double r, u, v;
             moments(c, f + (long long)k * Q * N, N, node, r, u, v);

@@ -120,6 +120,11 @@ void   orc_coarse_grain(int nxf, int nyf, int F, const double* rho, const double
 void   orc_lbm_step_closure(const orc_config* cfg, int M, const double* fin, double* fout, long long step,
                             const double* params, int bf16);
 
+/* Explicit observation sites (node indices) replacing the lattice of cfg in every
+ * observation function that follows (n < 0 or nodes NULL: back to the lattice).
+ * Local sets then scan the cutoff window by rows, then columns, ascending. */
+void   orc_set_obs_sites(int n, const int* nodes);
+
 /* ---- nudging and the Lyapunov perturbation ---------------------------------- */
 /* src/nudging.cu:40-142 (da_w_itp): M members f [M][9][ny][nx] updated in place;
  * rho may be NULL (rho_o = the member's own density) */

@@ -47,6 +47,7 @@ _sig = {
     "orc_coarse_grain": ([I, I, I, P, P, P, D, D, P, P], None),
     "orc_lbm_step_closure": ([P, I, P, P, C.c_longlong, P, I], None),
     "orc_nudge": ([P, I, P, P, P, P, D, I, I], None),
+    "orc_set_obs_sites": ([I, P], None),
     "orc_perturb_draws": ([P, C.c_ulonglong, P, P], None),
     "orc_perturb": ([P, P, P, P, D], None),
 }
@@ -284,3 +285,17 @@ def perturb(cfg, f, ur, vr, eps):
     out = _f64(f).copy()
     _o.orc_perturb(_c(cfg), _p(out), _p(_f64(ur)), _p(_f64(vr)), eps)
     return out
+
+
+def set_obs_sites(nodes):
+    """Use an explicit site list (None: back to the configuration's lattice)."""
+    global _sites_keep
+    if nodes is None:
+        _sites_keep = None
+        _o.orc_set_obs_sites(-1, None)
+        return
+    _sites_keep = np.ascontiguousarray(nodes, dtype=np.int32)
+    _o.orc_set_obs_sites(_sites_keep.size, _sites_keep.ctypes.data_as(C.c_void_p))
+
+
+_sites_keep = None
