@@ -305,6 +305,43 @@ int lbmletkf_trainer_loss_grad(lbmletkf_trainer* tr, long long first, long long 
 int lbmletkf_trainer_train(lbmletkf_trainer* tr, int batch, int steps, double lr, double w_norm,
                            float* loss_history);
 
+/* ---- tuning ----------------------------------------------------------------------
+ * Launch shapes of the MIXED variant: row-band height of the fused LBM kernel
+ * (lbm_tiles_per_block tiles of 64 x 16 per workgroup) and the chain length of
+ * the FP32 solve kernel (consecutive gridpoints per wavefront, each warm-started
+ * from the previous one's eigenvectors; the result changes by FP32 rounding).
+ * No counterpart in the reference. */
+int lbmletkf_set_tuning(lbmletkf_ctx* ctx, int lbm_tiles_per_block, int mix_chain);
+int lbmletkf_get_tuning(lbmletkf_ctx* ctx, int* lbm_tiles_per_block, int* mix_chain);
+/* Autotuner: times each candidate on this device with the context's own state
+ * and observation network (the LBM into the idle half of the double buffer, the
+ * solve kernel alone on observations of member 0: the ensemble is not changed)
+ * and keeps the fastest.  cache_path (may be NULL) is a text file of
+ * "<device/problem key> <tiles per block> <chain>" lines: a hit is applied without
+ * timing, a miss is appended.  report (may be NULL) receives the timings. */
+int lbmletkf_autotune(lbmletkf_ctx* ctx, const char* cache_path, char* report, int report_len);
+
+/* ---- groups: several OSSEs in shared launches ----------------------------------
+ * No counterpart in the reference (one OSSE per run).  A group of 1-8 contexts of
+ * different grids and ensemble sizes (MIXED variant, grids multiples of 64 x 16,
+ * Smagorinsky closure, one gram_passes / FP64-sweep setting) is stepped by ONE
+ * LBM launch per step over every member of every context, and analysed by ONE
+ * launch of each analysis kernel over every gridpoint of every context; the
+ * group cycle (analysis, then da_interval steps, equal in all contexts) is
+ * captured as one hipGraph.  The group works on its own stream after draining
+ * the contexts' streams; do not use a member context concurrently with a group
+ * call.  yo_device[k] is the observation vector (device) of the k-th context as
+ * passed to create.  Errors of the group calls read from lbmletkf_last_error(NULL). */
+typedef struct lbmletkf_group lbmletkf_group;
+int lbmletkf_group_create(lbmletkf_ctx* const* ctxs, int n, lbmletkf_group** out);
+int lbmletkf_group_destroy(lbmletkf_group* g);
+int lbmletkf_group_forecast(lbmletkf_group* g, int n_steps);
+int lbmletkf_group_analysis(lbmletkf_group* g, const float* const* yo_device);
+int lbmletkf_group_cycle(lbmletkf_group* g, const float* const* yo_device);
+/* waits for the group's stream and reports a NaN flagged by any member analysis */
+int lbmletkf_group_synchronize(lbmletkf_group* g);
+void* lbmletkf_group_get_stream(lbmletkf_group* g);
+
 /* Stream access for callers that time or order work around the context. */
 void* lbmletkf_get_stream(lbmletkf_ctx* ctx);
 int lbmletkf_synchronize(lbmletkf_ctx* ctx);

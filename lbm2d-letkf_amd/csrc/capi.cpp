@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <cstdio>
 #include <cstring>
 #include <new>
 #include <random>
@@ -521,6 +522,7 @@ int cycle_graph(Ctx* c) {
 
 namespace lbmletkf {
 void Ctx::drop_graphs() {
+    ++graph_gen;
     if ((cycle_exec[0] || cycle_exec[1]) && stream) (void)hipStreamSynchronize(stream);
     for (hipGraphExec_t& x : cycle_exec) {
         if (x) (void)hipGraphExecDestroy(x);
@@ -1182,6 +1184,422 @@ int lbmletkf_synchronize(lbmletkf_ctx* ctx) {
     Ctx* c = reinterpret_cast<Ctx*>(ctx);
     CHECK_CTX(c);
     return sync_and_check(c);
+}
+
+// ---- groups: several contexts stepped and analysed by shared launches -------------
+}  // extern "C"
+
+namespace {
+
+struct Group {
+    std::vector<Ctx*> cs;       // launch order: widest analyses first
+    std::vector<int> user_pos;  // cs[k] is the caller's context user_pos[k]
+    hipStream_t stream = nullptr;
+    struct Exec {
+        std::vector<int> cur;
+        std::vector<unsigned> gen;
+        hipGraphExec_t x;
+    };
+    std::vector<Exec> execs;
+    std::string last_error;
+    void drop() {
+        if (!execs.empty() && stream) (void)hipStreamSynchronize(stream);
+        for (Exec& e : execs) (void)hipGraphExecDestroy(e.x);
+        execs.clear();
+    }
+    ~Group() {
+        drop();
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+// Points every context's stream at the group's for the duration of a group call
+// (their launchers use Ctx::stream); the contexts' own streams were drained first.
+struct StreamLend {
+    Group* g;
+    std::vector<hipStream_t> saved;
+    explicit StreamLend(Group* g_) : g(g_) {
+        for (Ctx* c : g->cs) {
+            saved.push_back(c->stream);
+            c->stream = g->stream;
+        }
+    }
+    ~StreamLend() {
+        for (size_t k = 0; k < g->cs.size(); ++k) g->cs[k]->stream = saved[k];
+    }
+};
+
+int gfail(Group* g, int code, const std::string& msg) {
+    if (g) g->last_error = msg;
+    return fail(nullptr, code, msg);
+}
+
+#define G_HIP(g, expr)                                                                                 \
+    do {                                                                                              \
+        hipError_t e_ = (expr);                                                                       \
+        if (e_ != hipSuccess) return gfail((g), LBMLETKF_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+int group_drain(Group* g) {
+    for (Ctx* c : g->cs) G_HIP(g, hipStreamSynchronize(c->stream));
+    return LBMLETKF_OK;
+}
+
+// per-context observation work (sizes fixed) before any capture
+int group_prepare(Group* g, const float* const* yo) {
+    for (size_t k = 0; k < g->cs.size(); ++k) {
+        Ctx* c = g->cs[k];
+        if (!yo[g->user_pos[k]]) return gfail(g, LBMLETKF_ERR_ARG, "yo of a context is NULL");
+        if (int st = setup_obs_impl(c)) return gfail(g, st, c->last_error);
+        if (int st = ensure_mixed_buffers(c)) return gfail(g, st, c->last_error);
+    }
+    return LBMLETKF_OK;
+}
+
+int group_analysis_launches(Group* g) {
+    for (Ctx* c : g->cs) {
+        G_HIP(g, launch_obs_moments(*c, c->f[c->cur], c->d_hx, 0, c->M));
+        G_HIP(g, launch_obs_prepare(*c));
+    }
+    G_HIP(g, launch_letkf_mixed_group(g->cs.data(), int(g->cs.size()), g->stream));
+    for (Ctx* c : g->cs) c->dump_kind = 1;
+    return LBMLETKF_OK;
+}
+
+int group_forecast_launches(Group* g, int n_steps) {
+    for (int s = 0; s < n_steps; ++s) {
+        for (Ctx* c : g->cs)
+            if (c->lp.has_force) G_HIP(g, launch_force_field(*c, s));
+        G_HIP(g, launch_lbm_fused_group(g->cs.data(), int(g->cs.size()), g->stream));
+        for (Ctx* c : g->cs) c->cur = 1 - c->cur;
+    }
+    if (n_steps > 0)
+        for (Ctx* c : g->cs) G_HIP(g, launch_step_advance(*c, n_steps));
+    for (Ctx* c : g->cs) c->step += n_steps;
+    return LBMLETKF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbmletkf_group_create(lbmletkf_ctx* const* ctxs, int n, lbmletkf_group** out) {
+    if (!ctxs || !out) return fail(nullptr, LBMLETKF_ERR_ARG, "NULL argument");
+    *out = nullptr;
+    if (n < 1 || n > MIXED_GROUP_MAX) return fail(nullptr, LBMLETKF_ERR_ARG, "a group holds 1 to 8 contexts");
+    std::vector<int> order(n);
+    for (int k = 0; k < n; ++k) {
+        Ctx* c = reinterpret_cast<Ctx*>(ctxs[k]);
+        if (!c) return fail(nullptr, LBMLETKF_ERR_ARG, "context is NULL");
+        for (int j = 0; j < k; ++j)
+            if (ctxs[j] == ctxs[k]) return fail(nullptr, LBMLETKF_ERR_ARG, "a context appears twice");
+        if (c->cfg.variant != LBMLETKF_VARIANT_MIXED) return fail(nullptr, LBMLETKF_ERR_ARG, "group contexts must use the MIXED variant");
+        if (!lbm_fused_supported(c->nx, c->ny)) return fail(nullptr, LBMLETKF_ERR_ARG, "group grids must be multiples of 64 x 16");
+        if (c->cfg.closure_kind != LBMLETKF_CLOSURE_SMAGORINSKY)
+            return fail(nullptr, LBMLETKF_ERR_ARG, "group contexts must use the Smagorinsky closure");
+        if (c->M < 2) return fail(nullptr, LBMLETKF_ERR_ARG, "group contexts need at least two members");
+        Ctx* c0 = reinterpret_cast<Ctx*>(ctxs[0]);
+        if (c->cfg.gram_passes != c0->cfg.gram_passes || (c->cfg.jacobi_fp64_sweeps > 0) != (c0->cfg.jacobi_fp64_sweeps > 0))
+            return fail(nullptr, LBMLETKF_ERR_ARG, "group contexts must share gram_passes and the FP64 sweep switch");
+        if (c->device != c0->device) return fail(nullptr, LBMLETKF_ERR_ARG, "group contexts must be on one device");
+        order[k] = k;
+    }
+    // widest analyses (then largest grids) first: the heaviest workgroups start first
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        const Ctx* ca = reinterpret_cast<Ctx*>(ctxs[a]);
+        const Ctx* cb = reinterpret_cast<Ctx*>(ctxs[b]);
+        const int na = mixed_width(ca->M), nb = mixed_width(cb->M);
+        return na != nb ? na > nb : ca->nodes > cb->nodes;
+    });
+    Group* g = new Group();
+    for (int k : order) {
+        g->cs.push_back(reinterpret_cast<Ctx*>(ctxs[k]));
+        g->user_pos.push_back(k);
+    }
+    const hipError_t e = hipStreamCreate(&g->stream);
+    if (e != hipSuccess) {
+        delete g;
+        return fail(nullptr, LBMLETKF_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    *out = reinterpret_cast<lbmletkf_group*>(g);
+    return LBMLETKF_OK;
+}
+
+int lbmletkf_group_destroy(lbmletkf_group* grp) {
+    delete reinterpret_cast<Group*>(grp);
+    return LBMLETKF_OK;
+}
+
+int lbmletkf_group_forecast(lbmletkf_group* grp, int n_steps) {
+    Group* g = reinterpret_cast<Group*>(grp);
+    if (!g) return fail(nullptr, LBMLETKF_ERR_ARG, "group is NULL");
+    if (n_steps < 0) return gfail(g, LBMLETKF_ERR_ARG, "n_steps must be >= 0");
+    if (int st = group_drain(g)) return st;
+    StreamLend lend(g);
+    return group_forecast_launches(g, n_steps);
+}
+
+int lbmletkf_group_analysis(lbmletkf_group* grp, const float* const* yo_device) {
+    Group* g = reinterpret_cast<Group*>(grp);
+    if (!g || !yo_device) return fail(nullptr, LBMLETKF_ERR_ARG, "NULL argument");
+    if (int st = group_drain(g)) return st;
+    if (int st = group_prepare(g, yo_device)) return st;
+    StreamLend lend(g);
+    for (size_t k = 0; k < g->cs.size(); ++k) {
+        Ctx* c = g->cs[k];
+        const int n_obs = c->obs.n_sites * c->obs.n_vars;
+        G_HIP(g, hipMemcpyAsync(c->d_yo, yo_device[g->user_pos[k]], sizeof(float) * n_obs, hipMemcpyDeviceToDevice,
+                                g->stream));
+    }
+    return group_analysis_launches(g);
+}
+
+int lbmletkf_group_cycle(lbmletkf_group* grp, const float* const* yo_device) {
+    Group* g = reinterpret_cast<Group*>(grp);
+    if (!g || !yo_device) return fail(nullptr, LBMLETKF_ERR_ARG, "NULL argument");
+    const int steps = g->cs[0]->cfg.da_interval;
+    for (Ctx* c : g->cs)
+        if (c->cfg.da_interval != steps) return gfail(g, LBMLETKF_ERR_ARG, "group cycles need one da_interval");
+    if (int st = group_drain(g)) return st;
+    if (int st = group_prepare(g, yo_device)) return st;
+    StreamLend lend(g);
+    for (size_t k = 0; k < g->cs.size(); ++k) {
+        Ctx* c = g->cs[k];
+        const int n_obs = c->obs.n_sites * c->obs.n_vars;
+        G_HIP(g, hipMemcpyAsync(c->d_yo, yo_device[g->user_pos[k]], sizeof(float) * n_obs, hipMemcpyDeviceToDevice,
+                                g->stream));
+    }
+    // one graph per (parities of the double buffers, buffer generations)
+    std::vector<int> cur;
+    std::vector<unsigned> gen;
+    for (Ctx* c : g->cs) {
+        cur.push_back(c->cur);
+        gen.push_back(c->graph_gen);
+    }
+    hipGraphExec_t x = nullptr;
+    for (auto it = g->execs.begin(); it != g->execs.end();) {
+        if (it->gen != gen) {  // a context changed its buffers since: stale
+            (void)hipGraphExecDestroy(it->x);
+            it = g->execs.erase(it);
+            continue;
+        }
+        if (it->cur == cur) x = it->x;
+        ++it;
+    }
+    if (!x) {
+        std::vector<long long> step0;
+        for (Ctx* c : g->cs) step0.push_back(c->step);
+        G_HIP(g, hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
+        int st = group_analysis_launches(g);
+        if (st == LBMLETKF_OK) st = group_forecast_launches(g, steps);
+        hipGraph_t gr = nullptr;
+        const hipError_t ec = hipStreamEndCapture(g->stream, &gr);
+        for (size_t k = 0; k < g->cs.size(); ++k) {
+            g->cs[k]->cur = cur[k];
+            g->cs[k]->step = step0[k];
+        }
+        if (st != LBMLETKF_OK) {
+            if (gr) (void)hipGraphDestroy(gr);
+            return st;
+        }
+        G_HIP(g, ec);
+        const hipError_t ei = hipGraphInstantiate(&x, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        G_HIP(g, ei);
+        g->execs.push_back({cur, gen, x});
+    }
+    G_HIP(g, hipGraphLaunch(x, g->stream));
+    for (Ctx* c : g->cs) {
+        if (steps & 1) c->cur = 1 - c->cur;
+        c->step += steps;
+        c->dump_kind = 1;
+    }
+    return LBMLETKF_OK;
+}
+
+int lbmletkf_group_synchronize(lbmletkf_group* grp) {
+    Group* g = reinterpret_cast<Group*>(grp);
+    if (!g) return fail(nullptr, LBMLETKF_ERR_ARG, "group is NULL");
+    G_HIP(g, hipStreamSynchronize(g->stream));
+    for (Ctx* c : g->cs)
+        if (int st = sync_and_check(c)) return gfail(g, st, c->last_error);
+    return LBMLETKF_OK;
+}
+
+void* lbmletkf_group_get_stream(lbmletkf_group* grp) {
+    Group* g = reinterpret_cast<Group*>(grp);
+    return g ? reinterpret_cast<void*>(g->stream) : nullptr;
+}
+
+// ---- tuning ------------------------------------------------------------------------
+
+int lbmletkf_set_tuning(lbmletkf_ctx* ctx, int lbm_tiles_per_block, int mix_chain) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    CHECK_CTX(c);
+    if (lbm_tiles_per_block < 1 || lbm_tiles_per_block > 16 || mix_chain < 1 || mix_chain > 64)
+        return fail(c, LBMLETKF_ERR_ARG, "tiles per block must be in [1, 16] and the chain in [1, 64]");
+    if (lbm_fused_supported(c->nx, c->ny) && ((long long)(c->nx / 64) * (c->ny / 16) * c->M) % lbm_tiles_per_block)
+        return fail(c, LBMLETKF_ERR_ARG, "tiles per block must divide the number of tiles");
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->drop_graphs();
+    c->lbm_tiles_per_block = lbm_tiles_per_block;
+    c->mix_chain = mix_chain;
+    return LBMLETKF_OK;
+}
+
+int lbmletkf_get_tuning(lbmletkf_ctx* ctx, int* lbm_tiles_per_block, int* mix_chain) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    CHECK_CTX(c);
+    if (lbm_tiles_per_block) *lbm_tiles_per_block = c->lbm_tiles_per_block;
+    if (mix_chain) *mix_chain = c->mix_chain;
+    return LBMLETKF_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// mean time of `reps` launches of fn on the context's stream after `warm` untimed ones
+template <class Fn>
+int time_launches(Ctx* c, int warm, int reps, Fn fn, double& us) {
+    for (int k = 0; k < warm; ++k) HIP_TRY(c, fn());
+    hipEvent_t a = prof_event(c), b = prof_event(c);
+    if (!a || !b) return fail(c, LBMLETKF_ERR_HIP, "hipEventCreate failed");
+    struct Back {
+        Ctx* c;
+        hipEvent_t a, b;
+        ~Back() {
+            c->event_pool.push_back(a);
+            c->event_pool.push_back(b);
+        }
+    } back{c, a, b};
+    HIP_TRY(c, hipEventRecord(a, c->stream));
+    for (int k = 0; k < reps; ++k) HIP_TRY(c, fn());
+    HIP_TRY(c, hipEventRecord(b, c->stream));
+    HIP_TRY(c, hipEventSynchronize(b));
+    float ms = 0.f;
+    HIP_TRY(c, hipEventElapsedTime(&ms, a, b));
+    us = 1e3 * double(ms) / reps;
+    return LBMLETKF_OK;
+}
+
+std::string tune_key(const Ctx* c) {
+    hipDeviceProp_t prop{};
+    std::string dev = "unknown";
+    if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) dev = prop.gcnArchName;
+    for (char& ch : dev)
+        if (ch == ' ' || ch == ':') ch = '_';
+    const auto& g = c->cfg;
+    return dev + "/" + std::to_string(c->nx) + "x" + std::to_string(c->ny) + "/M" + std::to_string(c->M) + "/v" +
+           std::to_string(g.variant) + "/p" + std::to_string(g.gram_passes) + "/f" +
+           std::to_string(g.jacobi_fp64_sweeps > 0) + "/s" + std::to_string(c->obs.n_sites) + "/e" +
+           std::to_string(c->obs.n_entries);
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbmletkf_autotune(lbmletkf_ctx* ctx, const char* cache_path, char* report, int report_len) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    CHECK_CTX(c);
+    if (c->prof) return fail(c, LBMLETKF_ERR_STATE, "switch phase timing off before tuning");
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    const bool mixed = c->cfg.variant == LBMLETKF_VARIANT_MIXED;
+    if (mixed) {
+        if (c->M < 2) return fail(c, LBMLETKF_ERR_ARG, "the analysis needs at least two members");
+        if (int st = setup_obs_impl(c)) return st;
+        if (int st = ensure_mixed_buffers(c)) return st;
+    }
+    const std::string key = tune_key(c);
+    std::string rep;
+    auto put_report = [&]() {
+        if (report && report_len > 0) {
+            std::strncpy(report, rep.c_str(), size_t(report_len) - 1);
+            report[report_len - 1] = '\0';
+        }
+    };
+    // a cached choice for this device and problem
+    if (cache_path) {
+        if (std::FILE* fp = std::fopen(cache_path, "r")) {
+            char line[512];
+            while (std::fgets(line, sizeof line, fp)) {
+                char k[400];
+                int tpb = 0, ch = 0;
+                if (std::sscanf(line, "%399s %d %d", k, &tpb, &ch) == 3 && key == k) {
+                    std::fclose(fp);
+                    if (int st = lbmletkf_set_tuning(ctx, tpb, ch)) return st;
+                    rep = "cached " + key + ": lbm_tiles_per_block=" + std::to_string(tpb) +
+                          " mix_chain=" + std::to_string(ch);
+                    put_report();
+                    return LBMLETKF_OK;
+                }
+            }
+            std::fclose(fp);
+        }
+    }
+    c->drop_graphs();
+    char buf[128];
+    // 1. row-band height of the fused LBM kernel: 1, 2 or 4 tiles of 64x16 per workgroup
+    //    (the step writes the idle half of the double buffer only)
+    int best_tpb = c->lbm_tiles_per_block;
+    if (mixed && lbm_fused_supported(c->nx, c->ny)) {
+        const long long tiles = (long long)(c->nx / 64) * (c->ny / 16) * c->M;
+        double best = 1e300;
+        rep += "lbm us/step:";
+        for (int tpb : {1, 2, 4}) {
+            if (tiles % tpb) continue;
+            c->lbm_tiles_per_block = tpb;
+            double us = 0;
+            if (int st = time_launches(c, 2, 10, [&] { return launch_lbm_fused(*c, c->f[c->cur], c->f[1 - c->cur]); }, us))
+                return st;
+            std::snprintf(buf, sizeof buf, " %d:%.1f", tpb, us);
+            rep += buf;
+            if (us < best) {
+                best = us;
+                best_tpb = tpb;
+            }
+        }
+        c->lbm_tiles_per_block = best_tpb;
+    }
+    // 2. chain length of the FP32 solve kernel (consecutive gridpoints per wavefront,
+    //    warm-started from each other), timed on the ensemble's own local problems with
+    //    observations of member 0
+    int best_chain = c->mix_chain;
+    if (mixed) {
+        HIP_TRY(c, launch_obs_moments(*c, c->f[c->cur], c->d_hx1, 0, 1));
+        HIP_TRY(c, launch_observe(*c, c->d_hx1, c->d_yo, 0x7475ull, 1, 1));
+        HIP_TRY(c, launch_obs_moments(*c, c->f[c->cur], c->d_hx, 0, c->M));
+        HIP_TRY(c, launch_obs_prepare(*c));
+        double best = 1e300;
+        rep += std::string(rep.empty() ? "" : "; ") + "solve us:";
+        for (int ch : {1, 2, 4, 8, 16}) {
+            c->mix_chain = ch;
+            double us = 0;
+            if (int st = time_launches(c, 1, 3, [&] {
+                    return launch_letkf_mixed(*c, c->f[c->cur], nullptr, nullptr, c->cfg.gram_passes, nullptr, nullptr, true);
+                }, us))
+                return st;
+            std::snprintf(buf, sizeof buf, " %d:%.1f", ch, us);
+            rep += buf;
+            if (us < best) {
+                best = us;
+                best_chain = ch;
+            }
+        }
+        c->mix_chain = best_chain;
+        if (int st = sync_and_check(c)) return st;
+    }
+    rep = key + ": lbm_tiles_per_block=" + std::to_string(best_tpb) + " mix_chain=" + std::to_string(best_chain) +
+          " (" + rep + ")";
+    if (cache_path) {
+        if (std::FILE* fp = std::fopen(cache_path, "a")) {
+            std::fprintf(fp, "%s %d %d\n", key.c_str(), best_tpb, best_chain);
+            std::fclose(fp);
+        }
+    }
+    put_report();
+    return LBMLETKF_OK;
 }
 
 }  // extern "C"

@@ -13,6 +13,7 @@ namespace lbmletkf {
 constexpr int Q = 9;
 constexpr int MAX_MEMBERS = 64;      // one wavefront of lanes per ensemble matrix
 constexpr int MAX_FORCE_MODES = 100; // src/data.cu:258 prunes to 100 modes
+constexpr int MIXED_GROUP_MAX = 8;   // contexts in one group launch
 
 // Per-step LBM constants, in the arithmetic type of the kernels (FP32).
 struct LbmParams {
@@ -139,6 +140,7 @@ struct Ctx {
     // parity of `cur` (the captured kernels name the current population buffer)
     hipGraphExec_t cycle_exec[2] = {nullptr, nullptr};
     void drop_graphs();
+    unsigned graph_gen = 0;       // bumped by drop_graphs: graphs of groups holding this context are stale
 
     // learned closure (closure_kind = MLP): the network's weight tiles
     ClosureWeights* d_closure = nullptr;
@@ -153,6 +155,12 @@ int report_error(Ctx* c, int code, const std::string& msg);
 // ---- launchers (lbm.hip) ------------------------------------------------------
 hipError_t launch_force_field(Ctx& c, int step_offset);
 hipError_t launch_lbm_fused(Ctx& c, const float* fin, float* fout);
+// one step of every member of every context in one launch (fused kernel, contexts
+// of fused-supported sizes); each context advances from f[cur] into f[1 - cur]
+hipError_t launch_lbm_fused_group(Ctx* const* cs, int n, hipStream_t stream);
+// the MIXED analysis of several contexts (obs already prepared) in one launch of
+// each of its two kernels, on `stream`
+hipError_t launch_letkf_mixed_group(Ctx* const* cs, int n, hipStream_t stream);
 // one step with the learned closure for tiles [tile0, tile0 + n_tiles) of the
 // 64x16 tiling (member-major); nut_out (may be NULL) receives nu_t of those
 // tiles' nodes, indexed by node within the member
@@ -181,9 +189,10 @@ hipError_t launch_solve_fp64(Ctx& c, float* f, unsigned long long* sweeps);
 
 // ---- launchers (letkf_mixed.hip) ----------------------------------------------
 int mixed_width(int M);
-// ev_mid_end / ev_mid_begin (may be null) are recorded between the two kernels
+// ev_mid_end / ev_mid_begin (may be null) are recorded between the two kernels;
+// solve_only launches the first kernel alone (workspace only, the state untouched)
 hipError_t launch_letkf_mixed(Ctx& c, float* f, unsigned long long* sweeps32,
                               unsigned long long* sweeps64, int passes, hipEvent_t ev_mid_end,
-                              hipEvent_t ev_mid_begin);
+                              hipEvent_t ev_mid_begin, bool solve_only = false);
 
 }  // namespace lbmletkf

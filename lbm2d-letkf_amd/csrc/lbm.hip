@@ -33,6 +33,79 @@ constexpr int ROWS_PER_PASS = NT / TX;        // 4
 constexpr int NODES_PER_THREAD = TY / ROWS_PER_PASS;  // 4
 constexpr int HALO = 2 * (TX + 2) + 2 * TY;   // 164
 
+// Tile t (member-major order of 64x16 tiles) of one step.
+template <bool FORCE>
+__device__ __forceinline__ void lbm_tile(int t, const float* __restrict__ fin, float* __restrict__ fout,
+                                         const float* __restrict__ force, const LbmParams& p, int tiles_x,
+                                         int tiles_per_member) {
+    const int nx = p.nx, ny = p.ny;
+    const long long nodes = p.nodes;
+    __shared__ float su[TY + 2][TX + 2];
+    __shared__ float sv[TY + 2][TX + 2];
+    const int lx = threadIdx.x % TX;
+    const int ly0 = threadIdx.x / TX;
+    const int member = t / tiles_per_member;
+    const int tt = t - member * tiles_per_member;
+    const int x0 = (tt % tiles_x) * TX;
+    const int y0 = (tt / tiles_x) * TY;
+    const float* f = fin + (long long)member * p.member_stride;
+    float* fo = fout + (long long)member * p.member_stride;
+    const int i = x0 + lx;
+    const int im = wrap(i - 1, nx), ip = wrap(i + 1, nx);
+
+    __syncthreads();  // the previous tile's stencil reads of su/sv are done
+    float fc[NODES_PER_THREAD][Q];
+    float rr[NODES_PER_THREAD], uu[NODES_PER_THREAD], vv[NODES_PER_THREAD];
+#pragma unroll
+    for (int r = 0; r < NODES_PER_THREAD; ++r) {
+        const int ly = ly0 + r * ROWS_PER_PASS;
+        const int j = y0 + ly;
+        const long long r0 = (long long)j * nx;
+        const long long rm = (long long)wrap(j - 1, ny) * nx;
+        const long long rp = (long long)wrap(j + 1, ny) * nx;
+        pull9(f, nodes, im, i, ip, rm, r0, rp, fc[r]);
+        moments(fc[r], p.c, rr[r], uu[r], vv[r]);
+        su[ly + 1][lx + 1] = uu[r];
+        sv[ly + 1][lx + 1] = vv[r];
+    }
+    // halo ring of streamed velocities
+    if (threadIdx.x < HALO) {
+        int h = threadIdx.x, hx, hy;
+        if (h < TX + 2) { hx = h - 1; hy = -1; }
+        else if (h < 2 * (TX + 2)) { hx = h - (TX + 2) - 1; hy = TY; }
+        else if (h < 2 * (TX + 2) + TY) { hx = -1; hy = h - 2 * (TX + 2); }
+        else { hx = TX; hy = h - 2 * (TX + 2) - TY; }
+        const int gi = wrap(x0 + hx, nx), gj = wrap(y0 + hy, ny);
+        const int gim = wrap(gi - 1, nx), gip = wrap(gi + 1, nx);
+        const long long r0 = (long long)gj * nx;
+        const long long rm = (long long)wrap(gj - 1, ny) * nx;
+        const long long rp = (long long)wrap(gj + 1, ny) * nx;
+        float fh[Q], rh, uh, vh;
+        pull9(f, nodes, gim, gi, gip, rm, r0, rp, fh);
+        moments(fh, p.c, rh, uh, vh);
+        su[hy + 1][hx + 1] = uh;
+        sv[hy + 1][hx + 1] = vh;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < NODES_PER_THREAD; ++r) {
+        const int ly = ly0 + r * ROWS_PER_PASS;
+        const int j = y0 + ly;
+        const float nut = smagorinsky(su[ly + 1][lx + 2], su[ly + 1][lx], su[ly + 2][lx + 1],
+                                      su[ly][lx + 1], sv[ly + 1][lx + 2], sv[ly + 1][lx],
+                                      sv[ly + 2][lx + 1], sv[ly][lx + 1], p);
+        const long long node = (long long)j * nx + i;
+        float Fx = 0.f, Fy = 0.f;
+        if (FORCE && p.has_force) {
+            Fx = force[node];
+            Fy = force[nodes + node];
+        }
+        collide9<FORCE>(fc[r], rr[r], uu[r], vv[r], nut, Fx, Fy, p);
+#pragma unroll
+        for (int q = 0; q < Q; ++q) fo[q * nodes + node] = fc[r][q];
+    }
+}
+
 template <bool FORCE>
 __global__ __launch_bounds__(NT) void k_lbm_fused(const float* __restrict__ fin,
                                                   float* __restrict__ fout,
@@ -45,78 +118,33 @@ __global__ __launch_bounds__(NT) void k_lbm_fused(const float* __restrict__ fin,
     const int nb = gridDim.x;
     const int b = blockIdx.x;
     const int bb = (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
-    const int nx = p.nx, ny = p.ny;
-    const long long nodes = p.nodes;
+    for (int k = 0; k < tiles_per_block; ++k)
+        lbm_tile<FORCE>(bb * tiles_per_block + k, fin, fout, force, p, tiles_x, tiles_per_member);
+}
 
-    __shared__ float su[TY + 2][TX + 2];
-    __shared__ float sv[TY + 2][TX + 2];
+// One step of several contexts: each task owns a range of workgroups (one tile
+// each, the range padded to a multiple of 8 for the XCD-contiguous order).
+struct LbmTask {
+    const float* fin;
+    float* fout;
+    const float* force;
+    LbmParams p;
+    int tiles_x, tiles_per_member, tiles, b0, nb;
+};
+struct LbmGroup {
+    int n;
+    LbmTask t[MIXED_GROUP_MAX];
+};
 
-    const int lx = threadIdx.x % TX;
-    const int ly0 = threadIdx.x / TX;
-
-    for (int k = 0; k < tiles_per_block; ++k) {
-        const int t = bb * tiles_per_block + k;
-        const int member = t / tiles_per_member;
-        const int tt = t - member * tiles_per_member;
-        const int x0 = (tt % tiles_x) * TX;
-        const int y0 = (tt / tiles_x) * TY;
-        const float* f = fin + (long long)member * p.member_stride;
-        float* fo = fout + (long long)member * p.member_stride;
-        const int i = x0 + lx;
-        const int im = wrap(i - 1, nx), ip = wrap(i + 1, nx);
-
-        __syncthreads();  // the previous tile's stencil reads of su/sv are done
-        float fc[NODES_PER_THREAD][Q];
-        float rr[NODES_PER_THREAD], uu[NODES_PER_THREAD], vv[NODES_PER_THREAD];
-#pragma unroll
-        for (int r = 0; r < NODES_PER_THREAD; ++r) {
-            const int ly = ly0 + r * ROWS_PER_PASS;
-            const int j = y0 + ly;
-            const long long r0 = (long long)j * nx;
-            const long long rm = (long long)wrap(j - 1, ny) * nx;
-            const long long rp = (long long)wrap(j + 1, ny) * nx;
-            pull9(f, nodes, im, i, ip, rm, r0, rp, fc[r]);
-            moments(fc[r], p.c, rr[r], uu[r], vv[r]);
-            su[ly + 1][lx + 1] = uu[r];
-            sv[ly + 1][lx + 1] = vv[r];
-        }
-        // halo ring of streamed velocities
-        if (threadIdx.x < HALO) {
-            int h = threadIdx.x, hx, hy;
-            if (h < TX + 2) { hx = h - 1; hy = -1; }
-            else if (h < 2 * (TX + 2)) { hx = h - (TX + 2) - 1; hy = TY; }
-            else if (h < 2 * (TX + 2) + TY) { hx = -1; hy = h - 2 * (TX + 2); }
-            else { hx = TX; hy = h - 2 * (TX + 2) - TY; }
-            const int gi = wrap(x0 + hx, nx), gj = wrap(y0 + hy, ny);
-            const int gim = wrap(gi - 1, nx), gip = wrap(gi + 1, nx);
-            const long long r0 = (long long)gj * nx;
-            const long long rm = (long long)wrap(gj - 1, ny) * nx;
-            const long long rp = (long long)wrap(gj + 1, ny) * nx;
-            float fh[Q], rh, uh, vh;
-            pull9(f, nodes, gim, gi, gip, rm, r0, rp, fh);
-            moments(fh, p.c, rh, uh, vh);
-            su[hy + 1][hx + 1] = uh;
-            sv[hy + 1][hx + 1] = vh;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int r = 0; r < NODES_PER_THREAD; ++r) {
-            const int ly = ly0 + r * ROWS_PER_PASS;
-            const int j = y0 + ly;
-            const float nut = smagorinsky(su[ly + 1][lx + 2], su[ly + 1][lx], su[ly + 2][lx + 1],
-                                          su[ly][lx + 1], sv[ly + 1][lx + 2], sv[ly + 1][lx],
-                                          sv[ly + 2][lx + 1], sv[ly][lx + 1], p);
-            const long long node = (long long)j * nx + i;
-            float Fx = 0.f, Fy = 0.f;
-            if (FORCE && p.has_force) {
-                Fx = force[node];
-                Fy = force[nodes + node];
-            }
-            collide9<FORCE>(fc[r], rr[r], uu[r], vv[r], nut, Fx, Fy, p);
-#pragma unroll
-            for (int q = 0; q < Q; ++q) fo[q * nodes + node] = fc[r][q];
-        }
-    }
+template <bool FORCE>
+__global__ __launch_bounds__(NT) void k_lbm_fused_group(LbmGroup G) {
+    int k = 0;
+    while (k + 1 < G.n && int(blockIdx.x) >= G.t[k + 1].b0) ++k;
+    const LbmTask& T = G.t[k];
+    const int b = int(blockIdx.x) - T.b0;
+    const int bb = (b % 8) * (T.nb / 8) + b / 8;
+    if (bb >= T.tiles) return;  // padding (uniform per workgroup)
+    lbm_tile<FORCE>(bb, T.fin, T.fout, T.force, T.p, T.tiles_x, T.tiles_per_member);
 }
 
 // ----------------------------------------------------------------------------
@@ -396,6 +424,36 @@ hipError_t launch_lbm_fused(Ctx& c, const float* fin, float* fout) {
         k_lbm_fused<true><<<nb, NT, 0, c.stream>>>(fin, fout, c.force_xy, c.lp, tiles_x, tiles, per_block);
     else
         k_lbm_fused<false><<<nb, NT, 0, c.stream>>>(fin, fout, c.force_xy, c.lp, tiles_x, tiles, per_block);
+    return hipGetLastError();
+}
+
+hipError_t launch_lbm_fused_group(Ctx* const* cs, int n, hipStream_t stream) {
+    if (n < 1 || n > MIXED_GROUP_MAX) return hipErrorInvalidValue;
+    LbmGroup G{};
+    G.n = n;
+    long long b0 = 0;
+    bool force = false;
+    for (int k = 0; k < n; ++k) {
+        Ctx& c = *cs[k];
+        if (!lbm_fused_supported(c.nx, c.ny)) return hipErrorInvalidValue;
+        LbmTask& T = G.t[k];
+        T.fin = c.f[c.cur];
+        T.fout = c.f[1 - c.cur];
+        T.force = c.force_xy;
+        T.p = c.lp;
+        T.tiles_x = c.nx / TX;
+        T.tiles_per_member = T.tiles_x * (c.ny / TY);
+        T.tiles = T.tiles_per_member * c.M;
+        T.nb = (T.tiles + 7) / 8 * 8;
+        T.b0 = int(b0);
+        b0 += T.nb;
+        force = force || c.lp.has_force || c.lp.fric_over_dt != 0.f;
+    }
+    if (b0 > (1LL << 31) - 1) return hipErrorInvalidValue;
+    if (force)
+        k_lbm_fused_group<true><<<unsigned(b0), NT, 0, stream>>>(G);
+    else
+        k_lbm_fused_group<false><<<unsigned(b0), NT, 0, stream>>>(G);
     return hipGetLastError();
 }
 

@@ -52,6 +52,28 @@ struct MixedParams {
     unsigned long long* sweeps64;
 };
 
+// one analysis of a group launch
+struct MixedTask {
+    const int* loc_ptr;
+    const int* loc_site;
+    const float* loc_w;
+    const float* dy;
+    const float* dyo;
+    float* ws_u;
+    float* ws_b;
+    float* f;
+    double* dump;
+    int* status;
+    MixedParams mp;
+    int N;
+    int b0_solve, nb_solve;    // workgroup range of k_mixed_solve_group
+    int b0_finish, nb_finish;  // workgroup range of k_mixed_finish_group
+};
+struct MixedGroup {
+    int n;
+    MixedTask t[MIXED_GROUP_MAX];
+};
+
 __device__ inline double wave_sum_d(double x) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
@@ -239,15 +261,17 @@ __device__ inline void gram_to_lds(int g, int lane, const int* __restrict__ loc_
 // way, so the eigenpairs still come out as column norms and directions.
 // ws_u[g][i][j] = U[i][j], ws_b[g][i] = b_i.
 // ---------------------------------------------------------------------------------
+// Workgroup bx of nbx (nbx a multiple of 8, or the whole grid) of one analysis.
 template <int N, int PASSES>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_mixed_solve(
-    const int* __restrict__ loc_ptr, const int* __restrict__ loc_site, const float* __restrict__ loc_w,
-    const float* __restrict__ dy, const float* __restrict__ dyo, float* __restrict__ ws_u,
-    float* __restrict__ ws_b, MixedParams mp) {
-    __shared__ __align__(16) float cfull[N * N];
+__device__ __forceinline__ void solve_body(int bx, int nbx, const int* __restrict__ loc_ptr,
+                                           const int* __restrict__ loc_site, const float* __restrict__ loc_w,
+                                           const float* __restrict__ dy, const float* __restrict__ dyo,
+                                           float* __restrict__ ws_u, float* __restrict__ ws_b,
+                                           const MixedParams& mp, float* cfull) {
+    // cfull: N*N floats of LDS, 16-byte aligned
     const int lane = threadIdx.x;
     const int l = lane & (N - 1);
-    const int g_begin = xcd_order(blockIdx.x, gridDim.x) * mp.chain;
+    const int g_begin = xcd_order(bx, nbx) * mp.chain;
     const int g_end = min(g_begin + mp.chain, mp.nodes);
     for (int g = g_begin; g < g_end; ++g) {
         __syncthreads();  // the previous gridpoint's reads of cfull are done
@@ -312,6 +336,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
             for (int i = 0; i < N; ++i) ug[i * N + lane] = u[i] * sc;
         }
     }
+}
+
+template <int N, int PASSES>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_mixed_solve(
+    const int* __restrict__ loc_ptr, const int* __restrict__ loc_site, const float* __restrict__ loc_w,
+    const float* __restrict__ dy, const float* __restrict__ dyo, float* __restrict__ ws_u,
+    float* __restrict__ ws_b, MixedParams mp) {
+    __shared__ __align__(16) float cfull[N * N];
+    solve_body<N, PASSES>(blockIdx.x, gridDim.x, loc_ptr, loc_site, loc_w, dy, dyo, ws_u, ws_b, mp, cfull);
 }
 
 // ---------------------------------------------------------------------------------
@@ -379,20 +412,34 @@ __device__ inline void renorm_split(double (&u)[H], double& sc, double& al, int 
     al = pair_sum((a4[0] + a4[1]) + (a4[2] + a4[3]), buf, w, lane);
 }
 
-template <int N, bool SWEEP64>
-__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3))) void k_mixed_finish(
-    const float* __restrict__ ws_u, const float* __restrict__ ws_b, float* __restrict__ f, MixedParams mp,
-    double* __restrict__ dump, int* __restrict__ status) {
-    constexpr int H = FinishLds<N>::H, TQ = FinishLds<N>::TQ, TS = FinishLds<N>::T_STRIDE;
-    __shared__ double xpart[3][2 * 64];  // pair-sum buffers: two alternating rounds + norms/h
-    __shared__ double bvec[N];
-    __shared__ double xbar[Q];
-    __shared__ double Tq[N * TS];        // a TQ-column chunk of V, row-major
-    __shared__ double Zp[2][Q * N];      // each wave's part of s_i (dX V)[q][i]
-    __shared__ float dXs[Q * N];
-    __shared__ double hv[N];
+template <int N>
+struct FinishShared {
+    static constexpr int TS = FinishLds<N>::T_STRIDE;
+    double xpart[3][2 * 64];  // pair-sum buffers: two alternating rounds + norms/h
+    double bvec[N];
+    double xbar[Q];
+    double Tq[N * TS];        // a TQ-column chunk of V, row-major
+    double Zp[2][Q * N];      // each wave's part of s_i (dX V)[q][i]
+    float dXs[Q * N];
+    double hv[N];
+};
 
-    const int g = xcd_order(blockIdx.x, gridDim.x);
+template <int N, bool SWEEP64>
+__device__ __forceinline__ void finish_body(int bx, int nbx, const float* __restrict__ ws_u,
+                                            const float* __restrict__ ws_b, float* __restrict__ f,
+                                            const MixedParams& mp, double* __restrict__ dump,
+                                            int* __restrict__ status, FinishShared<N>& sh) {
+    constexpr int H = FinishLds<N>::H, TQ = FinishLds<N>::TQ, TS = FinishLds<N>::T_STRIDE;
+    auto& xpart = sh.xpart;
+    auto& bvec = sh.bvec;
+    auto& xbar = sh.xbar;
+    auto& Tq = sh.Tq;
+    auto& Zp = sh.Zp;
+    auto& dXs = sh.dXs;
+    auto& hv = sh.hv;
+
+    const int g = xcd_order(bx, nbx);
+    if (g >= mp.nodes) return;  // padding workgroup of a group launch (uniform per workgroup)
     const int w = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     const int M = mp.M;
@@ -535,20 +582,53 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3))) void k
     }
 }
 
-}  // namespace
-
-int mixed_width(int M) {
-    int N = 16;
-    while (N < M) N *= 2;
-    return N;
+template <int N, bool SWEEP64>
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3))) void k_mixed_finish(
+    const float* __restrict__ ws_u, const float* __restrict__ ws_b, float* __restrict__ f, MixedParams mp,
+    double* __restrict__ dump, int* __restrict__ status) {
+    __shared__ FinishShared<N> sh;
+    finish_body<N, SWEEP64>(blockIdx.x, gridDim.x, ws_u, ws_b, f, mp, dump, status, sh);
 }
 
-hipError_t launch_letkf_mixed(Ctx& c, float* f, unsigned long long* sweeps32, unsigned long long* sweeps64,
-                              int passes, hipEvent_t ev_mid_end, hipEvent_t ev_mid_begin) {
-    const int N = mixed_width(c.M);
-    if (!c.d_loc_ptr || !c.d_dy || !c.d_dyo || !f || (c.dump_n > 0 && !c.d_dump)) return hipErrorInvalidValue;
-    if (!c.d_mix_u || !c.d_mix_b || c.mix_width != N) return hipErrorInvalidValue;
-    MixedParams mp;
+// ---------------------------------------------------------------------------------
+// group launches: the analyses of several contexts (different grids and ensemble
+// sizes) in one launch of each kernel.  Each task owns a range of workgroups,
+// padded to a multiple of 8 so that its XCD-contiguous order holds; a workgroup
+// finds its task by scanning the (at most MIXED_GROUP_MAX) range starts.
+// ---------------------------------------------------------------------------------
+template <int PASSES>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_mixed_solve_group(MixedGroup G) {
+    int k = 0;
+    while (k + 1 < G.n && int(blockIdx.x) >= G.t[k + 1].b0_solve) ++k;
+    const MixedTask& T = G.t[k];
+    const int bx = int(blockIdx.x) - T.b0_solve;
+    __shared__ __align__(16) float cfull[64 * 64];  // one buffer for every width
+    switch (T.N) {
+        case 16: solve_body<16, PASSES>(bx, T.nb_solve, T.loc_ptr, T.loc_site, T.loc_w, T.dy, T.dyo, T.ws_u, T.ws_b, T.mp, cfull); break;
+        case 32: solve_body<32, PASSES>(bx, T.nb_solve, T.loc_ptr, T.loc_site, T.loc_w, T.dy, T.dyo, T.ws_u, T.ws_b, T.mp, cfull); break;
+        default: solve_body<64, PASSES>(bx, T.nb_solve, T.loc_ptr, T.loc_site, T.loc_w, T.dy, T.dyo, T.ws_u, T.ws_b, T.mp, cfull); break;
+    }
+}
+
+template <bool SWEEP64>
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3))) void k_mixed_finish_group(MixedGroup G) {
+    int k = 0;
+    while (k + 1 < G.n && int(blockIdx.x) >= G.t[k + 1].b0_finish) ++k;
+    const MixedTask& T = G.t[k];
+    const int bx = int(blockIdx.x) - T.b0_finish;
+    // one buffer for every width (the layouts of smaller widths fit inside N = 64's)
+    __shared__ FinishShared<64> sh;
+    switch (T.N) {
+        case 16: finish_body<16, SWEEP64>(bx, T.nb_finish, T.ws_u, T.ws_b, T.f, T.mp, T.dump, T.status,
+                                          *reinterpret_cast<FinishShared<16>*>(&sh)); break;
+        case 32: finish_body<32, SWEEP64>(bx, T.nb_finish, T.ws_u, T.ws_b, T.f, T.mp, T.dump, T.status,
+                                          *reinterpret_cast<FinishShared<32>*>(&sh)); break;
+        default: finish_body<64, SWEEP64>(bx, T.nb_finish, T.ws_u, T.ws_b, T.f, T.mp, T.dump, T.status, sh); break;
+    }
+}
+
+MixedParams mixed_params(const Ctx& c, unsigned long long* sweeps32, unsigned long long* sweeps64) {
+    MixedParams mp{};
     mp.M = c.M;
     mp.nodes = int(c.nodes);
     mp.n_vars = c.obs.n_vars;
@@ -569,6 +649,23 @@ hipError_t launch_letkf_mixed(Ctx& c, float* f, unsigned long long* sweeps32, un
     mp.sweeps32 = sweeps32;
     mp.sweeps64 = sweeps64;
     mp.chain = c.mix_chain;
+    return mp;
+}
+
+}  // namespace
+
+int mixed_width(int M) {
+    int N = 16;
+    while (N < M) N *= 2;
+    return N;
+}
+
+hipError_t launch_letkf_mixed(Ctx& c, float* f, unsigned long long* sweeps32, unsigned long long* sweeps64,
+                              int passes, hipEvent_t ev_mid_end, hipEvent_t ev_mid_begin, bool solve_only) {
+    const int N = mixed_width(c.M);
+    if (!c.d_loc_ptr || !c.d_dy || !c.d_dyo || !f || (c.dump_n > 0 && !c.d_dump)) return hipErrorInvalidValue;
+    if (!c.d_mix_u || !c.d_mix_b || c.mix_width != N) return hipErrorInvalidValue;
+    MixedParams mp = mixed_params(c, sweeps32, sweeps64);
     const dim3 grid(unsigned(c.nodes)), block(64), block2(128);
     const dim3 grid_solve(unsigned((c.nodes + mp.chain - 1) / mp.chain));
     const bool sweep64 = mp.max64 > 0;
@@ -590,7 +687,7 @@ hipError_t launch_letkf_mixed(Ctx& c, float* f, unsigned long long* sweeps32, un
         default: return hipErrorInvalidValue;
     }
     hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
+    if (e != hipSuccess || solve_only) return e;
     for (hipEvent_t ev : {ev_mid_end, ev_mid_begin}) {
         if (!ev) continue;
         e = hipEventRecord(ev, c.stream);
@@ -604,6 +701,55 @@ hipError_t launch_letkf_mixed(Ctx& c, float* f, unsigned long long* sweeps32, un
     }
 #undef LAUNCH_SOLVE
 #undef LAUNCH_FINISH
+    return hipGetLastError();
+}
+
+hipError_t launch_letkf_mixed_group(Ctx* const* cs, int n, hipStream_t stream) {
+    if (n < 1 || n > MIXED_GROUP_MAX) return hipErrorInvalidValue;
+    MixedGroup G{};
+    G.n = n;
+    const int passes = cs[0]->cfg.gram_passes;
+    const bool sweep64 = cs[0]->cfg.jacobi_fp64_sweeps > 0;
+    long long bs = 0, bf = 0;
+    for (int k = 0; k < n; ++k) {
+        Ctx& c = *cs[k];
+        const int N = mixed_width(c.M);
+        if (!c.d_loc_ptr || !c.d_dy || !c.d_dyo || !c.d_mix_u || !c.d_mix_b || c.mix_width != N ||
+            (c.dump_n > 0 && !c.d_dump))
+            return hipErrorInvalidValue;
+        if (c.cfg.gram_passes != passes || (c.cfg.jacobi_fp64_sweeps > 0) != sweep64) return hipErrorInvalidValue;
+        MixedTask& T = G.t[k];
+        T.loc_ptr = c.d_loc_ptr;
+        T.loc_site = c.d_loc_site;
+        T.loc_w = c.d_loc_w;
+        T.dy = c.d_dy;
+        T.dyo = c.d_dyo;
+        T.ws_u = c.d_mix_u;
+        T.ws_b = c.d_mix_b;
+        T.f = c.f[c.cur];
+        T.dump = c.d_dump;
+        T.status = c.d_status;
+        T.mp = mixed_params(c, nullptr, nullptr);
+        T.N = N;
+        const long long chains = (c.nodes + T.mp.chain - 1) / T.mp.chain;
+        T.nb_solve = int((chains + 7) / 8 * 8);
+        T.nb_finish = int((c.nodes + 7) / 8 * 8);
+        T.b0_solve = int(bs);
+        T.b0_finish = int(bf);
+        bs += T.nb_solve;
+        bf += T.nb_finish;
+    }
+    if (bs > (1LL << 31) - 1 || bf > (1LL << 31) - 1) return hipErrorInvalidValue;
+    if (passes == 3)
+        k_mixed_solve_group<3><<<unsigned(bs), 64, 0, stream>>>(G);
+    else
+        k_mixed_solve_group<1><<<unsigned(bs), 64, 0, stream>>>(G);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (sweep64)
+        k_mixed_finish_group<true><<<unsigned(bf), 128, 0, stream>>>(G);
+    else
+        k_mixed_finish_group<false><<<unsigned(bf), 128, 0, stream>>>(G);
     return hipGetLastError();
 }
 

@@ -120,6 +120,16 @@ def _load():
         "lbmletkf_trainer_get_params": ([P, P], I),
         "lbmletkf_trainer_loss_grad": ([P, C.c_longlong, C.c_longlong, D, C.POINTER(D), P], I),
         "lbmletkf_trainer_train": ([P, I, I, D, D, P], I),
+        "lbmletkf_set_tuning": ([P, I, I], I),
+        "lbmletkf_get_tuning": ([P, C.POINTER(I), C.POINTER(I)], I),
+        "lbmletkf_autotune": ([P, C.c_char_p, C.c_char_p, I], I),
+        "lbmletkf_group_create": ([P, I, C.POINTER(P)], I),
+        "lbmletkf_group_destroy": ([P], I),
+        "lbmletkf_group_forecast": ([P, I], I),
+        "lbmletkf_group_analysis": ([P, P], I),
+        "lbmletkf_group_cycle": ([P, P], I),
+        "lbmletkf_group_synchronize": ([P], I),
+        "lbmletkf_group_get_stream": ([P], P),
         "lbmletkf_get_stream": ([P], P),
         "lbmletkf_synchronize": ([P], I),
     }
@@ -308,6 +318,21 @@ class Context:
         self._ck(lib.lbmletkf_profile_read(self.h, phase.encode(), C.byref(sec), C.byref(n), C.byref(w)))
         return sec.value, n.value, w.value
 
+    def set_tuning(self, lbm_tiles_per_block: int, mix_chain: int):
+        self._ck(lib.lbmletkf_set_tuning(self.h, lbm_tiles_per_block, mix_chain))
+
+    def get_tuning(self):
+        """(lbm_tiles_per_block, mix_chain)"""
+        a, b = C.c_int(), C.c_int()
+        self._ck(lib.lbmletkf_get_tuning(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def autotune(self, cache_path: str | None = None) -> str:
+        """Time the launch-shape candidates and keep the fastest; returns the report."""
+        buf = C.create_string_buffer(1024)
+        self._ck(lib.lbmletkf_autotune(self.h, cache_path.encode() if cache_path else None, buf, 1024))
+        return buf.value.decode()
+
     def set_obs_sites(self, nodes):
         """Explicit observation network: sites at the given node indices."""
         a = np.ascontiguousarray(nodes, dtype=np.int32).ravel()
@@ -446,6 +471,52 @@ class Trainer:
         hist = np.empty(max(steps, 1), np.float32)
         _check(lib.lbmletkf_trainer_train(self.h, batch, steps, lr, w_norm, _ptr(hist)))
         return hist[:steps]
+
+
+class Group:
+    """Several contexts stepped and analysed by shared launches (see the C header)."""
+
+    def __init__(self, contexts):
+        self.contexts = list(contexts)
+        arr = (C.c_void_p * len(self.contexts))(*[c.h for c in self.contexts])
+        h = C.c_void_p()
+        _check(lib.lbmletkf_group_create(arr, len(self.contexts), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib.lbmletkf_group_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _yo(self, yo_devs):
+        if len(yo_devs) != len(self.contexts):
+            raise ValueError("one device observation vector per context")
+        return (C.c_void_p * len(yo_devs))(*[int(p) for p in yo_devs])
+
+    def forecast(self, n_steps: int = 1):
+        _check(lib.lbmletkf_group_forecast(self.h, n_steps))
+
+    def analysis_device(self, yo_devs):
+        _check(lib.lbmletkf_group_analysis(self.h, self._yo(yo_devs)))
+
+    def cycle_device(self, yo_devs):
+        _check(lib.lbmletkf_group_cycle(self.h, self._yo(yo_devs)))
+
+    def synchronize(self):
+        _check(lib.lbmletkf_group_synchronize(self.h))
+
+    @property
+    def stream(self) -> int:
+        return lib.lbmletkf_group_get_stream(self.h) or 0
 
 
 def update(ctx: Context, n_steps: int = 1):
