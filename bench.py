@@ -112,6 +112,10 @@ def bench_variant(lb, hip, cfg, variant, M, K, W, P, obs, truth, n_obs, log):
         return obs + 4 * n_obs * c
 
     ens.forecast(steps)  # to the time of the first observation
+    tuning = None
+    if variant == lb.VARIANT_MIXED:
+        log("[mixed] autotune: " + ens.autotune())
+        tuning = list(ens.get_tuning())
     for c in range(W):
         ens.cycle_device(yo(c))
     ens.synchronize()
@@ -145,7 +149,7 @@ def bench_variant(lb, hip, cfg, variant, M, K, W, P, obs, truth, n_obs, log):
     for ph, (sec, n, w) in phases.items():
         log(f"    {ph:10s} {sec * 1e3 / P:9.3f} ms/cycle  launches {n}  work {w}   (phase-timed pass, {P} cycles)")
     return dict(wall=wall, event_s=ev_ms / 1e3, phases=phases, stats=stats, steps=steps, state=final_state,
-                yo_index=ca, prof_cycles=P, variant=name)
+                yo_index=ca, prof_cycles=P, variant=name, tuning=tuning)
 
 
 def kernel_table(r, M, nodes):
@@ -229,6 +233,107 @@ def cpu_baseline(cfg, f_state, yo, log):
                 lbm_step_s=t_step, analysis_s=t_ana)
 
 
+def bench_closure_step(lb, hip, M, log):
+    """One LBM step of all members with the learned closure (fused MFMA network in
+    the collide kernel) against the Smagorinsky step, same workload."""
+    out = {}
+    for name, kind in (("smagorinsky", lb.CLOSURE_SMAGORINSKY), ("mlp", lb.CLOSURE_MLP)):
+        cfg = lb.preset("baseline")
+        cfg.closure_kind = kind
+        ens = lb.Context(cfg, M)
+        for m in range(M):
+            ens.init_member(m, 1000 + m)
+        if kind == lb.CLOSURE_MLP:
+            ens.set_closure(lb.closure_init_params(1))
+        ens.forecast(5)
+        ens.synchronize()
+        e0, e1 = hip.event(), hip.event()
+        n = 50
+        hip.record(e0, ens.stream)
+        ens.forecast(n)
+        hip.record(e1, ens.stream)
+        ens.synchronize()
+        us = hip.elapsed_ms(e0, e1) * 1e3 / n
+        ens.close()
+        nodes = cfg.nx * cfg.ny * M
+        out[name] = {"us_per_step": us, "node_updates_per_s": nodes / us * 1e6,
+                     "hbm_GBs_at_72B": LBM_BYTES_PER_NODE_UPDATE * nodes / us * 1e-3}
+        log(f"[closure] {name}: {us:.1f} us per step of {M} members")
+    out["mlp_over_smagorinsky_time"] = out["mlp"]["us_per_step"] / out["smagorinsky"]["us_per_step"]
+    return out
+
+
+UNEVEN_GRIDS = [(128, 8), (256, 16), (384, 32), (512, 64)]
+
+
+def bench_uneven(lb, hip, K, W, log, tune_cache=None):
+    """The uneven batch: four OSSEs (128^2..512^2, M = 8..64) with observation sites
+    at a seeded random density of 0.5-4% per region (4x4 regions), stepped and
+    analysed by shared launches (lbmletkf_group_cycle, one hipGraph per cycle);
+    each context's launch shapes come from the autotuner first."""
+    ctxs, nats, obs, n_obs, infos = [], [], [], [], []
+    for k, (n, M) in enumerate(UNEVEN_GRIDS):
+        cfg = lb.preset("baseline")
+        cfg.nx = cfg.ny = n
+        sites = lb.uneven_sites(n, n, seed=100 + k, regions=4, density=(0.005, 0.04))
+        ens = lb.Context(cfg, M)
+        for m in range(M):
+            ens.init_member(m, 1000 + m)
+        nat = lb.Context(cfg, 1)
+        nat.init_member(0, k)
+        ens.set_obs_sites(sites)
+        nat.set_obs_sites(sites)
+        no = nat.obs_count()[1]
+        buf = hip.malloc(4 * no * (W + K + 1))
+        for c in range(W + K + 1):
+            nat.forecast(cfg.da_interval)
+            nat.observe_device(0, 77 + k, c, buf + 4 * no * c, noise=True)
+        ens.forecast(cfg.da_interval)
+        rep = ens.autotune(tune_cache)
+        counts = [len(ens.local_obs(g)[0]) for g in range(0, n * n, max(1, n * n // 1500))]
+        infos.append(dict(grid=f"{n}x{n}", members=M, sites=int(sites.size), local_sites_min=min(counts),
+                          local_sites_max=max(counts), tuning=list(ens.get_tuning())))
+        log(f"[uneven] {n}^2 M={M}: {sites.size} sites, local sites {min(counts)}..{max(counts)}; {rep}")
+        ctxs.append(ens)
+        nats.append(nat)
+        obs.append(buf)
+        n_obs.append(no)
+    grp = lb.Group(ctxs)
+
+    def yos(c):
+        return [obs[k] + 4 * n_obs[k] * c for k in range(len(ctxs))]
+
+    for c in range(W):
+        grp.cycle_device(yos(c))
+    grp.synchronize()
+    e0, e1 = hip.event(), hip.event()
+    hip.sync()
+    t0 = time.perf_counter()
+    hip.record(e0, grp.stream)
+    for c in range(W, W + K):
+        grp.cycle_device(yos(c))
+    hip.record(e1, grp.stream)
+    grp.synchronize()
+    hip.sync()
+    wall = time.perf_counter() - t0
+    grp.analysis_device(yos(W + K))
+    grp.synchronize()
+    for info, ens, nat in zip(infos, ctxs, nats):
+        r, u, v = nat.get_macro(0)
+        info["analysis_rmse"] = ens.ensemble_stats(r, u, v)[0]
+    grp.close()
+    for ens, nat, buf in zip(ctxs, nats, obs):
+        ens.close()
+        nat.close()
+        hip.free(buf)
+    node_updates = sum(n * n * M for n, M in UNEVEN_GRIDS) * 100
+    log(f"[uneven] {K} group cycles in {wall:.3f} s: {K / wall:.2f} cycles/s ({hip.elapsed_ms(e0, e1) / K:.1f} ms/cycle)")
+    return {"workload": "4 OSSEs in shared launches: " + ", ".join(f"{n}^2 M={M}" for n, M in UNEVEN_GRIDS) +
+                        "; sites at 0.5-4% random density per region; 100 LBM steps + 1 analysis per cycle",
+            "cycles_per_s": K / wall, "ms_per_cycle": wall / K * 1e3, "steps": K, "warmup": W,
+            "lbm_node_updates_per_cycle": node_updates, "grids": infos}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -238,6 +343,8 @@ def main():
     ap.add_argument("--variants", default="plain,mixed")
     ap.add_argument("--prof-cycles", type=int, default=3, help="extra phase-timed cycles (eager launches)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-batch group measurement")
+    ap.add_argument("--uneven-steps", type=int, default=3, help="timed group cycles of the uneven batch")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (HIP's blit kernel): the "
                          "calibration of the FETCH_SIZE / WRITE_SIZE counters, then exit")
@@ -320,6 +427,7 @@ def main():
                    "n_obs": n_obs, "variant_timed": "mixed" if "mixed" in res else list(res)[0]},
         "variants": {k: {"cycles_per_s": K / r["wall"], "ms_per_cycle": r["wall"] / K * 1e3,
                          "analysis_rmse": r["stats"][0], "analysis_spread": r["stats"][1],
+                         "tuning_lbm_tiles_per_block_and_chain": r["tuning"],
                          "phase_ms_per_cycle": {p: r["phases"][p][0] / P * 1e3 for p in r["phases"]
                                                 if p != "solve_fp64"}}
                      for k, r in res.items()},
@@ -333,6 +441,9 @@ def main():
         out["speedup_mixed_over_plain"] = res["plain"]["wall"] / res["mixed"]["wall"]
         a, b = res["mixed"]["stats"][0], res["plain"]["stats"][0]
         out["rmse_rel_diff_mixed_vs_plain"] = abs(a - b) / b
+    if not args.no_uneven:
+        out["uneven_batch"] = bench_uneven(lb, hip, args.uneven_steps, 1, log)
+    out["closure_step"] = bench_closure_step(lb, hip, M, log)
     if not args.no_cpu:
         yo_last = np.empty(n_obs, np.float32)
         hip.d2h(yo_last, obs + 4 * n_obs * head["yo_index"])

@@ -100,6 +100,9 @@ typedef struct lbmletkf_config {
     double closure_strain_scale; /* MLP inputs are strain components / this scale            */
     double closure_nu_scale;     /* MLP output nu_t = this scale * softplus(z)               */
     double closure_nu_max;       /* nu_t above this (or not finite) falls back to Smagorinsky */
+    /* MIXED: the FP32 phase stops after the sweep whose largest pair cosine is at
+     * most this (0: 3e-4) */
+    double jacobi_tol32;
 } lbmletkf_config;
 
 typedef struct lbmletkf_ctx lbmletkf_ctx;
@@ -306,9 +309,10 @@ int lbmletkf_trainer_train(lbmletkf_trainer* tr, int batch, int steps, double lr
                            float* loss_history);
 
 /* ---- tuning ----------------------------------------------------------------------
- * Launch shapes of the MIXED variant: row-band height of the fused LBM kernel
- * (lbm_tiles_per_block tiles of 64 x 16 per workgroup) and the chain length of
- * the FP32 solve kernel (consecutive gridpoints per wavefront, each warm-started
+ * Launch shapes of the MIXED variant: the shape of the fused LBM step --
+ * lbm_tiles_per_block = k > 0: k tiles of 64 x 16 per workgroup; = -h: full-row
+ * strips of h rows (h = 8, 16, 32; nx a multiple of 64 up to 1024, one thread per
+ * column) -- and the chain length of the FP32 solve kernel (consecutive gridpoints per wavefront, each warm-started
  * from the previous one's eigenvectors; the result changes by FP32 rounding).
  * No counterpart in the reference. */
 int lbmletkf_set_tuning(lbmletkf_ctx* ctx, int lbm_tiles_per_block, int mix_chain);

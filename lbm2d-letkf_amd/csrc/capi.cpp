@@ -225,6 +225,7 @@ int check_config(const lbmletkf_config* cfg, int n_members, std::string& why) {
     if (!(cfg->jacobi_tol > 0) || cfg->jacobi_max_sweeps < 1) return bad("jacobi_tol > 0 and jacobi_max_sweeps >= 1 required");
     if (cfg->gram_passes != 1 && cfg->gram_passes != 3) return bad("gram_passes must be 1 or 3");
     if (cfg->jacobi_fp64_sweeps < 0 || cfg->jacobi_fp64_sweeps > 20) return bad("jacobi_fp64_sweeps must be in [0, 20]");
+    if (!(cfg->jacobi_tol32 >= 0 && cfg->jacobi_tol32 < 1)) return bad("jacobi_tol32 must be in [0, 1)");
     if (cfg->closure_kind != LBMLETKF_CLOSURE_SMAGORINSKY && cfg->closure_kind != LBMLETKF_CLOSURE_MLP)
         return bad("unknown closure_kind");
     if (cfg->closure_kind == LBMLETKF_CLOSURE_MLP &&
@@ -1436,10 +1437,13 @@ void* lbmletkf_group_get_stream(lbmletkf_group* grp) {
 int lbmletkf_set_tuning(lbmletkf_ctx* ctx, int lbm_tiles_per_block, int mix_chain) {
     Ctx* c = reinterpret_cast<Ctx*>(ctx);
     CHECK_CTX(c);
-    if (lbm_tiles_per_block < 1 || lbm_tiles_per_block > 16 || mix_chain < 1 || mix_chain > 64)
-        return fail(c, LBMLETKF_ERR_ARG, "tiles per block must be in [1, 16] and the chain in [1, 64]");
-    if (lbm_fused_supported(c->nx, c->ny) && ((long long)(c->nx / 64) * (c->ny / 16) * c->M) % lbm_tiles_per_block)
+    if (lbm_tiles_per_block == 0 || lbm_tiles_per_block > 16 || mix_chain < 1 || mix_chain > 64)
+        return fail(c, LBMLETKF_ERR_ARG, "the LBM shape must be 1..16 tiles or -8/-16/-32 rows, the chain in [1, 64]");
+    if (lbm_tiles_per_block > 0 && lbm_fused_supported(c->nx, c->ny) &&
+        ((long long)(c->nx / 64) * (c->ny / 16) * c->M) % lbm_tiles_per_block)
         return fail(c, LBMLETKF_ERR_ARG, "tiles per block must divide the number of tiles");
+    if (lbm_tiles_per_block < 0 && !lbm_rows_supported(c->nx, c->ny, -lbm_tiles_per_block))
+        return fail(c, LBMLETKF_ERR_ARG, "row strips need nx a multiple of 64 up to 1024 and ny a multiple of the height");
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     c->drop_graphs();
     c->lbm_tiles_per_block = lbm_tiles_per_block;
@@ -1547,8 +1551,9 @@ int lbmletkf_autotune(lbmletkf_ctx* ctx, const char* cache_path, char* report, i
         const long long tiles = (long long)(c->nx / 64) * (c->ny / 16) * c->M;
         double best = 1e300;
         rep += "lbm us/step:";
-        for (int tpb : {1, 2, 4}) {
-            if (tiles % tpb) continue;
+        for (int tpb : {1, 2, 4, -8, -16}) {
+            if (tpb > 0 && tiles % tpb) continue;
+            if (tpb < 0 && !lbm_rows_supported(c->nx, c->ny, -tpb)) continue;
             c->lbm_tiles_per_block = tpb;
             double us = 0;
             if (int st = time_launches(c, 2, 10, [&] { return launch_lbm_fused(*c, c->f[c->cur], c->f[1 - c->cur]); }, us))

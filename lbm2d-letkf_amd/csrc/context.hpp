@@ -112,7 +112,9 @@ struct Ctx {
     float* d_mix_b = nullptr;     // [nodes][N] b = dY^T R^-1 dyo
     int mix_width = 0;
     int mix_chain = 8;            // consecutive gridpoints per wavefront in k_mixed_solve
-    int lbm_tiles_per_block = 1;  // 64x16 tiles per workgroup in k_lbm_fused
+    // shape of the fused LBM step: k > 0 tiles of 64x16 per workgroup (k_lbm_fused),
+    // -h full-row strips of h rows (k_lbm_rows)
+    int lbm_tiles_per_block = 1;
 
     // transform dump
     int dump_g0 = 0, dump_n = 0;
@@ -176,6 +178,8 @@ hipError_t launch_ensemble_stats(Ctx& c, const float* truth_rho, const float* tr
 int ensemble_stats_blocks(const Ctx& c);
 hipError_t launch_step_advance(Ctx& c, int n);
 bool lbm_fused_supported(int nx, int ny);
+// full-row strips of h rows (the alternative shape of the fused step)
+bool lbm_rows_supported(int nx, int ny, int h);
 
 // ---- launchers (letkf.hip) ----------------------------------------------------
 hipError_t launch_local_count(Ctx& c, int* d_count);

@@ -122,6 +122,72 @@ __global__ __launch_bounds__(NT) void k_lbm_fused(const float* __restrict__ fin,
         lbm_tile<FORCE>(bb * tiles_per_block + k, fin, fout, force, p, tiles_x, tiles_per_member);
 }
 
+// Full-row strips: a workgroup of nx threads (one per column, nx a multiple of
+// 64 up to 1024) advances rows y0 .. y0+H-1 of one member, one row at a time.
+// The streamed velocities of the last rows sit in a 4-row LDS ring, so a row's
+// Smagorinsky stencil needs no halo in x (the row is whole and periodic) and only
+// the two rows y0-1 and y0+H are streamed twice (by neighbouring strips).  The
+// populations of row k+2 are requested before the barrier of row k.
+template <bool FORCE, int H>
+__global__ __launch_bounds__(1024) void k_lbm_rows(const float* __restrict__ fin, float* __restrict__ fout,
+                                                   const float* __restrict__ force, LbmParams p,
+                                                   int strips_per_member) {
+    extern __shared__ float ring[];  // [2][4][nx]: u, v of the rows k-1 .. k+2
+    const int nb = gridDim.x, b = blockIdx.x;
+    const int bb = (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
+    const int nx = p.nx, ny = p.ny;
+    const long long nodes = p.nodes;
+    const int member = bb / strips_per_member;
+    const int y0 = (bb - member * strips_per_member) * H;
+    const float* f = fin + (long long)member * p.member_stride;
+    float* fo = fout + (long long)member * p.member_stride;
+    float* ru = ring;
+    float* rv = ring + 4 * nx;
+    const int i = threadIdx.x;
+    const int im = i == 0 ? nx - 1 : i - 1, ip = i == nx - 1 ? 0 : i + 1;
+    auto pull = [&](int jrel, float fc[Q]) {
+        const int j = wrap(y0 + jrel, ny);
+        pull9(f, nodes, im, i, ip, (long long)wrap(j - 1, ny) * nx, (long long)j * nx,
+              (long long)wrap(j + 1, ny) * nx, fc);
+    };
+    auto vel = [&](int jrel, const float fc[Q], float& r, float& u, float& v) {
+        moments(fc, p.c, r, u, v);
+        ru[((jrel + 4) & 3) * nx + i] = u;
+        rv[((jrel + 4) & 3) * nx + i] = v;
+    };
+    float fa[Q], fb[Q], fn[Q], ra, ua, va, rb, ub, vb, rx, ux, vx;
+    pull(-1, fn);
+    vel(-1, fn, rx, ux, vx);
+    pull(0, fa);
+    vel(0, fa, ra, ua, va);
+    pull(1, fb);
+    for (int k = 0; k < H; ++k) {
+        if (k + 2 <= H) pull(k + 2, fn);
+        vel(k + 1, fb, rb, ub, vb);
+        __syncthreads();  // rows k-1 .. k+1 are in the ring; row k+2 overwrites the slot of k-2
+        const int s = ((k + 4) & 3) * nx, sn = ((k + 5) & 3) * nx, ss = ((k + 3) & 3) * nx;
+        const float nut = smagorinsky(ru[s + ip], ru[s + im], ru[sn + i], ru[ss + i], rv[s + ip], rv[s + im],
+                                      rv[sn + i], rv[ss + i], p);
+        const long long node = (long long)(y0 + k) * nx + i;
+        float Fx = 0.f, Fy = 0.f;
+        if (FORCE && p.has_force) {
+            Fx = force[node];
+            Fy = force[nodes + node];
+        }
+        collide9<FORCE>(fa, ra, ua, va, nut, Fx, Fy, p);
+#pragma unroll
+        for (int q = 0; q < Q; ++q) fo[q * nodes + node] = fa[q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            fa[q] = fb[q];
+            fb[q] = fn[q];
+        }
+        ra = rb;
+        ua = ub;
+        va = vb;
+    }
+}
+
 // One step of several contexts: each task owns a range of workgroups (one tile
 // each, the range padded to a multiple of 8 for the XCD-contiguous order).
 struct LbmTask {
@@ -412,14 +478,36 @@ hipError_t launch_step_advance(Ctx& c, int n) {
     return hipGetLastError();
 }
 
+bool lbm_rows_supported(int nx, int ny, int h) {
+    return nx % 64 == 0 && nx <= 1024 && (h == 8 || h == 16 || h == 32) && ny % h == 0 && ny >= 2;
+}
+
 hipError_t launch_lbm_fused(Ctx& c, const float* fin, float* fout) {
+    const bool force = c.lp.has_force || c.lp.fric_over_dt != 0.f;
+    if (c.lbm_tiles_per_block < 0) {
+        // full-row strips of -lbm_tiles_per_block rows
+        const int h = -c.lbm_tiles_per_block;
+        if (!lbm_rows_supported(c.nx, c.ny, h)) return hipErrorInvalidValue;
+        const int strips = c.ny / h;
+        const unsigned nb = unsigned(strips * c.M);
+        const size_t lds = sizeof(float) * 8 * size_t(c.nx);
+#define LBM_ROWS(HH)                                                                                       \
+    (force ? k_lbm_rows<true, HH><<<nb, c.nx, lds, c.stream>>>(fin, fout, c.force_xy, c.lp, strips)      \
+           : k_lbm_rows<false, HH><<<nb, c.nx, lds, c.stream>>>(fin, fout, c.force_xy, c.lp, strips))
+        switch (h) {
+            case 8: LBM_ROWS(8); break;
+            case 16: LBM_ROWS(16); break;
+            default: LBM_ROWS(32); break;
+        }
+#undef LBM_ROWS
+        return hipGetLastError();
+    }
     const int tiles_x = c.nx / TX, tiles_y = c.ny / TY;
     const int tiles = tiles_x * tiles_y;
     const int total = tiles * c.M;
     const int per_block = c.lbm_tiles_per_block;
     if (per_block < 1 || total % per_block != 0) return hipErrorInvalidValue;
     const int nb = total / per_block;
-    const bool force = c.lp.has_force || c.lp.fric_over_dt != 0.f;
     if (force)
         k_lbm_fused<true><<<nb, NT, 0, c.stream>>>(fin, fout, c.force_xy, c.lp, tiles_x, tiles, per_block);
     else

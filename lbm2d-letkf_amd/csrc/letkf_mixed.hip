@@ -637,10 +637,8 @@ MixedParams mixed_params(const Ctx& c, unsigned long long* sweeps32, unsigned lo
     for (int a = 0; a < 3; ++a) mp.coef[a] = c.obs.inv_sigma2[a];
     mp.sqm1 = sqrt(double(c.M - 1));
     // Cyclic Jacobi converges quadratically near the end: a sweep that starts at a
-    // largest pair cosine of 3e-4 leaves about 1e-7, the FP32 rounding floor.  With
-    // an FP64 sweep to follow, the FP32 phase can stop one sweep earlier (3e-3
-    // leaves ~1e-5, which the FP64 sweep squares away); the W error is unchanged.
-    mp.tol32 = c.cfg.jacobi_fp64_sweeps > 0 ? 3e-3f : 3e-4f;
+    // largest pair cosine of 3e-4 leaves about 1e-7, the FP32 rounding floor.
+    mp.tol32 = c.cfg.jacobi_tol32 > 0 ? float(c.cfg.jacobi_tol32) : 3e-4f;
     mp.tol64 = c.cfg.jacobi_tol;
     mp.max32 = c.cfg.jacobi_max_sweeps;
     mp.max64 = c.cfg.jacobi_fp64_sweeps;

@@ -52,6 +52,7 @@ class Config(C.Structure):
         ("gram_passes", C.c_int), ("jacobi_fp64_sweeps", C.c_int),
         ("closure_kind", C.c_int), ("closure_strain_scale", C.c_double), ("closure_nu_scale", C.c_double),
         ("closure_nu_max", C.c_double),
+        ("jacobi_tol32", C.c_double),
     ]
 
     def copy(self) -> "Config":
@@ -411,6 +412,7 @@ class Trainer:
     """Closure trainer: samples, parameters and Adam state on the GPU."""
 
     def __init__(self, coarse_cfg: Config, capacity: int):
+        self.h = None
         h = C.c_void_p()
         _check(lib.lbmletkf_trainer_create(C.byref(coarse_cfg), capacity, C.byref(h)))
         self.h = h
@@ -477,6 +479,7 @@ class Group:
     """Several contexts stepped and analysed by shared launches (see the C header)."""
 
     def __init__(self, contexts):
+        self.h = None
         self.contexts = list(contexts)
         arr = (C.c_void_p * len(self.contexts))(*[c.h for c in self.contexts])
         h = C.c_void_p()

@@ -39,6 +39,7 @@ typedef struct orc_config {
     int    gram_passes, jacobi_fp64_sweeps;  /* GPU precision knobs; unused by the oracle */
     int    closure_kind;
     double closure_strain_scale, closure_nu_scale, closure_nu_max;
+    double jacobi_tol32;
 } orc_config;
 
 int  orc_sizeof_config(void);

@@ -129,3 +129,25 @@ def test_full_size_batched_step(lb, orc):
     for s in range(3):
         f0 = orc.lbm_step(cfg, f0, s)
     assert _rel(f3, f0) < TOL_F
+
+
+@pytest.mark.parametrize("shape", [1, 2, -8, -16, -32])
+def test_launch_shapes_are_bit_identical(lb, orc, shape):
+    """Every shape of the fused step (tile bands and full-row strips) computes the
+    same per-node operations: bit-identical states, and the oracle's to TOL_F."""
+    cfg = small_cfg(lb, "ref", n=128)
+    cfg.variant = lb.VARIANT_MIXED
+    M = 3
+    print(f"case: launch shape {shape} 128x128 M={M}", flush=True)
+    out = []
+    for sh in (1, shape):
+        with _ensemble(lb, cfg, M) as ctx:
+            ctx.set_tuning(sh, 8)
+            f0 = ctx.get_all_states().astype(np.float64)
+            ctx.forecast(6)
+            out.append(ctx.get_all_states())
+    np.testing.assert_array_equal(out[1], out[0])
+    f = f0
+    for s in range(6):
+        f = orc.lbm_step(cfg, f, s)
+    assert _rel(out[1], f) < TOL_F

@@ -22,7 +22,7 @@ def test_autotune_keeps_state_and_caches(lb, orc, tmp_path):
         print("  " + rep, flush=True)
         np.testing.assert_array_equal(ens.get_all_states(), f0)
         tpb, chain = ens.get_tuning()
-        assert tpb in (1, 2, 4) and chain in (1, 2, 4, 8, 16)
+        assert tpb in (1, 2, 4, -8, -16) and chain in (1, 2, 4, 8, 16)
         assert "lbm us/step" in rep and "solve us" in rep
         # the second call is a cache hit with the same choice
         ens.set_tuning(1, 1)
@@ -54,3 +54,7 @@ def test_set_tuning_rejects(lb):
             ctx.set_tuning(8, 8)  # 12 tiles are not a multiple of 8
         ctx.set_tuning(4, 2)
         assert ctx.get_tuning() == (4, 2)
+        with pytest.raises(lb.LbmLetkfError):
+            ctx.set_tuning(-12, 2)  # strip heights are 8, 16 or 32
+        ctx.set_tuning(-16, 2)
+        assert ctx.get_tuning() == (-16, 2)
