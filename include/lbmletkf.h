@@ -94,14 +94,14 @@ typedef struct lbmletkf_config {
     double jacobi_tol;          /* relative off-diagonal tolerance of the eigensolver       */
     int    jacobi_max_sweeps;   /* cusolver syevj uses tol 1e-6, 100 sweeps (eigensolver_wrapper.hpp:192-193) */
     int    gram_passes;         /* MIXED: 1 = bf16 operands, 3 = split bf16 (hi*hi + hi*lo + lo*hi) */
-    int    jacobi_fp64_sweeps;  /* MIXED: at most this many FP64 sweeps after the FP32 phase */
+    int    jacobi_fp64_sweeps;  /* MIXED: at most this many FP64 sweeps after the FP32 phase (presets: 0) */
     /* eddy-viscosity closure of the fused LBM kernel (MIXED variant) */
     int    closure_kind;         /* LBMLETKF_CLOSURE_*                                       */
     double closure_strain_scale; /* MLP inputs are strain components / this scale            */
     double closure_nu_scale;     /* MLP output nu_t = this scale * softplus(z)               */
     double closure_nu_max;       /* nu_t above this (or not finite) falls back to Smagorinsky */
     /* MIXED: the FP32 phase stops after the sweep whose largest pair cosine is at
-     * most this (0: 3e-4) */
+     * most this (0: 3e-6), or once that cosine stops falling quadratically below 1e-4 */
     double jacobi_tol32;
 } lbmletkf_config;
 

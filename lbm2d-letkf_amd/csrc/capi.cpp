@@ -595,7 +595,7 @@ int lbmletkf_config_preset(const char* name, lbmletkf_config* cfg) {
         c.jacobi_tol = 1e-10;
         c.jacobi_max_sweeps = 100;
         c.gram_passes = 3;
-        c.jacobi_fp64_sweeps = 1;
+        c.jacobi_fp64_sweeps = 0;  // the FP32 phase reaches the FP32 floor; FP64 sweeps optional
         c.closure_kind = LBMLETKF_CLOSURE_SMAGORINSKY;
         c.closure_strain_scale = 1e-2 * c.c / c.dx;  // ~ u_rms k0 2pi/n at 256^2, k0 = 8
         c.closure_nu_scale = 1e-4 * c.c * c.dx;      // Smagorinsky nu_t at that strain, Cs dx = 0.1 dx
@@ -637,7 +637,7 @@ int lbmletkf_config_preset(const char* name, lbmletkf_config* cfg) {
         c.jacobi_tol = 1e-10;
         c.jacobi_max_sweeps = 100;
         c.gram_passes = 3;
-        c.jacobi_fp64_sweeps = 1;
+        c.jacobi_fp64_sweeps = 0;  // the FP32 phase reaches the FP32 floor; FP64 sweeps optional
         c.closure_kind = LBMLETKF_CLOSURE_SMAGORINSKY;
         c.closure_strain_scale = 1e-2 * c.c / c.dx;  // ~ u_rms k0 2pi/n at 256^2, k0 = 8
         c.closure_nu_scale = 1e-4 * c.c * c.dx;      // Smagorinsky nu_t at that strain, Cs dx = 0.1 dx

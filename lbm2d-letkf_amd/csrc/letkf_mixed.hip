@@ -319,7 +319,9 @@ __device__ __forceinline__ void solve_body(int bx, int nbx, const int* __restric
 
         // ---- 3. FP32 one-sided Jacobi ------------------------------------------------
         // lanes >= N (N < 64) hold zero columns: their pair Gram is zero, they never rotate
-        float sc = 1.f, al = 0.f;
+        // Stop at the tolerance, or when the largest cosine has stopped falling
+        // quadratically below 1e-4: that is the FP32 rounding floor of this matrix.
+        float sc = 1.f, al = 0.f, mc_prev = 1.f;
         int sw32 = 0;
         while (sw32 < mp.max32) {
             hestenes_renorm<N, float>(u, sc, al);
@@ -327,7 +329,8 @@ __device__ __forceinline__ void solve_body(int bx, int nbx, const int* __restric
             for (int m = 1; m < N; ++m) mc = fmaxf(mc, hestenes_round<N, float>(u, sc, al, m, lane));
             mc = wave_max(mc);
             ++sw32;
-            if (mc <= mp.tol32) break;
+            if (mc <= mp.tol32 || (mc <= 1e-4f && mc > 0.1f * mc_prev)) break;
+            mc_prev = mc;
         }
         if (lane == 0 && mp.sweeps32) atomicAdd(mp.sweeps32, (unsigned long long)sw32);
         if (lane < N) {
@@ -636,9 +639,11 @@ MixedParams mixed_params(const Ctx& c, unsigned long long* sweeps32, unsigned lo
     mp.diag = float(double(c.M - 1) / c.cfg.inflation);
     for (int a = 0; a < 3; ++a) mp.coef[a] = c.obs.inv_sigma2[a];
     mp.sqm1 = sqrt(double(c.M - 1));
-    // Cyclic Jacobi converges quadratically near the end: a sweep that starts at a
-    // largest pair cosine of 3e-4 leaves about 1e-7, the FP32 rounding floor.
-    mp.tol32 = c.cfg.jacobi_tol32 > 0 ? float(c.cfg.jacobi_tol32) : 3e-4f;
+    // The FP32 phase runs to the FP32 floor (largest pair cosine ~1e-6): the chained
+    // warm start takes the normalised columns of the previous gridpoint's U as its
+    // basis, and any non-orthogonality left there becomes an error of the next
+    // gridpoint's eigenpairs that no later sweep removes.
+    mp.tol32 = c.cfg.jacobi_tol32 > 0 ? float(c.cfg.jacobi_tol32) : 3e-6f;
     mp.tol64 = c.cfg.jacobi_tol;
     mp.max32 = c.cfg.jacobi_max_sweeps;
     mp.max64 = c.cfg.jacobi_fp64_sweeps;

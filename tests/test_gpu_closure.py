@@ -234,13 +234,14 @@ def test_trained_closure_gain_over_smagorinsky(lb):
     coarse = lb.preset("baseline")  # 256^2, lattice units, Cs = 0.1
     ct = _trainer_module()
     n = coarse.nx * coarse.ny
-    print("case: closure training on 1024^2 -> 256^2 snapshots", flush=True)
-    p, hist = ct.train_closure(coarse, factor=4, n_snap=12, every=25, spinup=150, seed=7, batch=4096,
-                               steps=1500, lr=2e-3)
+    print("case: closure training on 20 spectra 1024^2 -> 256^2", flush=True)
+    # 20 seeded spectra at 1024^2 (Cs = 0), 3 snapshots each, box filter x4
+    p, hist = ct.train_closure(coarse, factor=4, seeds=tuple(range(1, 21)), n_snap=3, every=25, spinup=150,
+                               batch=4096, steps=1500, lr=2e-3)
     print(f"  training loss {hist[:50].mean():.4f} -> {hist[-50:].mean():.4f}", flush=True)
     # a priori, on a held-out fine run
     with lb.Trainer(coarse, 2 * n) as ev:
-        ct.collect(ev, coarse, factor=4, n_snap=2, every=25, spinup=300, seed=99)
+        ct.collect(ev, coarse, factor=4, seeds=(99,), n_snap=2, every=25, spinup=300)
         smp = ev.get_samples()
         w = 1.0 / float(np.mean(smp[:, 20].astype(np.float64) ** 2 + smp[:, 21].astype(np.float64) ** 2))
         ev.set_params(p)

@@ -19,7 +19,9 @@ TOL_INC = 1e-4
 # (W and eigenvalue relative error, analysis-increment relative error, w relative error)
 # plain: FP32 Gram + FP64 Jacobi.  mixed3: split-bf16 MFMA Gram + FP32/FP64 Jacobi.
 # mixed1: single bf16 operands (8-bit mantissa) in the Gram matrix.
-TOLS = {"plain": (TOL_W, TOL_INC, 1e-4), "mixed3": (1e-3, 1e-3, 1e-3), "mixed1": (3e-2, 3e-2, 3e-2)}
+# mixed3f64: the same with one FP64 sweep after the FP32 phase.
+TOLS = {"plain": (TOL_W, TOL_INC, 1e-4), "mixed3": (1e-3, 1e-3, 1e-3), "mixed3f64": (1e-3, 1e-3, 1e-3),
+        "mixed1": (3e-2, 3e-2, 3e-2)}
 
 
 def _rel(a, b):
@@ -90,12 +92,13 @@ CASES = {
 
 
 @pytest.mark.parametrize("case", sorted(CASES))
-@pytest.mark.parametrize("form", ["plain", "mixed3", "mixed1"])
+@pytest.mark.parametrize("form", ["plain", "mixed3", "mixed3f64", "mixed1"])
 def test_analysis_matches_oracle(lb, orc, case, form):
     kw = dict(CASES[case])
     preset, n, M = kw.pop("preset"), kw.pop("n"), kw.pop("M")
     variant = lb.VARIANT_PLAIN if form == "plain" else lb.VARIANT_MIXED
-    cfg = small_cfg(lb, preset, n=n, variant=variant, gram_passes=3 if form == "mixed3" else 1, **kw)
+    cfg = small_cfg(lb, preset, n=n, variant=variant, gram_passes=1 if form == "mixed1" else 3,
+                    jacobi_fp64_sweeps=1 if form == "mixed3f64" else 0, **kw)
     tol_w, tol_inc, tol_wv = TOLS[form]
     print(f"case: {case} {form}", flush=True)
     ens, nat = _osse(lb, cfg, M)
