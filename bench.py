@@ -14,6 +14,12 @@ cycle replayed from a hipGraph).  `value` is the mixed variant's DA cycles/s.  T
 analysis RMSE of each variant is taken at the observation time after the timed
 cycles; a few further cycles with phase timing on give the per-kernel breakdown.
 
+The mixed variant's launch shapes come from the autotuner (lbmletkf_autotune)
+before timing.  The JSON line also carries the uneven batch (four OSSEs of
+128^2..512^2 and M = 8..64 with random site densities, stepped and analysed by
+shared launches; --no-uneven skips it) and one LBM step with the learned closure
+against the Smagorinsky step.
+
 Prints ONE JSON line.  --gpus above 1 is not supported (one GPU per box).
 """
 import argparse
@@ -150,6 +156,19 @@ def bench_variant(lb, hip, cfg, variant, M, K, W, P, obs, truth, n_obs, log):
         log(f"    {ph:10s} {sec * 1e3 / P:9.3f} ms/cycle  launches {n}  work {w}   (phase-timed pass, {P} cycles)")
     return dict(wall=wall, event_s=ev_ms / 1e3, phases=phases, stats=stats, steps=steps, state=final_state,
                 yo_index=ca, prof_cycles=P, variant=name, tuning=tuning)
+
+
+def measured_traffic(kernel, workgroups):
+    """HBM MB per launch of one kernel shape, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (a separate profiled run of this benchmark)."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_256x256_M64.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip().strip("`") for c in line.strip().strip("|").split("|")]
+        if len(cells) == 7 and cells[0] == kernel and cells[1] == str(workgroups):
+            return float(cells[6]) * 1e6
+    return None
 
 
 def kernel_table(r, M, nodes):
@@ -401,9 +420,17 @@ def main():
     kern = kernel_table(head, M, nodes)
     dom_name = max(kern, key=lambda k: kern[k]["share"])
     dom = kern[dom_name]
+    tune = head.get("tuning") or [1, 8]
+    solve_wg = (nodes + tune[1] - 1) // tune[1]
+    traffic_solve = measured_traffic("k_mixed_solve<64, 3>", solve_wg) if head["variant"] == "mixed" else None
+    lbm_kernel, lbm_wg = (("k_lbm_rows<false, %d>" % -tune[0], (cfg.ny // -tune[0]) * M) if tune[0] < 0
+                          else ("k_lbm_fused<false>", cfg.nx // 64 * (cfg.ny // 16) * M // tune[0]))
+    traffic_lbm = measured_traffic(lbm_kernel, lbm_wg) if head["variant"] == "mixed" else None
     roof = dict(kernel=dom_name, bound=dom.get("bound", "hbm"), achieved=dom.get("achieved"),
                 peak=dom.get("peak"), unit=dom.get("unit"),
-                frac=(dom["achieved"] / dom["peak"]) if dom.get("peak") else None, traffic=None,
+                frac=(dom["achieved"] / dom["peak"]) if dom.get("peak") else None,
+                traffic=traffic_solve if dom_name == "letkf_solve" else None,
+                traffic_unit="bytes per launch, rocprofv3 FETCH_SIZE x2 + WRITE_SIZE (profiles/hbm_traffic_256x256_M64.md)",
                 avg_launch_ms=dom["avg_ms"], share_of_cycle=dom["share"])
     if "engine" in dom:
         roof["engine"] = dom["engine"]
@@ -432,8 +459,9 @@ def main():
                                                 if p != "solve_fp64"}}
                      for k, r in res.items()},
         "roofline": roof,
-        "roofline_lbm": dict(bound="hbm", achieved=lbm["achieved"], peak=HBM_PEAK_GBS, unit="GB/s",
-                             frac=lbm["achieved"] / HBM_PEAK_GBS, traffic=None,
+        "roofline_lbm": dict(kernel=lbm_kernel, bound="hbm", achieved=lbm["achieved"], peak=HBM_PEAK_GBS,
+                             unit="GB/s", frac=lbm["achieved"] / HBM_PEAK_GBS, traffic=traffic_lbm,
+                             algorithmic_bytes=LBM_BYTES_PER_NODE_UPDATE * M * nodes,
                              avg_launch_ms=lbm["avg_ms"] / lbm["launches_per_step"]),
         "kernels": kern,
     }

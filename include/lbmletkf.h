@@ -101,7 +101,7 @@ typedef struct lbmletkf_config {
     double closure_nu_scale;     /* MLP output nu_t = this scale * softplus(z)               */
     double closure_nu_max;       /* nu_t above this (or not finite) falls back to Smagorinsky */
     /* MIXED: the FP32 phase stops after the sweep whose largest pair cosine is at
-     * most this (0: 3e-6), or once that cosine stops falling quadratically below 1e-4 */
+     * most this (0: 3e-6), or once that cosine, below 1e-4, no longer halves per sweep */
     double jacobi_tol32;
 } lbmletkf_config;
 
