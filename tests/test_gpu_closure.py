@@ -278,3 +278,22 @@ def test_trained_closure_gain_over_smagorinsky(lb):
     print(f"  a posteriori velocity RMSE vs filtered truth after {T} steps: persistence {base:.4e}  "
           f"Smagorinsky {res['smagorinsky']:.4e}  MLP {res['mlp']:.4e}", flush=True)
     assert np.isfinite(res["mlp"]) and res["mlp"] < base
+
+
+def test_trained_closure_in_the_letkf_osse(lb):
+    """The trained network inside the LETKF OSSE against Smagorinsky: both ensembles
+    assimilate observations of the filtered 1024^2 truth; the analysis RMSE of each
+    against that truth is reported, and both must beat the free ensemble."""
+    sys.path.insert(0, os.path.join(ROOT, "lbm2d-letkf_amd"))
+    import closure_osse
+
+    coarse = lb.preset("baseline")
+    ct = _trainer_module()
+    print("case: closure OSSE 256^2 M=16, 6 cycles against the filtered 1024^2 truth", flush=True)
+    p, _ = ct.train_closure(coarse, factor=4, seeds=tuple(range(1, 21)), n_snap=3, every=25, spinup=150,
+                            batch=4096, steps=1500, lr=2e-3)
+    r = closure_osse.closure_osse(coarse, p, members=16, cycles=6)
+    for k in ("free", "smagorinsky", "mlp"):
+        print(f"  {k:12s} analysis rmse " + " ".join(f"{x:.3e}" for x in r[k]), flush=True)
+    assert np.isfinite(r["mlp"]).all() and np.isfinite(r["smagorinsky"]).all()
+    assert r["mlp"][-1] < r["free"][-1] and r["smagorinsky"][-1] < r["free"][-1]
