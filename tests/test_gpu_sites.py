@@ -75,3 +75,23 @@ def test_uneven_analysis_matches_oracle(lb, orc, form, sites_reset):
         X = fb[:, :, g // n, g % n]
         xb = X.mean(0)
         np.testing.assert_allclose(fa[:, :, g // n, g % n], xb + math.sqrt(beta) * (X - xb), rtol=2e-6)
+
+
+def test_site_list_errors(lb):
+    cfg = small_cfg(lb, "baseline", n=64, loc_scale=4.0, loc_cutoff=10.0)
+    with lb.Context(cfg, 4) as ctx:
+        for bad in ([0, 5, 5], [-1], [64 * 64]):
+            with pytest.raises(lb.LbmLetkfError) as e:
+                ctx.set_obs_sites(bad)
+            assert e.value.status == lb.ERR_ARG
+        ctx.set_obs_sites([])  # no sites at all: every analysis is the inflated prior
+        assert ctx.obs_count() == (0, 0)
+        fb = ctx.get_all_states().astype(np.float64)
+        ctx.analysis(np.zeros(1, np.float32))
+        fa = ctx.get_all_states().astype(np.float64)
+        xb = fb.mean(0, keepdims=True)
+        np.testing.assert_allclose(fa, xb + math.sqrt(cfg.inflation) * (fb - xb), rtol=2e-6, atol=1e-9)
+    gc = small_cfg(lb, "ref", n=64)  # GC-window localization: lattice only
+    with lb.Context(gc, 2) as ctx:
+        with pytest.raises(lb.LbmLetkfError):
+            ctx.set_obs_sites([1, 2, 3])
