@@ -101,11 +101,12 @@ class Hip:
             return "unknown"
 
 
-def bench_variant(lb, hip, cfg, variant, M, K, W, P, obs, truth, n_obs, log):
+def bench_variant(lb, hip, cfg, variant, M, K, W, P, obs, truth, n_obs, log, repeats=1):
     """K timed DA cycles (after W untimed ones) through lbmletkf_cycle -- for the
     mixed variant the captured graph is replayed -- then the analysis RMSE at the
-    next observation time, then P more cycles with phase timing on (eager launches)
-    for the per-kernel breakdown."""
+    next observation time; then repeats-1 further timed runs of K cycles (their
+    median and spread are reported), then P more cycles with phase timing on (eager
+    launches) for the per-kernel breakdown."""
     cfg = cfg.copy()
     cfg.variant = variant
     ens = lb.Context(cfg, M)
@@ -125,37 +126,54 @@ def bench_variant(lb, hip, cfg, variant, M, K, W, P, obs, truth, n_obs, log):
     for c in range(W):
         ens.cycle_device(yo(c))
     ens.synchronize()
-    e0, e1 = hip.event(), hip.event()
-    hip.sync()
-    t0 = time.perf_counter()
-    hip.record(e0, ens.stream)
-    for c in range(W, W + K):
-        ens.cycle_device(yo(c))
-    hip.record(e1, ens.stream)
-    ens.synchronize()
-    hip.sync()
-    wall = time.perf_counter() - t0
-    ev_ms = hip.elapsed_ms(e0, e1)
+
+    def timed(c0):
+        e0, e1 = hip.event(), hip.event()
+        hip.sync()
+        t0 = time.perf_counter()
+        hip.record(e0, ens.stream)
+        for c in range(c0, c0 + K):
+            ens.cycle_device(yo(c))
+        hip.record(e1, ens.stream)
+        ens.synchronize()
+        hip.sync()
+        return time.perf_counter() - t0, hip.elapsed_ms(e0, e1)
+
+    walls, evs = [], []
+    w, e = timed(W)
+    walls.append(w)
+    evs.append(e)
     # analysis at the next observation time, against the nature run there
     ca = W + K
     ens.analysis_device(yo(ca))
     tp = truth + 4 * 3 * nodes * ca
     stats = ens.ensemble_stats_device(tp, tp + 4 * nodes, tp + 8 * nodes)
     final_state = ens.get_all_states()
-    ens.profile_enable(True)
-    for c in range(ca + 1, ca + 1 + P):
+    c = ca + 1
+    for _ in range(repeats - 1):
         ens.forecast(steps)
-        ens.analysis_device(yo(c))
+        w, e = timed(c)
+        walls.append(w)
+        evs.append(e)
+        c += K
+    ens.profile_enable(True)
+    for cc in range(c, c + P):
+        ens.forecast(steps)
+        ens.analysis_device(yo(cc))
     ens.synchronize()
     phases = {ph: ens.profile_read(ph) for ph in ("forecast", "obs", "gram", "solve", "finish", "solve_fp64")}
     ens.close()
     name = "plain" if variant == lb.VARIANT_PLAIN else "mixed"
-    log(f"[{name}] {K} cycles in {wall:.3f} s wall ({ev_ms / 1e3:.3f} s events): "
+    ws = sorted(walls)
+    wall = ws[len(ws) // 2]
+    q1, q3 = (np.percentile(ws, 25), np.percentile(ws, 75)) if len(ws) > 1 else (ws[0], ws[0])
+    log(f"[{name}] {K} cycles x {len(ws)} run(s): median {wall:.3f} s wall ({np.median(evs) / 1e3:.3f} s events): "
         f"{K / wall:.2f} cycles/s; analysis rmse {stats[0]:.4e} spread {stats[1]:.4e}")
-    for ph, (sec, n, w) in phases.items():
-        log(f"    {ph:10s} {sec * 1e3 / P:9.3f} ms/cycle  launches {n}  work {w}   (phase-timed pass, {P} cycles)")
-    return dict(wall=wall, event_s=ev_ms / 1e3, phases=phases, stats=stats, steps=steps, state=final_state,
-                yo_index=ca, prof_cycles=P, variant=name, tuning=tuning)
+    for ph, (sec, n, wk) in phases.items():
+        log(f"    {ph:10s} {sec * 1e3 / P:9.3f} ms/cycle  launches {n}  work {wk}   (phase-timed pass, {P} cycles)")
+    return dict(wall=wall, walls=walls, iqr_cycles_per_s=[K / q3, K / q1], event_s=float(np.median(evs)) / 1e3,
+                phases=phases, stats=stats, steps=steps, state=final_state, yo_index=ca, prof_cycles=P,
+                variant=name, tuning=tuning)
 
 
 def measured_traffic(kernel, workgroups):
@@ -356,8 +374,9 @@ def bench_uneven(lb, hip, K, W, log, tune_cache=None):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20, help="timed DA cycles")
-    ap.add_argument("--warmup", type=int, default=2, help="untimed DA cycles")
+    ap.add_argument("--steps", type=int, default=20, help="timed DA cycles per run")
+    ap.add_argument("--warmup", type=int, default=10, help="untimed DA cycles")
+    ap.add_argument("--runs", type=int, default=3, help="timed runs of the mixed variant (median reported)")
     ap.add_argument("--members", type=int, default=64)
     ap.add_argument("--variants", default="plain,mixed")
     ap.add_argument("--prof-cycles", type=int, default=3, help="extra phase-timed cycles (eager launches)")
@@ -400,7 +419,8 @@ def main():
     nat.init_member(0, 0)
     _, n_obs = nat.obs_count()
     P = args.prof_cycles
-    n_cyc = W + K + 1 + P
+    R = max(1, args.runs)
+    n_cyc = W + K * R + 1 + P
     obs = hip.malloc(4 * n_obs * n_cyc)
     truth = hip.malloc(4 * 3 * nodes * n_cyc)
     for c in range(n_cyc):
@@ -415,7 +435,8 @@ def main():
     res = {}
     for v in args.variants.split(","):
         var = lb.VARIANT_PLAIN if v == "plain" else lb.VARIANT_MIXED
-        res[v] = bench_variant(lb, hip, cfg, var, M, K, W, P, obs, truth, n_obs, log)
+        res[v] = bench_variant(lb, hip, cfg, var, M, K, W, P, obs, truth, n_obs, log,
+                               repeats=R if var == lb.VARIANT_MIXED else 1)
     head = res.get("mixed") or res[list(res)[0]]
     kern = kernel_table(head, M, nodes)
     dom_name = max(kern, key=lambda k: kern[k]["share"])
@@ -452,7 +473,10 @@ def main():
                                "(u,v) every 8th node, Gaussian L=12 cutoff 43.8, beta=1.1",
                    "nx": cfg.nx, "ny": cfg.ny, "members": M, "steps_per_cycle": cfg.da_interval,
                    "n_obs": n_obs, "variant_timed": "mixed" if "mixed" in res else list(res)[0]},
+        "runs": len(head["walls"]),
+        "cycles_per_s_iqr": head["iqr_cycles_per_s"],
         "variants": {k: {"cycles_per_s": K / r["wall"], "ms_per_cycle": r["wall"] / K * 1e3,
+                         "timed_runs": len(r["walls"]),
                          "analysis_rmse": r["stats"][0], "analysis_spread": r["stats"][1],
                          "tuning_lbm_tiles_per_block_and_chain": r["tuning"],
                          "phase_ms_per_cycle": {p: r["phases"][p][0] / P * 1e3 for p in r["phases"]
