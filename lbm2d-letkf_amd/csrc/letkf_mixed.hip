@@ -344,7 +344,7 @@ __device__ __forceinline__ void solve_body(int bx, int nbx, const int* __restric
 }
 
 template <int N, int PASSES>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_mixed_solve(
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_mixed_solve(
     const int* __restrict__ loc_ptr, const int* __restrict__ loc_site, const float* __restrict__ loc_w,
     const float* __restrict__ dy, const float* __restrict__ dyo, float* __restrict__ ws_u,
     float* __restrict__ ws_b, MixedParams mp) {
@@ -602,7 +602,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3))) void k
 // finds its task by scanning the (at most MIXED_GROUP_MAX) range starts.
 // ---------------------------------------------------------------------------------
 template <int PASSES>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_mixed_solve_group(MixedGroup G) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_mixed_solve_group(MixedGroup G) {
     int k = 0;
     while (k + 1 < G.n && int(blockIdx.x) >= G.t[k + 1].b0_solve) ++k;
     const MixedTask& T = G.t[k];
