@@ -102,44 +102,180 @@ __device__ inline int xcd_order(int b, int nb) { return (nb % 8 == 0) ? (b % 8) 
 [[maybe_unused]] __device__ inline float rot_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 [[maybe_unused]] __device__ inline double rot_rsq(double x) { return 1.0 / sqrt(x); }
 
-// One round of the one-sided (Hestenes) Jacobi method on the column pairs
-// (p, p ^ m), p with the top bit of m clear: over the N - 1 rounds m = 1..N-1 of a
-// sweep every pair of columns meets once.  Lane j holds column j of U as sc * u[]
-// and al = ||column||^2.  The rotation that makes the two columns orthogonal is
-// the Jacobi rotation of their 2x2 Gram matrix [[a_p, g], [g, a_q]]
-// (t = sgn(z) / (|z| + sqrt(1 + z^2)), z = (a_q - a_p) / 2g, c = 1 / sqrt(1 + t^2)):
-//   u_p' = c (u_p - t u_q),  u_q' = c (u_q + t u_p),
-// applied as one FMA per element with the factor c folded into the column scale.
-// Both lanes of a pair form g, z, t from the same operands in the same order, so
-// they agree on the rotation bit for bit.  Returns |g| / sqrt(a_p a_q) before the
-// rotation.  The partner column is fetched once and held in registers.
-template <int N, class T>
-__device__ inline T hestenes_round(T (&u)[N], T& sc, T& al, int m, int lane) {
-    const bool low = (lane & (1 << (31 - __clz(m)))) == 0;
-    T o[N];
-    T g4[4] = {T(0), T(0), T(0), T(0)};
+// Cross-lane reads of the sweeps.  A lane reads the value of lane ^ m: through the
+// LDS crossbar (ds_bpermute, any m) or, for the masks the data-parallel primitives
+// express within a row of 16 lanes, as a VALU move: quad_perm [1,0,3,2] (^1),
+// [2,3,0,1] (^2), [3,2,1,0] (^3), row_ror:8 (^8).
+struct XorBperm {
+    int addr;  // (lane ^ m) * 4
+    __device__ float operator()(float v) const {
+        return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(v)));
+    }
+};
+template <int CTRL>
+struct XorDpp {
+    __device__ float operator()(float v) const {
+        return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
+    }
+};
+constexpr int DPP_XOR1 = 0xB1, DPP_XOR2 = 0x4E, DPP_XOR3 = 0x1B, DPP_XOR8 = 0x128;
+
+// Jacobi rotation of the 2x2 Gram matrix [[a_p, g], [g, a_q]] of a column pair:
+// t = sgn(z) / (|z| + sqrt(1 + z^2)), z = (a_q - a_p) / 2g, c = 1 / sqrt(1 + t^2);
+// u_p' = c (u_p - t u_q), u_q' = c (u_q + t u_p).  gam = g scaled by the column
+// scales; t = 0, c = 1 when gam = 0 (|z| -> inf for tiny g gives t -> 0 through
+// the reciprocal of inf).
+__device__ inline void jacobi_tc(float ap, float aq, float gam, float& t, float& c) {
+    t = 0.f;
+    c = 1.f;
+    if (gam != 0.f) {
+        const float z = (aq - ap) * rot_rcp(2.f * gam);
+        t = (z >= 0.f ? 1.f : -1.f) * rot_rcp(fabsf(z) + rot_sqrt(z * z + 1.f));
+        c = rot_rsq(t * t + 1.f);
+    }
+}
+
+// One round of the one-sided (Hestenes) Jacobi method with whole columns: lane j
+// holds column j as sc * u[] and al = ||column||^2; the pair of lane j is the lane
+// that `xch` reads (lane ^ m).  The factor c of the rotation is folded into the
+// column scale, so the update is one FMA per element.  Both lanes of a pair form
+// g, z, t from the same operands in the same order, so they agree on the rotation
+// bit for bit.  Returns |g| / sqrt(a_p a_q) before the rotation.
+template <int N, class X>
+__device__ inline float hestenes_round(float (&u)[N], float& sc, float& al, bool low, X xch) {
+    float o[N];
+    float g4[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int i = 0; i < N; ++i) {
-        o[i] = __shfl_xor(u[i], m);
-        g4[i & 3] = fma(u[i], o[i], g4[i & 3]);
+        o[i] = xch(u[i]);
+        g4[i & 3] = fmaf(u[i], o[i], g4[i & 3]);
     }
-    const T sc_o = __shfl_xor(sc, m), al_o = __shfl_xor(al, m);
-    const T gam = (sc * sc_o) * ((g4[0] + g4[1]) + (g4[2] + g4[3]));
-    const T ap = low ? al : al_o, aq = low ? al_o : al;
-    T t = T(0), c = T(1);
-    if (gam != T(0)) {
-        // |z| -> inf (tiny g) gives t -> 0 through the reciprocal of inf
-        const T z = (aq - ap) * rot_rcp(T(2) * gam);
-        t = (z >= T(0) ? T(1) : T(-1)) * rot_rcp(fabs(z) + rot_sqrt(z * z + T(1)));
-        c = rot_rsq(t * t + T(1));
-    }
-    const T coef = (low ? -t : t) * (sc_o * rot_rcp(sc));
+    const float sc_o = xch(sc), al_o = xch(al);
+    const float gam = (sc * sc_o) * ((g4[0] + g4[1]) + (g4[2] + g4[3]));
+    const float ap = low ? al : al_o, aq = low ? al_o : al;
+    float t, c;
+    jacobi_tc(ap, aq, gam, t, c);
+    const float coef = (low ? -t : t) * (sc_o * rot_rcp(sc));
 #pragma unroll
-    for (int i = 0; i < N; ++i) u[i] = fma(coef, o[i], u[i]);
+    for (int i = 0; i < N; ++i) u[i] = fmaf(coef, o[i], u[i]);
     sc *= c;
     al = low ? ap - t * gam : aq + t * gam;
-    const T den = ap * aq;
-    return den > T(0) ? fabs(gam) * rot_rsq(den) : T(0);
+    const float den = ap * aq;
+    return den > 0.f ? fabsf(gam) * rot_rsq(den) : 0.f;
+}
+
+// One round with split columns.  For the pair (P, Q) = (p, p ^ m), p with bit b
+// of m clear, the lane of P holds rows [0, N/2) of both columns and the lane of Q
+// rows [N/2, N): x[] the lane's own column, y[] its partner's ("guest" rows),
+// each with its scale and squared norm (sx, ax; sy, ay), kept bit-identical in
+// both lanes.  The pair inner product is the sum of the two lanes' half sums (one
+// exchange); each lane then updates its rows of both columns.  No column moves.
+template <int H>
+__device__ inline float split_round(float (&x)[H], float (&y)[H], float& sx, float& ax, float& sy, float& ay,
+                                    bool hi, XorBperm xch) {
+    float s4[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int i = 0; i < H; ++i) s4[i & 3] = fmaf(x[i], y[i], s4[i & 3]);
+    const float s = (s4[0] + s4[1]) + (s4[2] + s4[3]);
+    const float g = s + xch(s);  // the same sum in both lanes (addition commutes)
+    const float aP = hi ? ay : ax, aQ = hi ? ax : ay;
+    const float sP = hi ? sy : sx, sQ = hi ? sx : sy;
+    const float gam = (sP * sQ) * g;
+    float t, c;
+    jacobi_tc(aP, aQ, gam, t, c);
+    const float cP = -t * (sQ * rot_rcp(sP)), cQ = t * (sP * rot_rcp(sQ));
+    const float co = hi ? cQ : cP, cg = hi ? cP : cQ;
+#pragma unroll
+    for (int i = 0; i < H; ++i) {
+        const float xo = x[i];
+        x[i] = fmaf(co, y[i], x[i]);
+        y[i] = fmaf(cg, xo, y[i]);
+    }
+    sx *= c;
+    sy *= c;
+    const float tg = t * gam;
+    const float aPn = aP - tg, aQn = aQ + tg;
+    ax = hi ? aQn : aPn;
+    ay = hi ? aPn : aQn;
+    const float den = aP * aQ;
+    return den > 0.f ? fabsf(gam) * rot_rsq(den) : 0.f;
+}
+
+// guest rows move to lane ^ (1 << j): y'[lane] = y[lane ^ (1 << j)]
+template <int H, class X>
+__device__ inline void move_guest(float (&y)[H], float& sy, float& ay, X xch) {
+#pragma unroll
+    for (int i = 0; i < H; ++i) y[i] = xch(y[i]);
+    sy = xch(sy);
+    ay = xch(ay);
+}
+
+// One cyclic sweep (every column pair once) of the FP32 one-sided Jacobi method
+// on columns held whole (lane j = column j, N columns), returning the largest pair
+// cosine met.  The rounds m = 4..N-1 go in groups of equal top bit b of m: for
+// group b, lanes with bit b clear keep the first N/2 rows of their column and
+// receive the same rows of their partner's, the others the last N/2 rows (one
+// crossbar exchange of half a column), so each round costs half the inner-product
+// FMAs and no column exchange.  Inside a group the masks follow a Gray code: from
+// m to m ^ (1 << j) the guest rows move to lane ^ (1 << j), a quad permutation or a
+// row rotation for j = 0, 1, 3.  The group ends by returning the guest rows to their
+// lanes.  The rounds m = 1, 2, 3 pair lanes inside a quad and exchange whole
+// columns by quad permutations.
+template <int N>
+__device__ inline float hestenes_sweep(float (&u)[N], float& sc, float& al, int lane) {
+    constexpr int H = N / 2;
+    constexpr int LG = (N == 64) ? 6 : (N == 32) ? 5 : 4;
+    static_assert(N == 16 || N == 32 || N == 64, "column count");
+    float mc = 0.f;
+    for (int b = LG - 1; b >= 2; --b) {
+        const bool hi = ((lane >> b) & 1) != 0;
+        const int gsize = 1 << b;
+        float x[H], y[H];
+        {
+            const XorBperm to{(lane ^ gsize) << 2};
+#pragma unroll
+            for (int i = 0; i < H; ++i) {
+                x[i] = hi ? u[H + i] : u[i];
+                y[i] = to(hi ? u[i] : u[H + i]);
+            }
+        }
+        float sx = sc, ax = al;
+        float sy = XorBperm{(lane ^ gsize) << 2}(sc), ay = XorBperm{(lane ^ gsize) << 2}(al);
+        int m = gsize;
+        // rounds in twos (gsize is even): after round k the Gray code flips bit
+        // ctz(k + 1), which is bit 0 for even k;
+        // the pairs let the two rounds' updates alternate registers without copies
+        for (int k = 0;; k += 2) {
+            mc = fmaxf(mc, split_round<H>(x, y, sx, ax, sy, ay, hi, XorBperm{(lane ^ m) << 2}));
+            m ^= 1;
+            move_guest<H>(y, sy, ay, XorDpp<DPP_XOR1>{});
+            mc = fmaxf(mc, split_round<H>(x, y, sx, ax, sy, ay, hi, XorBperm{(lane ^ m) << 2}));
+            if (k + 2 == gsize) break;
+            const int j = __builtin_ctz(k + 2);  // the Gray-code bit that flips
+            m ^= 1 << j;
+            if (j == 1)
+                move_guest<H>(y, sy, ay, XorDpp<DPP_XOR2>{});
+            else if (j == 3)
+                move_guest<H>(y, sy, ay, XorDpp<DPP_XOR8>{});
+            else
+                move_guest<H>(y, sy, ay, XorBperm{(lane ^ (1 << j)) << 2});
+        }
+        {
+            const XorBperm back{(lane ^ m) << 2};
+#pragma unroll
+            for (int i = 0; i < H; ++i) {
+                const float tv = back(y[i]);
+                u[i] = hi ? tv : x[i];
+                u[H + i] = hi ? x[i] : tv;
+            }
+        }
+        sc = sx;
+        al = ax;
+    }
+    mc = fmaxf(mc, hestenes_round<N>(u, sc, al, (lane & 1) == 0, XorDpp<DPP_XOR1>{}));
+    mc = fmaxf(mc, hestenes_round<N>(u, sc, al, (lane & 2) == 0, XorDpp<DPP_XOR2>{}));
+    mc = fmaxf(mc, hestenes_round<N>(u, sc, al, (lane & 2) == 0, XorDpp<DPP_XOR3>{}));
+    return mc;
 }
 
 // fold the column scale into u and refresh the squared norm
@@ -327,8 +463,7 @@ __device__ __forceinline__ void solve_body(int bx, int nbx, const int* __restric
         int sw32 = 0;
         while (sw32 < mp.max32) {
             hestenes_renorm<N, float>(u, sc, al);
-            float mc = 0.f;
-            for (int m = 1; m < N; ++m) mc = fmaxf(mc, hestenes_round<N, float>(u, sc, al, m, lane));
+            float mc = hestenes_sweep<N>(u, sc, al, lane);
             mc = wave_max(mc);
             ++sw32;
             if (mc <= mp.tol32 || (mc <= 1e-4f && mc > 0.5f * mc_prev)) break;
