@@ -2,7 +2,7 @@
 // src/util/letkf_solver.h:235-349 at every gridpoint, one wavefront per gridpoint,
 // in two kernels:
 //
-// k_mixed_solve (FP32, ~150 VGPRs, C in 16 KB of LDS; a chain of neighbouring
+// k_mixed_solve (FP32, ~200 VGPRs, C in 16 KB of LDS; a chain of neighbouring
 // gridpoints per wavefront, each warm-started from its predecessor's eigenvectors)
 //  1. gather: the rows of dY (and the innovation) of the gridpoint's local
 //     observations, scaled by sqrt(R^-1 o localization), straight from L2;
@@ -10,11 +10,14 @@
 //     with v_mfma_f32_16x16x32_bf16 (FP32 accumulation); PASSES = 3 splits each
 //     operand into bf16 hi + lo parts (hi*hi + hi*lo + lo*hi);
 //  3. cyclic one-sided (Hestenes) Jacobi in FP32 on U = C, register resident:
-//     lane j holds column j, and the rotation of the column pair (p, p^m) of
-//     round m needs only the partner column (__shfl_xor by m) and one FMA per
-//     element.  No LDS and no barrier inside a sweep.  As C is symmetric positive
-//     definite, U -> C V = V D, so the eigenpairs are d_j = ||u_j||, v_j = u_j / d_j
-//     and V is never accumulated.  U and b go to a workspace in HBM.
+//     lane j owns column j; round m rotates the column pairs (p, p^m).  For most
+//     rounds the two lanes of a pair each hold half the rows of both columns, so
+//     a round is a half inner product per lane, one exchange of the partial sum
+//     and one FMA per held element; between rounds the partner's rows move by DPP
+//     lane permutations, with few crossbar exchanges.  No LDS memory and no
+//     barrier inside a sweep.  As C is symmetric positive definite,
+//     U -> C V = V D, so the eigenpairs are d_j = ||u_j||, v_j = u_j / d_j and V
+//     is never accumulated.  U and b go to a workspace in HBM.
 //
 // k_mixed_finish (FP64, two wavefronts per gridpoint, each with half the rows)
 //  4. the same sweeps in FP64 on the widened columns (partner half-column held in
@@ -185,11 +188,13 @@ __device__ inline float split_round(float (&x)[H], float (&y)[H], float& sx, flo
     jacobi_tc(aP, aQ, gam, t, c);
     const float cP = -t * (sQ * rot_rcp(sP)), cQ = t * (sP * rot_rcp(sQ));
     const float co = hi ? cQ : cP, cg = hi ? cP : cQ;
+    // the guest rows' new values go to fresh registers (the move that follows
+    // writes them back), the own rows are updated in place
 #pragma unroll
     for (int i = 0; i < H; ++i) {
-        const float xo = x[i];
+        const float yn = fmaf(cg, x[i], y[i]);
         x[i] = fmaf(co, y[i], x[i]);
-        y[i] = fmaf(cg, xo, y[i]);
+        y[i] = yn;
     }
     sx *= c;
     sy *= c;
