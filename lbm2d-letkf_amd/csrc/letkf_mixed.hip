@@ -123,29 +123,30 @@ struct XorDpp {
 };
 constexpr int DPP_XOR1 = 0xB1, DPP_XOR2 = 0x4E, DPP_XOR3 = 0x1B, DPP_XOR8 = 0x128;
 
-// Jacobi rotation of the 2x2 Gram matrix [[a_p, g], [g, a_q]] of a column pair:
-// t = sgn(z) / (|z| + sqrt(1 + z^2)), z = (a_q - a_p) / 2g, c = 1 / sqrt(1 + t^2);
-// u_p' = c (u_p - t u_q), u_q' = c (u_q + t u_p).  gam = g scaled by the column
-// scales; t = 0, c = 1 when gam = 0 (|z| -> inf for tiny g gives t -> 0 through
-// the reciprocal of inf).
+// Jacobi rotation of the 2x2 Gram matrix [[a_p, g], [g, a_q]] of a column pair,
+// u_p' = c (u_p - t u_q), u_q' = c (u_q + t u_p), with t = tan(theta) of the smaller
+// angle: for d = a_q - a_p and e = 2g (scaled by the column scales),
+//   t = sgn(d) e / (|d| + sqrt(d^2 + e^2)),   c = 1 / sqrt(1 + t^2),
+// which is the textbook t = sgn(z) / (|z| + sqrt(1 + z^2)), z = d / e, multiplied
+// through by e: one square root and one reciprocal, no quotient d / e, and t = 0
+// for e = 0.  d^2 + e^2 stays finite while the squared column norms stay below
+// ~1e19, which the caller ensures by scaling U by a power of two.
 __device__ inline void jacobi_tc(float ap, float aq, float gam, float& t, float& c) {
-    t = 0.f;
-    c = 1.f;
-    if (gam != 0.f) {
-        const float z = (aq - ap) * rot_rcp(2.f * gam);
-        t = (z >= 0.f ? 1.f : -1.f) * rot_rcp(fabsf(z) + rot_sqrt(z * z + 1.f));
-        c = rot_rsq(t * t + 1.f);
-    }
+    const float d = aq - ap, e = 2.f * gam;
+    const float den = fabsf(d) + rot_sqrt(fmaf(d, d, e * e));
+    t = den > 0.f ? (d >= 0.f ? e : -e) * rot_rcp(den) : 0.f;
+    c = rot_rsq(fmaf(t, t, 1.f));
 }
 
 // One round of the one-sided (Hestenes) Jacobi method with whole columns: lane j
-// holds column j as sc * u[] and al = ||column||^2; the pair of lane j is the lane
-// that `xch` reads (lane ^ m).  The factor c of the rotation is folded into the
-// column scale, so the update is one FMA per element.  Both lanes of a pair form
-// g, z, t from the same operands in the same order, so they agree on the rotation
-// bit for bit.  Returns |g| / sqrt(a_p a_q) before the rotation.
+// holds column j as sc * u[], its reciprocal scale isc and al = ||column||^2; the
+// pair of lane j is the lane that `xch` reads (lane ^ m).  The factor c of the
+// rotation is folded into the column scale (and 1/c = (1 + t^2) c into isc), so the
+// update is one FMA per element.  Both lanes of a pair form the rotation from the
+// same operands in the same order, so they agree on it bit for bit.  Returns
+// |g| / sqrt(a_p a_q) before the rotation.
 template <int N, class X>
-__device__ inline float hestenes_round(float (&u)[N], float& sc, float& al, bool low, X xch) {
+__device__ inline float hestenes_round(float (&u)[N], float& sc, float& isc, float& al, bool low, X xch) {
     float o[N];
     float g4[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -158,10 +159,11 @@ __device__ inline float hestenes_round(float (&u)[N], float& sc, float& al, bool
     const float ap = low ? al : al_o, aq = low ? al_o : al;
     float t, c;
     jacobi_tc(ap, aq, gam, t, c);
-    const float coef = (low ? -t : t) * (sc_o * rot_rcp(sc));
+    const float coef = (low ? -t : t) * (sc_o * isc);
 #pragma unroll
     for (int i = 0; i < N; ++i) u[i] = fmaf(coef, o[i], u[i]);
     sc *= c;
+    isc *= fmaf(t, t, 1.f) * c;
     al = low ? ap - t * gam : aq + t * gam;
     const float den = ap * aq;
     return den > 0.f ? fabsf(gam) * rot_rsq(den) : 0.f;
@@ -170,12 +172,13 @@ __device__ inline float hestenes_round(float (&u)[N], float& sc, float& al, bool
 // One round with split columns.  For the pair (P, Q) = (p, p ^ m), p with bit b
 // of m clear, the lane of P holds rows [0, N/2) of both columns and the lane of Q
 // rows [N/2, N): x[] the lane's own column, y[] its partner's ("guest" rows),
-// each with its scale and squared norm (sx, ax; sy, ay), kept bit-identical in
-// both lanes.  The pair inner product is the sum of the two lanes' half sums (one
-// exchange); each lane then updates its rows of both columns.  No column moves.
+// each with its scale, reciprocal scale and squared norm (sx, ix, ax; sy, iy, ay),
+// kept bit-identical in both lanes.  The pair inner product is the sum of the two
+// lanes' half sums (one exchange); each lane then updates its rows of both
+// columns.  No column moves.
 template <int H>
-__device__ inline float split_round(float (&x)[H], float (&y)[H], float& sx, float& ax, float& sy, float& ay,
-                                    bool hi, XorBperm xch) {
+__device__ inline float split_round(float (&x)[H], float (&y)[H], float& sx, float& ix, float& ax, float& sy,
+                                    float& iy, float& ay, bool hi, XorBperm xch) {
     float s4[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int i = 0; i < H; ++i) s4[i & 3] = fmaf(x[i], y[i], s4[i & 3]);
@@ -183,10 +186,11 @@ __device__ inline float split_round(float (&x)[H], float (&y)[H], float& sx, flo
     const float g = s + xch(s);  // the same sum in both lanes (addition commutes)
     const float aP = hi ? ay : ax, aQ = hi ? ax : ay;
     const float sP = hi ? sy : sx, sQ = hi ? sx : sy;
+    const float iP = hi ? iy : ix, iQ = hi ? ix : iy;
     const float gam = (sP * sQ) * g;
     float t, c;
     jacobi_tc(aP, aQ, gam, t, c);
-    const float cP = -t * (sQ * rot_rcp(sP)), cQ = t * (sP * rot_rcp(sQ));
+    const float cP = -t * (sQ * iP), cQ = t * (sP * iQ);
     const float co = hi ? cQ : cP, cg = hi ? cP : cQ;
     // the guest rows' new values go to fresh registers (the move that follows
     // writes them back), the own rows are updated in place
@@ -196,8 +200,11 @@ __device__ inline float split_round(float (&x)[H], float (&y)[H], float& sx, flo
         x[i] = fmaf(co, y[i], x[i]);
         y[i] = yn;
     }
+    const float ic = fmaf(t, t, 1.f) * c;
     sx *= c;
     sy *= c;
+    ix *= ic;
+    iy *= ic;
     const float tg = t * gam;
     const float aPn = aP - tg, aQn = aQ + tg;
     ax = hi ? aQn : aPn;
@@ -208,11 +215,25 @@ __device__ inline float split_round(float (&x)[H], float (&y)[H], float& sx, flo
 
 // guest rows move to lane ^ (1 << j): y'[lane] = y[lane ^ (1 << j)]
 template <int H, class X>
-__device__ inline void move_guest(float (&y)[H], float& sy, float& ay, X xch) {
+__device__ inline void move_guest(float (&y)[H], float& sy, float& iy, float& ay, X xch) {
 #pragma unroll
     for (int i = 0; i < H; ++i) y[i] = xch(y[i]);
     sy = xch(sy);
+    iy = xch(iy);
     ay = xch(ay);
+}
+
+// guest rows move to lane ^ (1 << lb): a DPP move for lb = 0, 1, 3, else the crossbar
+template <int H>
+__device__ inline void move_guest_bit(float (&y)[H], float& sy, float& iy, float& ay, int lb, int lane) {
+    if (lb == 0)
+        move_guest<H>(y, sy, iy, ay, XorDpp<DPP_XOR1>{});
+    else if (lb == 1)
+        move_guest<H>(y, sy, iy, ay, XorDpp<DPP_XOR2>{});
+    else if (lb == 3)
+        move_guest<H>(y, sy, iy, ay, XorDpp<DPP_XOR8>{});
+    else
+        move_guest<H>(y, sy, iy, ay, XorBperm{(lane ^ (1 << lb)) << 2});
 }
 
 // One cyclic sweep (every column pair once) of the FP32 one-sided Jacobi method
@@ -227,7 +248,7 @@ __device__ inline void move_guest(float (&y)[H], float& sy, float& ay, X xch) {
 // lanes.  The rounds m = 1, 2, 3 pair lanes inside a quad and exchange whole
 // columns by quad permutations.
 template <int N>
-__device__ inline float hestenes_sweep(float (&u)[N], float& sc, float& al, int lane) {
+__device__ inline float hestenes_sweep(float (&u)[N], float& sc, float& isc, float& al, int lane) {
     constexpr int H = N / 2;
     constexpr int LG = (N == 64) ? 6 : (N == 32) ? 5 : 4;
     static_assert(N == 16 || N == 32 || N == 64, "column count");
@@ -244,26 +265,24 @@ __device__ inline float hestenes_sweep(float (&u)[N], float& sc, float& al, int 
                 y[i] = to(hi ? u[i] : u[H + i]);
             }
         }
-        float sx = sc, ax = al;
-        float sy = XorBperm{(lane ^ gsize) << 2}(sc), ay = XorBperm{(lane ^ gsize) << 2}(al);
+        float sx = sc, ix = isc, ax = al;
+        float sy = XorBperm{(lane ^ gsize) << 2}(sc), iy = XorBperm{(lane ^ gsize) << 2}(isc),
+              ay = XorBperm{(lane ^ gsize) << 2}(al);
         int m = gsize;
         // rounds in twos (gsize is even): after round k the Gray code flips bit
-        // ctz(k + 1), which is bit 0 for even k;
-        // the pairs let the two rounds' updates alternate registers without copies
+        // ctz(k + 1), which is bit 0 (a quad permutation) for every even k; the pairs
+        // also let the two rounds' updates alternate registers without copies.
+        // (Putting the frequent move on the crossbar instead, to offload the VALU,
+        // measured 5% slower: the crossbar is the scarcer of the two.)
         for (int k = 0;; k += 2) {
-            mc = fmaxf(mc, split_round<H>(x, y, sx, ax, sy, ay, hi, XorBperm{(lane ^ m) << 2}));
+            mc = fmaxf(mc, split_round<H>(x, y, sx, ix, ax, sy, iy, ay, hi, XorBperm{(lane ^ m) << 2}));
             m ^= 1;
-            move_guest<H>(y, sy, ay, XorDpp<DPP_XOR1>{});
-            mc = fmaxf(mc, split_round<H>(x, y, sx, ax, sy, ay, hi, XorBperm{(lane ^ m) << 2}));
+            move_guest<H>(y, sy, iy, ay, XorDpp<DPP_XOR1>{});
+            mc = fmaxf(mc, split_round<H>(x, y, sx, ix, ax, sy, iy, ay, hi, XorBperm{(lane ^ m) << 2}));
             if (k + 2 == gsize) break;
-            const int j = __builtin_ctz(k + 2);  // the Gray-code bit that flips
+            const int j = __builtin_ctz(k + 2);  // the Gray-code bit that flips (>= 1)
             m ^= 1 << j;
-            if (j == 1)
-                move_guest<H>(y, sy, ay, XorDpp<DPP_XOR2>{});
-            else if (j == 3)
-                move_guest<H>(y, sy, ay, XorDpp<DPP_XOR8>{});
-            else
-                move_guest<H>(y, sy, ay, XorBperm{(lane ^ (1 << j)) << 2});
+            move_guest_bit<H>(y, sy, iy, ay, j, lane);
         }
         {
             const XorBperm back{(lane ^ m) << 2};
@@ -275,11 +294,12 @@ __device__ inline float hestenes_sweep(float (&u)[N], float& sc, float& al, int 
             }
         }
         sc = sx;
+        isc = ix;
         al = ax;
     }
-    mc = fmaxf(mc, hestenes_round<N>(u, sc, al, (lane & 1) == 0, XorDpp<DPP_XOR1>{}));
-    mc = fmaxf(mc, hestenes_round<N>(u, sc, al, (lane & 2) == 0, XorDpp<DPP_XOR2>{}));
-    mc = fmaxf(mc, hestenes_round<N>(u, sc, al, (lane & 2) == 0, XorDpp<DPP_XOR3>{}));
+    mc = fmaxf(mc, hestenes_round<N>(u, sc, isc, al, (lane & 1) == 0, XorDpp<DPP_XOR1>{}));
+    mc = fmaxf(mc, hestenes_round<N>(u, sc, isc, al, (lane & 2) == 0, XorDpp<DPP_XOR2>{}));
+    mc = fmaxf(mc, hestenes_round<N>(u, sc, isc, al, (lane & 2) == 0, XorDpp<DPP_XOR3>{}));
     return mc;
 }
 
@@ -452,10 +472,20 @@ __device__ __forceinline__ void solve_body(int bx, int nbx, const int* __restric
             __syncthreads();
         }
         // column `lane` (C symmetric, so row and column read alike)
+        float a0 = 0.f;
 #pragma unroll
         for (int i = 0; i < N; ++i) {
             const float x = cfull[i * N + l];
             u[i] = (lane < N) ? x : 0.f;
+            a0 = fmaf(u[i], u[i], a0);
+        }
+        // keep the squared column norms below 1e18 (the range of the rotation
+        // formula): scale U by 2^-32 when needed, exactly, and undo it on output
+        float uscale = 1.f;
+        if (wave_max(a0) > 1e18f) {
+            uscale = 0x1p32f;
+#pragma unroll
+            for (int i = 0; i < N; ++i) u[i] *= 0x1p-32f;
         }
 
         // ---- 3. FP32 one-sided Jacobi ------------------------------------------------
@@ -468,7 +498,8 @@ __device__ __forceinline__ void solve_body(int bx, int nbx, const int* __restric
         int sw32 = 0;
         while (sw32 < mp.max32) {
             hestenes_renorm<N, float>(u, sc, al);
-            float mc = hestenes_sweep<N>(u, sc, al, lane);
+            float isc = 1.f;
+            float mc = hestenes_sweep<N>(u, sc, isc, al, lane);
             mc = wave_max(mc);
             ++sw32;
             if (mc <= mp.tol32 || (mc <= 1e-4f && mc > 0.5f * mc_prev)) break;
@@ -477,8 +508,9 @@ __device__ __forceinline__ void solve_body(int bx, int nbx, const int* __restric
         if (lane == 0 && mp.sweeps32) atomicAdd(mp.sweeps32, (unsigned long long)sw32);
         if (lane < N) {
             float* ug = ws_u + (long long)g * N * N;
+            const float so = sc * uscale;
 #pragma unroll
-            for (int i = 0; i < N; ++i) ug[i * N + lane] = u[i] * sc;
+            for (int i = 0; i < N; ++i) ug[i * N + lane] = u[i] * so;
         }
     }
 }

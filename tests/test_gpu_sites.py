@@ -68,8 +68,9 @@ def test_uneven_analysis_matches_oracle(lb, orc, form, sites_reset):
     empty = [i for i in range(nd) if orc.local_obs(cfg, g0 + i)[0].size == 0]
     assert len(empty) > 5
     for i in empty:
-        # C = (M-1)/beta I is formed in FP32, so W = sqrt(beta) I to FP32 rounding
-        np.testing.assert_allclose(W[i], math.sqrt(beta) * np.eye(M), atol=2e-7)
+        # C = (M-1)/beta I is formed in FP32, so W = sqrt(beta) I to a few FP32 ulps
+        # (MIXED: V is the warm-start basis made orthogonal to FP32 rounding)
+        np.testing.assert_allclose(W[i], math.sqrt(beta) * np.eye(M), atol=4e-7)
         np.testing.assert_allclose(w[i], 0.0, atol=1e-15)
         g = g0 + i
         X = fb[:, :, g // n, g % n]
