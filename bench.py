@@ -189,6 +189,12 @@ def measured_traffic(kernel, workgroups):
     return None
 
 
+def lbm_steps_per_launch(tune):
+    """Steps one launch of the tuned LBM shape takes: two for the two-step row
+    strips (shape code -(100 + h); the benchmark flow has no body force)."""
+    return 2 if tune and tune[0] <= -100 else 1
+
+
 def kernel_table(r, M, nodes):
     """Average per-launch duration and algorithmic rate of the main kernels, from the
     phase-timed cycles (HIP events on the context's stream around each phase)."""
@@ -196,10 +202,14 @@ def kernel_table(r, M, nodes):
     P = r["prof_cycles"]
     sec, n, work = r["phases"]["forecast"]
     n_steps = r["steps"] * P
-    lbm_launches_per_step = max(1, (n - 1) // n_steps) if n_steps else 1
+    spl = lbm_steps_per_launch(r.get("tuning"))
     avg = sec / n_steps  # one forecast step (all launches of the step)
-    out["lbm_step"] = dict(avg_ms=avg * 1e3, launches_per_step=lbm_launches_per_step,
-                           achieved=LBM_BYTES_PER_NODE_UPDATE * M * nodes / avg / 1e9, unit="GB/s",
+    # algorithmic HBM bytes of one launch: every population read once and written
+    # once, whether the launch takes one step or two
+    out["lbm_step"] = dict(avg_ms=avg * 1e3, steps_per_launch=spl, avg_launch_ms=avg * spl * 1e3,
+                           achieved=LBM_BYTES_PER_NODE_UPDATE * M * nodes / (avg * spl) / 1e9, unit="GB/s",
+                           node_updates_per_s=M * nodes / avg,
+                           bytes_of_single_steps_GBs=LBM_BYTES_PER_NODE_UPDATE * M * nodes / avg / 1e9,
                            peak=HBM_PEAK_GBS, bound="hbm", share=sec)
     sec, n, sweeps = r["phases"]["solve"]
     if r["variant"] == "mixed":
@@ -444,8 +454,12 @@ def main():
     tune = head.get("tuning") or [1, 8]
     solve_wg = (nodes + tune[1] - 1) // tune[1]
     traffic_solve = measured_traffic("k_mixed_solve<64, 3>", solve_wg) if head["variant"] == "mixed" else None
-    lbm_kernel, lbm_wg = (("k_lbm_rows<false, %d>" % -tune[0], (cfg.ny // -tune[0]) * M) if tune[0] < 0
-                          else ("k_lbm_fused<false>", cfg.nx // 64 * (cfg.ny // 16) * M // tune[0]))
+    if tune[0] <= -100:
+        lbm_kernel, lbm_wg = "k_lbm_rows2<%d>" % (-tune[0] - 100), (cfg.ny // (-tune[0] - 100)) * M
+    elif tune[0] < 0:
+        lbm_kernel, lbm_wg = "k_lbm_rows<false, %d>" % -tune[0], (cfg.ny // -tune[0]) * M
+    else:
+        lbm_kernel, lbm_wg = "k_lbm_fused<false>", cfg.nx // 64 * (cfg.ny // 16) * M // tune[0]
     traffic_lbm = measured_traffic(lbm_kernel, lbm_wg) if head["variant"] == "mixed" else None
     roof = dict(kernel=dom_name, bound=dom.get("bound", "hbm"), achieved=dom.get("achieved"),
                 peak=dom.get("peak"), unit=dom.get("unit"),
@@ -486,7 +500,8 @@ def main():
         "roofline_lbm": dict(kernel=lbm_kernel, bound="hbm", achieved=lbm["achieved"], peak=HBM_PEAK_GBS,
                              unit="GB/s", frac=lbm["achieved"] / HBM_PEAK_GBS, traffic=traffic_lbm,
                              algorithmic_bytes=LBM_BYTES_PER_NODE_UPDATE * M * nodes,
-                             avg_launch_ms=lbm["avg_ms"] / lbm["launches_per_step"]),
+                             steps_per_launch=lbm["steps_per_launch"], avg_launch_ms=lbm["avg_launch_ms"],
+                             ms_per_step=lbm["avg_ms"]),
         "kernels": kern,
     }
     if "plain" in res and "mixed" in res:

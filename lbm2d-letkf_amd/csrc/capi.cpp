@@ -422,22 +422,30 @@ int forecast_launches(Ctx* c, int n_steps) {
     if (mlp && !c->closure_ready) return fail(c, LBMLETKF_ERR_STATE, "closure_kind is MLP: call lbmletkf_set_closure first");
     const bool fused = mlp || (c->cfg.variant == LBMLETKF_VARIANT_MIXED && lbm_fused_supported(c->nx, c->ny));
     hipEvent_t ev = prof_begin(c);
-    for (int s = 0; s < n_steps; ++s) {
+    // the two-step strips take two steps per launch (no forcing: no per-step field)
+    const int two = (fused && !mlp) ? lbm_launch_steps(*c) : 1;
+    long long launches = 0;
+    for (int s = 0; s < n_steps;) {
+        const int k = (two == 2 && s + 2 <= n_steps) ? 2 : 1;
         if (c->lp.has_force) HIP_TRY(c, launch_force_field(*c, s));
         const float* fin = c->f[c->cur];
         float* fout = c->f[1 - c->cur];
         if (mlp) {
             HIP_TRY(c, launch_lbm_mlp(*c, fin, fout, 0, (c->nx / 64) * (c->ny / 16) * c->M, nullptr));
+            ++launches;
         } else if (fused) {
-            HIP_TRY(c, launch_lbm_fused(*c, fin, fout));
+            HIP_TRY(c, launch_lbm_fused(*c, fin, fout, k));
+            ++launches;
         } else {
             for (int m = 0; m < c->M; ++m) HIP_TRY(c, launch_lbm_plain_member(*c, fin, fout, m));
+            launches += 3LL * c->M;
         }
+        if (c->lp.has_force) ++launches;
         c->cur = 1 - c->cur;
+        s += k;
     }
     if (n_steps > 0) HIP_TRY(c, launch_step_advance(*c, n_steps));
-    const long long per_step = (fused ? 1 : 3LL * c->M) + (c->lp.has_force ? 1 : 0);
-    prof_end(c, PH_FORECAST, ev, per_step * n_steps + (n_steps > 0), (long long)n_steps * c->M * c->nodes);
+    prof_end(c, PH_FORECAST, ev, launches + (n_steps > 0), (long long)n_steps * c->M * c->nodes);
     c->step += n_steps;
     return LBMLETKF_OK;
 }
@@ -494,6 +502,7 @@ int cycle_graph(Ctx* c) {
         if (st == LBMLETKF_OK) st = forecast_launches(c, c->cfg.da_interval);
         hipGraph_t g = nullptr;
         const hipError_t ec = hipStreamEndCapture(c->stream, &g);
+        c->cycle_cur_after[k] = c->cur;
         c->cur = cur0;
         c->step = step0;
         if (st != LBMLETKF_OK) {
@@ -508,7 +517,7 @@ int cycle_graph(Ctx* c) {
         c->cycle_exec[k] = x;
     }
     HIP_TRY(c, hipGraphLaunch(c->cycle_exec[k], c->stream));
-    if (c->cfg.da_interval & 1) c->cur = 1 - c->cur;
+    c->cur = c->cycle_cur_after[k];
     c->step += c->cfg.da_interval;
     c->dump_kind = 1;
     return LBMLETKF_OK;
@@ -1437,13 +1446,19 @@ void* lbmletkf_group_get_stream(lbmletkf_group* grp) {
 int lbmletkf_set_tuning(lbmletkf_ctx* ctx, int lbm_tiles_per_block, int mix_chain) {
     Ctx* c = reinterpret_cast<Ctx*>(ctx);
     CHECK_CTX(c);
-    if (lbm_tiles_per_block == 0 || lbm_tiles_per_block > 16 || mix_chain < 1 || mix_chain > 64)
-        return fail(c, LBMLETKF_ERR_ARG, "the LBM shape must be 1..16 tiles or -8/-16/-32 rows, the chain in [1, 64]");
+    const bool two = lbm_tiles_per_block <= -LBM_TWO_STEP;
+    if (lbm_tiles_per_block == 0 || lbm_tiles_per_block > 16 || mix_chain < 1 || mix_chain > 64 ||
+        (lbm_tiles_per_block < -32 && !two))
+        return fail(c, LBMLETKF_ERR_ARG,
+                    "the LBM shape must be 1..16 tiles, -8/-16/-32 rows or -116/-132 two-step rows, the chain in [1, 64]");
     if (lbm_tiles_per_block > 0 && lbm_fused_supported(c->nx, c->ny) &&
         ((long long)(c->nx / 64) * (c->ny / 16) * c->M) % lbm_tiles_per_block)
         return fail(c, LBMLETKF_ERR_ARG, "tiles per block must divide the number of tiles");
-    if (lbm_tiles_per_block < 0 && !lbm_rows_supported(c->nx, c->ny, -lbm_tiles_per_block))
+    if (lbm_tiles_per_block < 0 && !two && !lbm_rows_supported(c->nx, c->ny, -lbm_tiles_per_block))
         return fail(c, LBMLETKF_ERR_ARG, "row strips need nx a multiple of 64 up to 1024 and ny a multiple of the height");
+    if (two && !lbm_rows2_supported(c->nx, c->ny, -lbm_tiles_per_block - LBM_TWO_STEP))
+        return fail(c, LBMLETKF_ERR_ARG,
+                    "two-step row strips need nx a multiple of 64 up to 256 and ny a multiple of the height (16 or 32)");
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     c->drop_graphs();
     c->lbm_tiles_per_block = lbm_tiles_per_block;
@@ -1551,13 +1566,16 @@ int lbmletkf_autotune(lbmletkf_ctx* ctx, const char* cache_path, char* report, i
         const long long tiles = (long long)(c->nx / 64) * (c->ny / 16) * c->M;
         double best = 1e300;
         rep += "lbm us/step:";
-        for (int tpb : {1, 2, 4, -8, -16}) {
+        for (int tpb : {1, 2, 4, -8, -16, -LBM_TWO_STEP - 16, -LBM_TWO_STEP - 32}) {
             if (tpb > 0 && tiles % tpb) continue;
-            if (tpb < 0 && !lbm_rows_supported(c->nx, c->ny, -tpb)) continue;
+            if (tpb < 0 && tpb > -LBM_TWO_STEP && !lbm_rows_supported(c->nx, c->ny, -tpb)) continue;
+            if (tpb <= -LBM_TWO_STEP && !lbm_rows2_supported(c->nx, c->ny, -tpb - LBM_TWO_STEP)) continue;
             c->lbm_tiles_per_block = tpb;
+            const int k = lbm_launch_steps(*c);  // time per step (a launch may take two)
             double us = 0;
-            if (int st = time_launches(c, 2, 10, [&] { return launch_lbm_fused(*c, c->f[c->cur], c->f[1 - c->cur]); }, us))
+            if (int st = time_launches(c, 2, 10, [&] { return launch_lbm_fused(*c, c->f[c->cur], c->f[1 - c->cur], k); }, us))
                 return st;
+            us /= k;
             std::snprintf(buf, sizeof buf, " %d:%.1f", tpb, us);
             rep += buf;
             if (us < best) {

@@ -14,6 +14,7 @@ constexpr int Q = 9;
 constexpr int MAX_MEMBERS = 64;      // one wavefront of lanes per ensemble matrix
 constexpr int MAX_FORCE_MODES = 100; // src/data.cu:258 prunes to 100 modes
 constexpr int MIXED_GROUP_MAX = 8;   // contexts in one group launch
+constexpr int LBM_TWO_STEP = 100;    // shape code offset of the two-step strips
 
 // Per-step LBM constants, in the arithmetic type of the kernels (FP32).
 struct LbmParams {
@@ -113,7 +114,8 @@ struct Ctx {
     int mix_width = 0;
     int mix_chain = 8;            // consecutive gridpoints per wavefront in k_mixed_solve
     // shape of the fused LBM step: k > 0 tiles of 64x16 per workgroup (k_lbm_fused),
-    // -h full-row strips of h rows (k_lbm_rows)
+    // -h full-row strips of h rows (k_lbm_rows), -(LBM_TWO_STEP + h) full-row strips
+    // of h rows advanced two steps per launch (k_lbm_rows2)
     int lbm_tiles_per_block = 1;
 
     // transform dump
@@ -141,6 +143,7 @@ struct Ctx {
     // MIXED cycle (analysis + da_interval steps) as an instantiated hipGraph, one per
     // parity of `cur` (the captured kernels name the current population buffer)
     hipGraphExec_t cycle_exec[2] = {nullptr, nullptr};
+    int cycle_cur_after[2] = {0, 1};  // `cur` after a replay of cycle_exec[k]
     void drop_graphs();
     unsigned graph_gen = 0;       // bumped by drop_graphs: graphs of groups holding this context are stale
 
@@ -156,7 +159,12 @@ int report_error(Ctx* c, int code, const std::string& msg);
 
 // ---- launchers (lbm.hip) ------------------------------------------------------
 hipError_t launch_force_field(Ctx& c, int step_offset);
-hipError_t launch_lbm_fused(Ctx& c, const float* fin, float* fout);
+// one launch of the fused step in the tuned shape, advancing `steps` steps (2 only
+// where lbm_launch_steps(c) == 2)
+hipError_t launch_lbm_fused(Ctx& c, const float* fin, float* fout, int steps = 1);
+// steps one launch of the tuned shape can take: 2 for the two-step strips
+// (lbm_tiles_per_block <= -LBM_TWO_STEP) without forcing, else 1
+int lbm_launch_steps(const Ctx& c);
 // one step of every member of every context in one launch (fused kernel, contexts
 // of fused-supported sizes); each context advances from f[cur] into f[1 - cur]
 hipError_t launch_lbm_fused_group(Ctx* const* cs, int n, hipStream_t stream);
@@ -180,6 +188,8 @@ hipError_t launch_step_advance(Ctx& c, int n);
 bool lbm_fused_supported(int nx, int ny);
 // full-row strips of h rows (the alternative shape of the fused step)
 bool lbm_rows_supported(int nx, int ny, int h);
+// full-row strips of h rows advanced two steps per launch (k_lbm_rows2)
+bool lbm_rows2_supported(int nx, int ny, int h);
 
 // ---- launchers (letkf.hip) ----------------------------------------------------
 hipError_t launch_local_count(Ctx& c, int* d_count);

@@ -188,6 +188,115 @@ __global__ __launch_bounds__(1024) void k_lbm_rows(const float* __restrict__ fin
     }
 }
 
+// Two steps per launch (no body force, no friction): a workgroup of nx threads
+// (one per column, nx <= 256) produces rows [y0, y0+H) of one member at t+2 from
+// the populations at t; the intermediate step never leaves the chip.  Iteration k
+// runs four stages, each on another row and each followed by a barrier:
+//   A (row k):   pull from global (t), moments -> step-1 velocity ring u1/v1
+//   B (row k-1): Smagorinsky from u1/v1, collide -> populations at t+1 -> ring p1
+//   C (row k-2): pull from p1, moments -> step-2 velocity ring u2/v2
+//   D (row k-3): Smagorinsky from u2/v2, collide -> global (t+2)
+// Stage A covers strip rows -3 .. H+2 (it reads rows -4 .. H+3; the outer rows
+// are also read by the neighbouring strips, mostly from L2), B rows -2 .. H+1,
+// C rows -1 .. H and D rows 0 .. H-1.  Every ring holds 4 rows; a slot is
+// rewritten only after the stage reading it has passed two barriers.  Each node
+// runs the arithmetic of the single-step kernels (pull9, moments, smagorinsky,
+// collide9), so two of these steps equal two single steps bit for bit.  HBM
+// traffic: 36 B read + 36 B written per node per launch, i.e. per two steps.
+template <int H>
+__global__ __launch_bounds__(256) void k_lbm_rows2(const float* __restrict__ fin, float* __restrict__ fout,
+                                                   LbmParams p, int strips_per_member) {
+    extern __shared__ float lds2[];  // u1, v1, u2, v2: [4][nx] each; p1: [4][Q][nx]
+    const int nx = p.nx, ny = p.ny;
+    const long long nodes = p.nodes;
+    float* u1 = lds2;
+    float* v1 = u1 + 4 * nx;
+    float* u2 = v1 + 4 * nx;
+    float* v2 = u2 + 4 * nx;
+    float* p1 = v2 + 4 * nx;
+    const int nb = gridDim.x, b = blockIdx.x;
+    const int bb = (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
+    const int member = bb / strips_per_member;
+    const int y0 = (bb - member * strips_per_member) * H;
+    const float* f = fin + (long long)member * p.member_stride;
+    float* fo = fout + (long long)member * p.member_stride;
+    const int i = threadIdx.x;
+    const int im = i == 0 ? nx - 1 : i - 1, ip = i == nx - 1 ? 0 : i + 1;
+    auto sl = [](int r) { return (r + 8) & 3; };  // ring slot of strip row r >= -8
+    auto pull_g = [&](int r, float fc[Q]) {
+        const int j = wrap(y0 + r, ny);
+        pull9(f, nodes, im, i, ip, (long long)wrap(j - 1, ny) * nx, (long long)j * nx,
+              (long long)wrap(j + 1, ny) * nx, fc);
+    };
+    float fA[Q], fB[Q], fC[Q], fD[Q], fP[Q];
+    float rA, uA, vA, rB = 0.f, uB = 0.f, vB = 0.f, rC, uC, vC, rD = 0.f, uD = 0.f, vD = 0.f;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) fB[q] = fD[q] = 0.f;
+    pull_g(-3, fP);
+    for (int k = -3; k <= H + 2; ++k) {
+        // A: row k (the pull was issued one iteration earlier)
+#pragma unroll
+        for (int q = 0; q < Q; ++q) fA[q] = fP[q];
+        if (k < H + 2) pull_g(k + 1, fP);
+        moments(fA, p.c, rA, uA, vA);
+        u1[sl(k) * nx + i] = uA;
+        v1[sl(k) * nx + i] = vA;
+        __syncthreads();
+        // B: row k-1, the populations at t+1
+        if (k >= -1) {
+            const int s = sl(k - 1) * nx, sn = sl(k) * nx, ss = sl(k - 2) * nx;
+            const float nut = smagorinsky(u1[s + ip], u1[s + im], u1[sn + i], u1[ss + i], v1[s + ip],
+                                          v1[s + im], v1[sn + i], v1[ss + i], p);
+            collide9<false>(fB, rB, uB, vB, nut, 0.f, 0.f, p);
+            float* pr = p1 + sl(k - 1) * Q * nx;
+#pragma unroll
+            for (int q = 0; q < Q; ++q) pr[q * nx + i] = fB[q];
+        }
+        __syncthreads();
+        // C: row k-2, pulled from the ring (row above = k-1, below = k-3)
+        if (k >= 1) {
+            const float* pp = p1 + sl(k - 1) * Q * nx;
+            const float* p0 = p1 + sl(k - 2) * Q * nx;
+            const float* pm = p1 + sl(k - 3) * Q * nx;
+            fC[0] = pp[0 * nx + ip];
+            fC[1] = pp[1 * nx + i];
+            fC[2] = pp[2 * nx + im];
+            fC[3] = p0[3 * nx + ip];
+            fC[4] = p0[4 * nx + i];
+            fC[5] = p0[5 * nx + im];
+            fC[6] = pm[6 * nx + ip];
+            fC[7] = pm[7 * nx + i];
+            fC[8] = pm[8 * nx + im];
+            moments(fC, p.c, rC, uC, vC);
+            u2[sl(k - 2) * nx + i] = uC;
+            v2[sl(k - 2) * nx + i] = vC;
+        }
+        __syncthreads();
+        // D: row k-3, the populations at t+2
+        if (k >= 3) {
+            const int s = sl(k - 3) * nx, sn = sl(k - 2) * nx, ss = sl(k - 4) * nx;
+            const float nut = smagorinsky(u2[s + ip], u2[s + im], u2[sn + i], u2[ss + i], v2[s + ip],
+                                          v2[s + im], v2[sn + i], v2[ss + i], p);
+            collide9<false>(fD, rD, uD, vD, nut, 0.f, 0.f, p);
+            const long long node = (long long)(y0 + k - 3) * nx + i;
+#pragma unroll
+            for (int q = 0; q < Q; ++q) fo[q * nodes + node] = fD[q];
+        }
+        // the next iteration's B takes this A row, its D this C row
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            fB[q] = fA[q];
+            fD[q] = fC[q];
+        }
+        rB = rA;
+        uB = uA;
+        vB = vA;
+        rD = rC;
+        uD = uC;
+        vD = vC;
+    }
+}
+
 // One step of several contexts: each task owns a range of workgroups (one tile
 // each, the range padded to a multiple of 8 for the XCD-contiguous order).
 struct LbmTask {
@@ -482,26 +591,54 @@ bool lbm_rows_supported(int nx, int ny, int h) {
     return nx % 64 == 0 && nx <= 1024 && (h == 8 || h == 16 || h == 32) && ny % h == 0 && ny >= 2;
 }
 
-hipError_t launch_lbm_fused(Ctx& c, const float* fin, float* fout) {
-    const bool force = c.lp.has_force || c.lp.fric_over_dt != 0.f;
-    if (c.lbm_tiles_per_block < 0) {
-        // full-row strips of -lbm_tiles_per_block rows
-        const int h = -c.lbm_tiles_per_block;
-        if (!lbm_rows_supported(c.nx, c.ny, h)) return hipErrorInvalidValue;
-        const int strips = c.ny / h;
-        const unsigned nb = unsigned(strips * c.M);
-        const size_t lds = sizeof(float) * 8 * size_t(c.nx);
+// one step in full-row strips of h rows
+static hipError_t launch_rows1(Ctx& c, const float* fin, float* fout, int h, bool force) {
+    if (!lbm_rows_supported(c.nx, c.ny, h)) return hipErrorInvalidValue;
+    const int strips = c.ny / h;
+    const unsigned nb = unsigned(strips * c.M);
+    const size_t lds = sizeof(float) * 8 * size_t(c.nx);
 #define LBM_ROWS(HH)                                                                                       \
     (force ? k_lbm_rows<true, HH><<<nb, c.nx, lds, c.stream>>>(fin, fout, c.force_xy, c.lp, strips)      \
            : k_lbm_rows<false, HH><<<nb, c.nx, lds, c.stream>>>(fin, fout, c.force_xy, c.lp, strips))
-        switch (h) {
-            case 8: LBM_ROWS(8); break;
-            case 16: LBM_ROWS(16); break;
-            default: LBM_ROWS(32); break;
-        }
-#undef LBM_ROWS
-        return hipGetLastError();
+    switch (h) {
+        case 8: LBM_ROWS(8); break;
+        case 16: LBM_ROWS(16); break;
+        default: LBM_ROWS(32); break;
     }
+#undef LBM_ROWS
+    return hipGetLastError();
+}
+
+bool lbm_rows2_supported(int nx, int ny, int h) {
+    return nx % 64 == 0 && nx <= 256 && (h == 16 || h == 32) && ny % h == 0 && ny >= 2;
+}
+
+int lbm_launch_steps(const Ctx& c) {
+    const bool force = c.lp.has_force || c.lp.fric_over_dt != 0.f;
+    return (c.lbm_tiles_per_block <= -LBM_TWO_STEP && !force) ? 2 : 1;
+}
+
+hipError_t launch_lbm_fused(Ctx& c, const float* fin, float* fout, int steps) {
+    const bool force = c.lp.has_force || c.lp.fric_over_dt != 0.f;
+    if (steps != 1 && !(steps == 2 && lbm_launch_steps(c) == 2)) return hipErrorInvalidValue;
+    if (c.lbm_tiles_per_block <= -LBM_TWO_STEP) {
+        // full-row strips of h rows, two steps per launch (or one, for a remaining
+        // odd step or a forced flow)
+        const int h = -c.lbm_tiles_per_block - LBM_TWO_STEP;
+        if (steps == 2) {
+            if (!lbm_rows2_supported(c.nx, c.ny, h)) return hipErrorInvalidValue;
+            const int strips = c.ny / h;
+            const unsigned nb = unsigned(strips * c.M);
+            const size_t lds = sizeof(float) * (16 + 4 * Q) * size_t(c.nx);
+            if (h == 16)
+                k_lbm_rows2<16><<<nb, c.nx, lds, c.stream>>>(fin, fout, c.lp, strips);
+            else
+                k_lbm_rows2<32><<<nb, c.nx, lds, c.stream>>>(fin, fout, c.lp, strips);
+            return hipGetLastError();
+        }
+        return launch_rows1(c, fin, fout, h, force);
+    }
+    if (c.lbm_tiles_per_block < 0) return launch_rows1(c, fin, fout, -c.lbm_tiles_per_block, force);
     const int tiles_x = c.nx / TX, tiles_y = c.ny / TY;
     const int tiles = tiles_x * tiles_y;
     const int total = tiles * c.M;

@@ -8,6 +8,12 @@ namespace lbmnode {
 
 constexpr float W0 = 4.f / 9.f, W1 = 1.f / 9.f, W2 = 1.f / 36.f;
 
+// Multiply-adds are fused only inside one statement in the per-node functions, so
+// their rounding does not depend on the kernel they are inlined into: every shape
+// of the step (tiles, row strips, two-step strips, the learned-closure step's
+// fallback) gives the same bits for the same node.
+#define LBMNODE_CONTRACT_PER_STATEMENT _Pragma("clang fp contract(on)")
+
 __device__ __forceinline__ int wrap(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
 
 // Pull streaming of the 9 populations into node (i,j): fc[q] = f[q](i-ex_q, j-ey_q)
@@ -29,6 +35,7 @@ __device__ __forceinline__ void pull9(const float* __restrict__ f, long long nod
 
 // rho, u = c sum f ex / rho, v = c sum f ey / rho   (src/lbm.h:72-109)
 __device__ __forceinline__ void moments(const float fc[Q], float c, float& rho, float& u, float& v) {
+    LBMNODE_CONTRACT_PER_STATEMENT
     float r = 0.f;
 #pragma unroll
     for (int q = 0; q < Q; ++q) r += fc[q];
@@ -42,6 +49,7 @@ __device__ __forceinline__ void moments(const float fc[Q], float c, float& rho, 
 // nu_t = Cs^2 dx^2 sqrt(2 S:S) from central differences (src/lbm.cu:147-171).
 __device__ __forceinline__ float smagorinsky(float uE, float uW, float uN, float uS, float vE,
                                              float vW, float vN, float vS, const LbmParams& p) {
+    LBMNODE_CONTRACT_PER_STATEMENT
     const float ux = (uE - uW) * p.inv_2dx;
     const float uy = (uN - uS) * p.inv_2dx;
     const float vx = (vE - vW) * p.inv_2dx;
@@ -57,6 +65,7 @@ __device__ __forceinline__ float smagorinsky(float uE, float uW, float uN, float
 // and step; the subtraction keeps sum_q feq = rho up to unbiased rounding.
 __device__ __forceinline__ void equilibrium9(float rho, float u, float v, const LbmParams& p,
                                              float feq[Q]) {
+    LBMNODE_CONTRACT_PER_STATEMENT
     const float uu = (u * u + v * v) * p.inv_c2;
     float moving = 0.f;
 #pragma unroll
@@ -77,6 +86,7 @@ __device__ __forceinline__ void equilibrium9(float rho, float u, float v, const 
 template <bool FORCE>
 __device__ __forceinline__ void collide9(float fc[Q], float rho, float u, float v, float nut,
                                          float Fx, float Fy, const LbmParams& p) {
+    LBMNODE_CONTRACT_PER_STATEMENT
     const float tau = 0.5f + p.k_tau * (p.nu + nut);
     const float omega = 1.f / tau;
     const float fxp = (Fx - p.fric_over_dt * u) * p.dt;

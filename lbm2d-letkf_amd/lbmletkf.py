@@ -320,6 +320,8 @@ class Context:
         return sec.value, n.value, w.value
 
     def set_tuning(self, lbm_tiles_per_block: int, mix_chain: int):
+        """LBM shape (k > 0 tiles per workgroup, -h row strips, -(100 + h) two-step
+        row strips) and FP32 solve chain length; see lbmletkf_set_tuning."""
         self._ck(lib.lbmletkf_set_tuning(self.h, lbm_tiles_per_block, mix_chain))
 
     def get_tuning(self):

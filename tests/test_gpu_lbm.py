@@ -131,6 +131,44 @@ def test_full_size_batched_step(lb, orc):
     assert _rel(f3, f0) < TOL_F
 
 
+@pytest.mark.parametrize("shape", [-116, -132])
+def test_two_step_strips_are_bit_identical(lb, orc, shape):
+    """Two steps per launch (no body force, no friction): the same per-node
+    operations as single steps, so 7 steps (three two-step launches and one single
+    step) give the single steps' state bit for bit, and the oracle's to TOL_F."""
+    cfg = small_cfg(lb, "baseline", n=128)
+    cfg.variant = lb.VARIANT_MIXED
+    M = 3
+    print(f"case: two-step strips {shape} 128x128 M={M}", flush=True)
+    out = []
+    for sh in (1, -16, shape):
+        with _ensemble(lb, cfg, M) as ctx:
+            ctx.set_tuning(sh, 8)
+            f0 = ctx.get_all_states().astype(np.float64)
+            ctx.forecast(7)
+            out.append(ctx.get_all_states())
+    np.testing.assert_array_equal(out[1], out[0])  # tiles and one-step strips
+    np.testing.assert_array_equal(out[2], out[0])
+    f = f0
+    for s in range(7):
+        f = orc.lbm_step(cfg, f, s)
+    assert _rel(out[1], f) < TOL_F
+
+
+def test_two_step_strips_with_forcing_fall_back(lb):
+    """A forced flow (ref preset) steps one step per launch in the two-step shape:
+    the state equals the one-step strips' bit for bit."""
+    cfg = small_cfg(lb, "ref", n=128)
+    cfg.variant = lb.VARIANT_MIXED
+    out = []
+    for sh in (-16, -116):
+        with _ensemble(lb, cfg, 2) as ctx:
+            ctx.set_tuning(sh, 8)
+            ctx.forecast(4)
+            out.append(ctx.get_all_states())
+    np.testing.assert_array_equal(out[1], out[0])
+
+
 @pytest.mark.parametrize("shape", [1, 2, -8, -16, -32])
 def test_launch_shapes_are_bit_identical(lb, orc, shape):
     """Every shape of the fused step (tile bands and full-row strips) computes the

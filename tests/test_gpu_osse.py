@@ -117,7 +117,8 @@ def test_graph_cycle_matches_eager(lb, hipbuf):
     # cycles launched eagerly (phase timing on): identical kernels, identical bits.
     # Replays are separated by host synchronisation and state reads; an odd interval
     # makes the two parities of the double buffer alternate, and set_dump in between
-    # forces a fresh capture.
+    # forces a fresh capture.  The same runs with the two-step strips (three two-step
+    # launches and one single step per interval of 7) give the same bits.
     cfg = small_cfg(lb, "baseline", n=64, obs_stride=4, loc_scale=6.0, loc_cutoff=14.0, da_interval=7)
     M = 16
     nat = lb.Context(cfg, 1)
@@ -129,10 +130,12 @@ def test_graph_cycle_matches_eager(lb, hipbuf):
     nat.close()
     yo_dev = [hipbuf.from_host(y) for y in obs]
     runs = []
-    for timed in (False, True):
+    for shape, timed in ((1, False), (1, True), (-116, False), (-116, True)):
+        print(f"case: shape {shape}, {'eager' if timed else 'graph'}", flush=True)
         ens = lb.Context(cfg, M)
         for k in range(M):
             ens.init_member(k, 1000 + k)
+        ens.set_tuning(shape, 8)
         ens.profile_enable(timed)
         states, dumps = [], []
         for c in range(6):
@@ -146,12 +149,13 @@ def test_graph_cycle_matches_eager(lb, hipbuf):
         assert ens.get_step() == 6 * cfg.da_interval
         ens.close()
         runs.append((states, dumps))
-    for c in range(6):
-        print(f"cycle {c}: graph vs eager max |diff| {np.abs(runs[0][0][c] - runs[1][0][c]).max():.3e}")
-        np.testing.assert_array_equal(runs[0][0][c], runs[1][0][c])
-    for a, b in zip(runs[0][1], runs[1][1]):
-        for x, y in zip(a, b):
-            np.testing.assert_array_equal(x, y)
+    for r in range(1, len(runs)):
+        for c in range(6):
+            print(f"run {r} cycle {c}: max |diff| to run 0 {np.abs(runs[0][0][c] - runs[r][0][c]).max():.3e}")
+            np.testing.assert_array_equal(runs[0][0][c], runs[r][0][c])
+        for a, b in zip(runs[0][1], runs[r][1]):
+            for x, y in zip(a, b):
+                np.testing.assert_array_equal(x, y)
 
 
 def test_plain_cycle_matches_analysis_then_forecast(lb, hipbuf):

@@ -58,3 +58,8 @@ def test_set_tuning_rejects(lb):
             ctx.set_tuning(-12, 2)  # strip heights are 8, 16 or 32
         ctx.set_tuning(-16, 2)
         assert ctx.get_tuning() == (-16, 2)
+        ctx.set_tuning(-116, 2)  # two-step strips of 16 rows
+        assert ctx.get_tuning() == (-116, 2)
+        for bad in (-108, -100, -50):  # two-step strips are 16 or 32 rows
+            with pytest.raises(lb.LbmLetkfError):
+                ctx.set_tuning(bad, 2)
