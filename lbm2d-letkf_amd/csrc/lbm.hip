@@ -199,14 +199,19 @@ __global__ __launch_bounds__(1024) void k_lbm_rows(const float* __restrict__ fin
 // Stage A covers strip rows -3 .. H+2 (it reads rows -4 .. H+3; the outer rows
 // are also read by the neighbouring strips, mostly from L2), B rows -2 .. H+1,
 // C rows -1 .. H and D rows 0 .. H-1.  Every ring holds 4 rows; a slot is
-// rewritten only after the stage reading it has passed two barriers.  Each node
-// runs the arithmetic of the single-step kernels (pull9, moments, smagorinsky,
-// collide9), so two of these steps equal two single steps bit for bit.  HBM
-// traffic: 36 B read + 36 B written per node per launch, i.e. per two steps.
+// rewritten only after the stage reading it has passed two barriers.  The ring p1
+// holds the six populations that stream across columns (ex != 0); the three with
+// ex = 0 (q = 1, 4, 7) stream within the column, so each thread keeps its own in
+// registers (one and two rows back).  Each node runs the arithmetic of the
+// single-step kernels (pull9, moments, smagorinsky, collide9), so two of these
+// steps equal two single steps bit for bit.  HBM traffic: 36 B read + 36 B
+// written per node per launch, i.e. per two steps.
+constexpr int P1Q = 6;  // populations in the ring: q = 0, 2, 3, 5, 6, 8
+__device__ __forceinline__ constexpr int p1_slot(int q) { return (q / 3) * 2 + (q % 3 == 2 ? 1 : 0); }
 template <int H>
 __global__ __launch_bounds__(256) void k_lbm_rows2(const float* __restrict__ fin, float* __restrict__ fout,
                                                    LbmParams p, int strips_per_member) {
-    extern __shared__ float lds2[];  // u1, v1, u2, v2: [4][nx] each; p1: [4][Q][nx]
+    extern __shared__ float lds2[];  // u1, v1, u2, v2: [4][nx] each; p1: [4][P1Q][nx]
     const int nx = p.nx, ny = p.ny;
     const long long nodes = p.nodes;
     float* u1 = lds2;
@@ -230,6 +235,7 @@ __global__ __launch_bounds__(256) void k_lbm_rows2(const float* __restrict__ fin
     };
     float fA[Q], fB[Q], fC[Q], fD[Q], fP[Q];
     float rA, uA, vA, rB = 0.f, uB = 0.f, vB = 0.f, rC, uC, vC, rD = 0.f, uD = 0.f, vD = 0.f;
+    float q4m1 = 0.f, q7m1 = 0.f, q7m2 = 0.f;  // f1[4] of row k-2, f1[7] of rows k-2, k-3
 #pragma unroll
     for (int q = 0; q < Q; ++q) fB[q] = fD[q] = 0.f;
     pull_g(-3, fP);
@@ -248,29 +254,34 @@ __global__ __launch_bounds__(256) void k_lbm_rows2(const float* __restrict__ fin
             const float nut = smagorinsky(u1[s + ip], u1[s + im], u1[sn + i], u1[ss + i], v1[s + ip],
                                           v1[s + im], v1[sn + i], v1[ss + i], p);
             collide9<false>(fB, rB, uB, vB, nut, 0.f, 0.f, p);
-            float* pr = p1 + sl(k - 1) * Q * nx;
+            float* pr = p1 + sl(k - 1) * P1Q * nx;
 #pragma unroll
-            for (int q = 0; q < Q; ++q) pr[q * nx + i] = fB[q];
+            for (int q = 0; q < Q; ++q)
+                if (q % 3 != 1) pr[p1_slot(q) * nx + i] = fB[q];
         }
         __syncthreads();
-        // C: row k-2, pulled from the ring (row above = k-1, below = k-3)
+        // C: row k-2, pulled from the ring (row above = k-1, below = k-3) and, for
+        // q = 1, 4, 7, from this thread's own rows
         if (k >= 1) {
-            const float* pp = p1 + sl(k - 1) * Q * nx;
-            const float* p0 = p1 + sl(k - 2) * Q * nx;
-            const float* pm = p1 + sl(k - 3) * Q * nx;
-            fC[0] = pp[0 * nx + ip];
-            fC[1] = pp[1 * nx + i];
-            fC[2] = pp[2 * nx + im];
-            fC[3] = p0[3 * nx + ip];
-            fC[4] = p0[4 * nx + i];
-            fC[5] = p0[5 * nx + im];
-            fC[6] = pm[6 * nx + ip];
-            fC[7] = pm[7 * nx + i];
-            fC[8] = pm[8 * nx + im];
+            const float* pp = p1 + sl(k - 1) * P1Q * nx;
+            const float* p0 = p1 + sl(k - 2) * P1Q * nx;
+            const float* pm = p1 + sl(k - 3) * P1Q * nx;
+            fC[0] = pp[p1_slot(0) * nx + ip];
+            fC[1] = fB[1];
+            fC[2] = pp[p1_slot(2) * nx + im];
+            fC[3] = p0[p1_slot(3) * nx + ip];
+            fC[4] = q4m1;
+            fC[5] = p0[p1_slot(5) * nx + im];
+            fC[6] = pm[p1_slot(6) * nx + ip];
+            fC[7] = q7m2;
+            fC[8] = pm[p1_slot(8) * nx + im];
             moments(fC, p.c, rC, uC, vC);
             u2[sl(k - 2) * nx + i] = uC;
             v2[sl(k - 2) * nx + i] = vC;
         }
+        q7m2 = q7m1;
+        q7m1 = fB[7];
+        q4m1 = fB[4];
         __syncthreads();
         // D: row k-3, the populations at t+2
         if (k >= 3) {
@@ -629,7 +640,7 @@ hipError_t launch_lbm_fused(Ctx& c, const float* fin, float* fout, int steps) {
             if (!lbm_rows2_supported(c.nx, c.ny, h)) return hipErrorInvalidValue;
             const int strips = c.ny / h;
             const unsigned nb = unsigned(strips * c.M);
-            const size_t lds = sizeof(float) * (16 + 4 * Q) * size_t(c.nx);
+            const size_t lds = sizeof(float) * (16 + 4 * P1Q) * size_t(c.nx);
             if (h == 16)
                 k_lbm_rows2<16><<<nb, c.nx, lds, c.stream>>>(fin, fout, c.lp, strips);
             else

@@ -33,7 +33,8 @@ __device__ __forceinline__ void pull9(const float* __restrict__ f, long long nod
     fc[8] = f[8 * nodes + rm + im];
 }
 
-// rho, u = c sum f ex / rho, v = c sum f ey / rho   (src/lbm.h:72-109)
+// rho, u = c sum f ex / rho, v = c sum f ey / rho   (src/lbm.h:72-109), the
+// quotients through the hardware reciprocal of rho (1 ulp; rho ~ 1)
 __device__ __forceinline__ void moments(const float fc[Q], float c, float& rho, float& u, float& v) {
     LBMNODE_CONTRACT_PER_STATEMENT
     float r = 0.f;
@@ -41,9 +42,10 @@ __device__ __forceinline__ void moments(const float fc[Q], float c, float& rho, 
     for (int q = 0; q < Q; ++q) r += fc[q];
     const float jx = (fc[2] + fc[5] + fc[8]) - (fc[0] + fc[3] + fc[6]);
     const float jy = (fc[6] + fc[7] + fc[8]) - (fc[0] + fc[1] + fc[2]);
+    const float ir = __builtin_amdgcn_rcpf(r);
     rho = r;
-    u = c * jx / r;
-    v = c * jy / r;
+    u = (c * jx) * ir;
+    v = (c * jy) * ir;
 }
 
 // nu_t = Cs^2 dx^2 sqrt(2 S:S) from central differences (src/lbm.cu:147-171).
@@ -55,7 +57,7 @@ __device__ __forceinline__ float smagorinsky(float uE, float uW, float uN, float
     const float vx = (vE - vW) * p.inv_2dx;
     const float vy = (vN - vS) * p.inv_2dx;
     const float s01 = 0.5f * (uy + vx);
-    const float ss = sqrtf(2.f * (ux * ux + 2.f * s01 * s01 + vy * vy));
+    const float ss = __builtin_amdgcn_sqrtf(2.f * (ux * ux + 2.f * s01 * s01 + vy * vy));  // 1 ulp
     return p.les_coef * ss;
 }
 
@@ -88,7 +90,7 @@ __device__ __forceinline__ void collide9(float fc[Q], float rho, float u, float 
                                          float Fx, float Fy, const LbmParams& p) {
     LBMNODE_CONTRACT_PER_STATEMENT
     const float tau = 0.5f + p.k_tau * (p.nu + nut);
-    const float omega = 1.f / tau;
+    const float omega = __builtin_amdgcn_rcpf(tau);  // 1 ulp
     const float fxp = (Fx - p.fric_over_dt * u) * p.dt;
     const float fyp = (Fy - p.fric_over_dt * v) * p.dt;
     float feq[Q];
