@@ -213,7 +213,7 @@ __global__ __launch_bounds__(256) void k_lbm_rows2(const float* __restrict__ fin
                                                    LbmParams p, int strips_per_member) {
     extern __shared__ float lds2[];  // u1, v1, u2, v2: [4][nx] each; p1: [4][P1Q][nx]
     const int nx = p.nx, ny = p.ny;
-    const long long nodes = p.nodes;
+    const int nodes = int(p.nodes);  // offsets within one member fit in 32 bits
     float* u1 = lds2;
     float* v1 = u1 + 4 * nx;
     float* u2 = v1 + 4 * nx;
@@ -228,36 +228,48 @@ __global__ __launch_bounds__(256) void k_lbm_rows2(const float* __restrict__ fin
     const int i = threadIdx.x;
     const int im = i == 0 ? nx - 1 : i - 1, ip = i == nx - 1 ? 0 : i + 1;
     auto sl = [](int r) { return (r + 8) & 3; };  // ring slot of strip row r >= -8
-    auto pull_g = [&](int r, float fc[Q]) {
+    // unsigned 32-bit offsets from the member's base: the loads and stores address
+    // as uniform base + per-lane offset
+    const unsigned un = unsigned(nodes), ui = unsigned(i), uim = unsigned(im), uip = unsigned(ip);
+    auto pull_g = [&](int r, float (&fc)[Q]) {
         const int j = wrap(y0 + r, ny);
-        pull9(f, nodes, im, i, ip, (long long)wrap(j - 1, ny) * nx, (long long)j * nx,
-              (long long)wrap(j + 1, ny) * nx, fc);
+        const unsigned rm = unsigned(wrap(j - 1, ny) * nx), r0 = unsigned(j * nx), rp = unsigned(wrap(j + 1, ny) * nx);
+        fc[0] = f[0 * un + rp + uip];
+        fc[1] = f[1 * un + rp + ui];
+        fc[2] = f[2 * un + rp + uim];
+        fc[3] = f[3 * un + r0 + uip];
+        fc[4] = f[4 * un + r0 + ui];
+        fc[5] = f[5 * un + r0 + uim];
+        fc[6] = f[6 * un + rm + uip];
+        fc[7] = f[7 * un + rm + ui];
+        fc[8] = f[8 * un + rm + uim];
     };
-    float fA[Q], fB[Q], fC[Q], fD[Q], fP[Q];
-    float rA, uA, vA, rB = 0.f, uB = 0.f, vB = 0.f, rC, uC, vC, rD = 0.f, uD = 0.f, vD = 0.f;
     float q4m1 = 0.f, q7m1 = 0.f, q7m2 = 0.f;  // f1[4] of row k-2, f1[7] of rows k-2, k-3
-#pragma unroll
-    for (int q = 0; q < Q; ++q) fB[q] = fD[q] = 0.f;
-    pull_g(-3, fP);
-    for (int k = -3; k <= H + 2; ++k) {
+    float fP[Q];
+    // One iteration.  A writes (fa, ma) for row k; B reads the previous A row (fb,
+    // mb); C writes (fc, mc) for row k-2; D reads the previous C row (fd, md).  The
+    // loop runs iterations in pairs with the two register sets swapped, so no row
+    // is copied from one set to the other.
+    auto iter = [&](int k, float (&fa)[Q], float (&ma)[3], float (&fb)[Q], float (&mb)[3], float (&fc)[Q],
+                    float (&mc)[3], float (&fd)[Q], float (&md)[3]) {
         // A: row k (the pull was issued one iteration earlier)
 #pragma unroll
-        for (int q = 0; q < Q; ++q) fA[q] = fP[q];
+        for (int q = 0; q < Q; ++q) fa[q] = fP[q];
         if (k < H + 2) pull_g(k + 1, fP);
-        moments(fA, p.c, rA, uA, vA);
-        u1[sl(k) * nx + i] = uA;
-        v1[sl(k) * nx + i] = vA;
+        moments(fa, p.c, ma[0], ma[1], ma[2]);
+        u1[sl(k) * nx + i] = ma[1];
+        v1[sl(k) * nx + i] = ma[2];
         __syncthreads();
         // B: row k-1, the populations at t+1
         if (k >= -1) {
             const int s = sl(k - 1) * nx, sn = sl(k) * nx, ss = sl(k - 2) * nx;
             const float nut = smagorinsky(u1[s + ip], u1[s + im], u1[sn + i], u1[ss + i], v1[s + ip],
                                           v1[s + im], v1[sn + i], v1[ss + i], p);
-            collide9<false>(fB, rB, uB, vB, nut, 0.f, 0.f, p);
+            collide9<false>(fb, mb[0], mb[1], mb[2], nut, 0.f, 0.f, p);
             float* pr = p1 + sl(k - 1) * P1Q * nx;
 #pragma unroll
             for (int q = 0; q < Q; ++q)
-                if (q % 3 != 1) pr[p1_slot(q) * nx + i] = fB[q];
+                if (q % 3 != 1) pr[p1_slot(q) * nx + i] = fb[q];
         }
         __syncthreads();
         // C: row k-2, pulled from the ring (row above = k-1, below = k-3) and, for
@@ -266,45 +278,42 @@ __global__ __launch_bounds__(256) void k_lbm_rows2(const float* __restrict__ fin
             const float* pp = p1 + sl(k - 1) * P1Q * nx;
             const float* p0 = p1 + sl(k - 2) * P1Q * nx;
             const float* pm = p1 + sl(k - 3) * P1Q * nx;
-            fC[0] = pp[p1_slot(0) * nx + ip];
-            fC[1] = fB[1];
-            fC[2] = pp[p1_slot(2) * nx + im];
-            fC[3] = p0[p1_slot(3) * nx + ip];
-            fC[4] = q4m1;
-            fC[5] = p0[p1_slot(5) * nx + im];
-            fC[6] = pm[p1_slot(6) * nx + ip];
-            fC[7] = q7m2;
-            fC[8] = pm[p1_slot(8) * nx + im];
-            moments(fC, p.c, rC, uC, vC);
-            u2[sl(k - 2) * nx + i] = uC;
-            v2[sl(k - 2) * nx + i] = vC;
+            fc[0] = pp[p1_slot(0) * nx + ip];
+            fc[1] = fb[1];
+            fc[2] = pp[p1_slot(2) * nx + im];
+            fc[3] = p0[p1_slot(3) * nx + ip];
+            fc[4] = q4m1;
+            fc[5] = p0[p1_slot(5) * nx + im];
+            fc[6] = pm[p1_slot(6) * nx + ip];
+            fc[7] = q7m2;
+            fc[8] = pm[p1_slot(8) * nx + im];
+            moments(fc, p.c, mc[0], mc[1], mc[2]);
+            u2[sl(k - 2) * nx + i] = mc[1];
+            v2[sl(k - 2) * nx + i] = mc[2];
         }
         q7m2 = q7m1;
-        q7m1 = fB[7];
-        q4m1 = fB[4];
+        q7m1 = fb[7];
+        q4m1 = fb[4];
         __syncthreads();
         // D: row k-3, the populations at t+2
         if (k >= 3) {
             const int s = sl(k - 3) * nx, sn = sl(k - 2) * nx, ss = sl(k - 4) * nx;
             const float nut = smagorinsky(u2[s + ip], u2[s + im], u2[sn + i], u2[ss + i], v2[s + ip],
                                           v2[s + im], v2[sn + i], v2[ss + i], p);
-            collide9<false>(fD, rD, uD, vD, nut, 0.f, 0.f, p);
-            const long long node = (long long)(y0 + k - 3) * nx + i;
+            collide9<false>(fd, md[0], md[1], md[2], nut, 0.f, 0.f, p);
+            const unsigned node = unsigned((y0 + k - 3) * nx) + ui;
 #pragma unroll
-            for (int q = 0; q < Q; ++q) fo[q * nodes + node] = fD[q];
+            for (int q = 0; q < Q; ++q) fo[unsigned(q) * un + node] = fd[q];
         }
-        // the next iteration's B takes this A row, its D this C row
+    };
+    float fX[Q], fY[Q], fZ[Q], fW[Q], mX[3], mY[3] = {0.f, 0.f, 0.f}, mZ[3], mW[3] = {0.f, 0.f, 0.f};
 #pragma unroll
-        for (int q = 0; q < Q; ++q) {
-            fB[q] = fA[q];
-            fD[q] = fC[q];
-        }
-        rB = rA;
-        uB = uA;
-        vB = vA;
-        rD = rC;
-        uD = uC;
-        vD = vC;
+    for (int q = 0; q < Q; ++q) fY[q] = fW[q] = 0.f;
+    pull_g(-3, fP);
+    static_assert((H + 6) % 2 == 0, "iterations come in pairs");
+    for (int k = -3; k <= H + 2; k += 2) {
+        iter(k, fX, mX, fY, mY, fZ, mZ, fW, mW);
+        iter(k + 1, fY, mY, fX, mX, fW, mW, fZ, mZ);
     }
 }
 
