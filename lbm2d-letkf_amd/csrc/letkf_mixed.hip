@@ -176,6 +176,11 @@ __device__ inline float hestenes_round(float (&u)[N], float& sc, float& isc, flo
 // kept bit-identical in both lanes.  The pair inner product is the sum of the two
 // lanes' half sums (one exchange); each lane then updates its rows of both
 // columns.  No column moves.
+// The rotation is written from the lane's own view, d' = a_guest - a_own: the
+// lane of Q sees d' = -d and so t' = -t exactly (sign flips are exact, and a tie
+// d = 0 takes t > 0 from P's view), which gives both lanes the coefficients
+//   own:   -t' s_guest / s_own,   guest:  t' s_own / s_guest
+// and the norms a_own - t' g, a_guest + t' g without selecting by role.
 template <int H>
 __device__ inline float split_round(float (&x)[H], float (&y)[H], float& sx, float& ix, float& ax, float& sy,
                                     float& iy, float& ay, bool hi, XorBperm xch) {
@@ -183,15 +188,14 @@ __device__ inline float split_round(float (&x)[H], float (&y)[H], float& sx, flo
 #pragma unroll
     for (int i = 0; i < H; ++i) s4[i & 3] = fmaf(x[i], y[i], s4[i & 3]);
     const float s = (s4[0] + s4[1]) + (s4[2] + s4[3]);
-    const float g = s + xch(s);  // the same sum in both lanes (addition commutes)
-    const float aP = hi ? ay : ax, aQ = hi ? ax : ay;
-    const float sP = hi ? sy : sx, sQ = hi ? sx : sy;
-    const float iP = hi ? iy : ix, iQ = hi ? ix : iy;
-    const float gam = (sP * sQ) * g;
-    float t, c;
-    jacobi_tc(aP, aQ, gam, t, c);
-    const float cP = -t * (sQ * iP), cQ = t * (sP * iQ);
-    const float co = hi ? cQ : cP, cg = hi ? cP : cQ;
+    const float g = s + xch(s);      // the same sum in both lanes (addition commutes)
+    const float gam = (sx * sy) * g;  // (multiplication commutes)
+    const float dp = ay - ax, e = 2.f * gam;
+    const float den = fabsf(dp) + rot_sqrt(fmaf(dp, dp, e * e));
+    const bool pos = (dp > 0.f) | ((dp == 0.f) & !hi);  // no short-circuit branch
+    const float t = den > 0.f ? (pos ? e : -e) * rot_rcp(den) : 0.f;
+    const float c = rot_rsq(fmaf(t, t, 1.f));
+    const float co = -t * (sy * ix), cg = t * (sx * iy);
     // the guest rows' new values go to fresh registers (the move that follows
     // writes them back), the own rows are updated in place
 #pragma unroll
@@ -205,14 +209,13 @@ __device__ inline float split_round(float (&x)[H], float (&y)[H], float& sx, flo
     sy *= c;
     ix *= ic;
     iy *= ic;
-    const float tg = t * gam;
-    const float aPn = aP - tg, aQn = aQ + tg;
-    ax = hi ? aQn : aPn;
-    ay = hi ? aPn : aQn;
-    const float den = aP * aQ;
-    return den > 0.f ? fabsf(gam) * rot_rsq(den) : 0.f;
+    // explicit FMAs: the lane of P forms a_Q as fma(t', g, a_Q) and the lane of Q
+    // as fma(-t', g, a_Q) with its t' = -t, the same rounding
+    const float den2 = ax * ay;
+    ax = fmaf(-t, gam, ax);
+    ay = fmaf(t, gam, ay);
+    return den2 > 0.f ? fabsf(gam) * rot_rsq(den2) : 0.f;
 }
-
 // guest rows move to lane ^ (1 << j): y'[lane] = y[lane ^ (1 << j)]
 template <int H, class X>
 __device__ inline void move_guest(float (&y)[H], float& sy, float& iy, float& ay, X xch) {
