@@ -313,7 +313,7 @@ int lbmletkf_trainer_train(lbmletkf_trainer* tr, int batch, int steps, double lr
  * lbm_tiles_per_block = k > 0: k tiles of 64 x 16 per workgroup; = -h: full-row
  * strips of h rows (h = 8, 16, 32; nx a multiple of 64 up to 1024, one thread per
  * column); = -(100 + h): full-row strips of h rows advanced two steps per launch
- * (h = 16, 32; nx a multiple of 64 up to 256; flows without body force or friction,
+ * (h = 16, 32; nx a multiple of 64 up to 1024; flows without body force or friction,
  * others take one step per launch; bit-identical to single steps) -- and the chain
  * length of the FP32 solve kernel (consecutive gridpoints per wavefront, each
  * warm-started from the previous one's eigenvectors; the result changes by FP32

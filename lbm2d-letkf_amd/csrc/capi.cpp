@@ -1206,9 +1206,10 @@ struct Group {
     std::vector<int> user_pos;  // cs[k] is the caller's context user_pos[k]
     hipStream_t stream = nullptr;
     struct Exec {
-        std::vector<int> cur;
+        std::vector<int> cur;        // parities of the contexts' double buffers at capture
         std::vector<unsigned> gen;
         hipGraphExec_t x;
+        std::vector<int> cur_after;  // their parities after a replay
     };
     std::vector<Exec> execs;
     std::string last_error;
@@ -1276,16 +1277,23 @@ int group_analysis_launches(Group* g) {
     return LBMLETKF_OK;
 }
 
+// Each context steps in its own tuned shape on the group's stream: the launches of
+// the large grids fill the device by themselves, and the tuned shapes (one- or
+// two-step row strips) beat one tile launch over all contexts, which measured the
+// same as the sum of the contexts' one-step launches.
 int group_forecast_launches(Group* g, int n_steps) {
-    for (int s = 0; s < n_steps; ++s) {
-        for (Ctx* c : g->cs)
+    for (Ctx* c : g->cs) {
+        const int two = lbm_launch_steps(*c);
+        for (int s = 0; s < n_steps;) {
+            const int k = (two == 2 && s + 2 <= n_steps) ? 2 : 1;
             if (c->lp.has_force) G_HIP(g, launch_force_field(*c, s));
-        G_HIP(g, launch_lbm_fused_group(g->cs.data(), int(g->cs.size()), g->stream));
-        for (Ctx* c : g->cs) c->cur = 1 - c->cur;
+            G_HIP(g, launch_lbm_fused(*c, c->f[c->cur], c->f[1 - c->cur], k));
+            c->cur = 1 - c->cur;
+            s += k;
+        }
+        if (n_steps > 0) G_HIP(g, launch_step_advance(*c, n_steps));
+        c->step += n_steps;
     }
-    if (n_steps > 0)
-        for (Ctx* c : g->cs) G_HIP(g, launch_step_advance(*c, n_steps));
-    for (Ctx* c : g->cs) c->step += n_steps;
     return LBMLETKF_OK;
 }
 
@@ -1387,13 +1395,17 @@ int lbmletkf_group_cycle(lbmletkf_group* grp, const float* const* yo_device) {
         gen.push_back(c->graph_gen);
     }
     hipGraphExec_t x = nullptr;
+    std::vector<int> cur_after;
     for (auto it = g->execs.begin(); it != g->execs.end();) {
-        if (it->gen != gen) {  // a context changed its buffers since: stale
+        if (it->gen != gen) {  // a context changed its buffers or shapes since: stale
             (void)hipGraphExecDestroy(it->x);
             it = g->execs.erase(it);
             continue;
         }
-        if (it->cur == cur) x = it->x;
+        if (it->cur == cur) {
+            x = it->x;
+            cur_after = it->cur_after;
+        }
         ++it;
     }
     if (!x) {
@@ -1404,7 +1416,9 @@ int lbmletkf_group_cycle(lbmletkf_group* grp, const float* const* yo_device) {
         if (st == LBMLETKF_OK) st = group_forecast_launches(g, steps);
         hipGraph_t gr = nullptr;
         const hipError_t ec = hipStreamEndCapture(g->stream, &gr);
+        cur_after.clear();
         for (size_t k = 0; k < g->cs.size(); ++k) {
+            cur_after.push_back(g->cs[k]->cur);
             g->cs[k]->cur = cur[k];
             g->cs[k]->step = step0[k];
         }
@@ -1416,11 +1430,12 @@ int lbmletkf_group_cycle(lbmletkf_group* grp, const float* const* yo_device) {
         const hipError_t ei = hipGraphInstantiate(&x, gr, nullptr, nullptr, 0);
         (void)hipGraphDestroy(gr);
         G_HIP(g, ei);
-        g->execs.push_back({cur, gen, x});
+        g->execs.push_back({cur, gen, x, cur_after});
     }
     G_HIP(g, hipGraphLaunch(x, g->stream));
-    for (Ctx* c : g->cs) {
-        if (steps & 1) c->cur = 1 - c->cur;
+    for (size_t k = 0; k < g->cs.size(); ++k) {
+        Ctx* c = g->cs[k];
+        c->cur = cur_after[k];
         c->step += steps;
         c->dump_kind = 1;
     }
@@ -1458,7 +1473,7 @@ int lbmletkf_set_tuning(lbmletkf_ctx* ctx, int lbm_tiles_per_block, int mix_chai
         return fail(c, LBMLETKF_ERR_ARG, "row strips need nx a multiple of 64 up to 1024 and ny a multiple of the height");
     if (two && !lbm_rows2_supported(c->nx, c->ny, -lbm_tiles_per_block - LBM_TWO_STEP))
         return fail(c, LBMLETKF_ERR_ARG,
-                    "two-step row strips need nx a multiple of 64 up to 256 and ny a multiple of the height (16 or 32)");
+                    "two-step row strips need nx a multiple of 64 up to 1024 and ny a multiple of the height (16 or 32)");
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     c->drop_graphs();
     c->lbm_tiles_per_block = lbm_tiles_per_block;

@@ -165,9 +165,6 @@ hipError_t launch_lbm_fused(Ctx& c, const float* fin, float* fout, int steps = 1
 // steps one launch of the tuned shape can take: 2 for the two-step strips
 // (lbm_tiles_per_block <= -LBM_TWO_STEP) without forcing, else 1
 int lbm_launch_steps(const Ctx& c);
-// one step of every member of every context in one launch (fused kernel, contexts
-// of fused-supported sizes); each context advances from f[cur] into f[1 - cur]
-hipError_t launch_lbm_fused_group(Ctx* const* cs, int n, hipStream_t stream);
 // the MIXED analysis of several contexts (obs already prepared) in one launch of
 // each of its two kernels, on `stream`
 hipError_t launch_letkf_mixed_group(Ctx* const* cs, int n, hipStream_t stream);

@@ -189,7 +189,7 @@ __global__ __launch_bounds__(1024) void k_lbm_rows(const float* __restrict__ fin
 }
 
 // Two steps per launch (no body force, no friction): a workgroup of nx threads
-// (one per column, nx <= 256) produces rows [y0, y0+H) of one member at t+2 from
+// (one per column, nx <= 1024) produces rows [y0, y0+H) of one member at t+2 from
 // the populations at t; the intermediate step never leaves the chip.  Iteration k
 // runs four stages, each on another row and each followed by a barrier:
 //   A (row k):   pull from global (t), moments -> step-1 velocity ring u1/v1
@@ -209,7 +209,7 @@ __global__ __launch_bounds__(1024) void k_lbm_rows(const float* __restrict__ fin
 constexpr int P1Q = 6;  // populations in the ring: q = 0, 2, 3, 5, 6, 8
 __device__ __forceinline__ constexpr int p1_slot(int q) { return (q / 3) * 2 + (q % 3 == 2 ? 1 : 0); }
 template <int H>
-__global__ __launch_bounds__(256) void k_lbm_rows2(const float* __restrict__ fin, float* __restrict__ fout,
+__global__ __launch_bounds__(1024) void k_lbm_rows2(const float* __restrict__ fin, float* __restrict__ fout,
                                                    LbmParams p, int strips_per_member) {
     extern __shared__ float lds2[];  // u1, v1, u2, v2: [4][nx] each; p1: [4][P1Q][nx]
     const int nx = p.nx, ny = p.ny;
@@ -315,31 +315,6 @@ __global__ __launch_bounds__(256) void k_lbm_rows2(const float* __restrict__ fin
         iter(k, fX, mX, fY, mY, fZ, mZ, fW, mW);
         iter(k + 1, fY, mY, fX, mX, fW, mW, fZ, mZ);
     }
-}
-
-// One step of several contexts: each task owns a range of workgroups (one tile
-// each, the range padded to a multiple of 8 for the XCD-contiguous order).
-struct LbmTask {
-    const float* fin;
-    float* fout;
-    const float* force;
-    LbmParams p;
-    int tiles_x, tiles_per_member, tiles, b0, nb;
-};
-struct LbmGroup {
-    int n;
-    LbmTask t[MIXED_GROUP_MAX];
-};
-
-template <bool FORCE>
-__global__ __launch_bounds__(NT) void k_lbm_fused_group(LbmGroup G) {
-    int k = 0;
-    while (k + 1 < G.n && int(blockIdx.x) >= G.t[k + 1].b0) ++k;
-    const LbmTask& T = G.t[k];
-    const int b = int(blockIdx.x) - T.b0;
-    const int bb = (b % 8) * (T.nb / 8) + b / 8;
-    if (bb >= T.tiles) return;  // padding (uniform per workgroup)
-    lbm_tile<FORCE>(bb, T.fin, T.fout, T.force, T.p, T.tiles_x, T.tiles_per_member);
 }
 
 // ----------------------------------------------------------------------------
@@ -629,8 +604,10 @@ static hipError_t launch_rows1(Ctx& c, const float* fin, float* fout, int h, boo
     return hipGetLastError();
 }
 
+// LDS of the two-step strips: 160 B per column (16 velocity rows + 4 x 6 population
+// rows of 4 B), 160 KB at nx = 1024, the MI355X limit per workgroup
 bool lbm_rows2_supported(int nx, int ny, int h) {
-    return nx % 64 == 0 && nx <= 256 && (h == 16 || h == 32) && ny % h == 0 && ny >= 2;
+    return nx % 64 == 0 && nx <= 1024 && (h == 16 || h == 32) && ny % h == 0 && ny >= 2;
 }
 
 int lbm_launch_steps(const Ctx& c) {
@@ -650,6 +627,12 @@ hipError_t launch_lbm_fused(Ctx& c, const float* fin, float* fout, int steps) {
             const int strips = c.ny / h;
             const unsigned nb = unsigned(strips * c.M);
             const size_t lds = sizeof(float) * (16 + 4 * P1Q) * size_t(c.nx);
+            const void* fn = h == 16 ? reinterpret_cast<const void*>(&k_lbm_rows2<16>)
+                                     : reinterpret_cast<const void*>(&k_lbm_rows2<32>);
+            if (lds > 65536) {  // beyond the default dynamic LDS of a workgroup
+                const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+                if (e != hipSuccess) return e;
+            }
             if (h == 16)
                 k_lbm_rows2<16><<<nb, c.nx, lds, c.stream>>>(fin, fout, c.lp, strips);
             else
@@ -669,36 +652,6 @@ hipError_t launch_lbm_fused(Ctx& c, const float* fin, float* fout, int steps) {
         k_lbm_fused<true><<<nb, NT, 0, c.stream>>>(fin, fout, c.force_xy, c.lp, tiles_x, tiles, per_block);
     else
         k_lbm_fused<false><<<nb, NT, 0, c.stream>>>(fin, fout, c.force_xy, c.lp, tiles_x, tiles, per_block);
-    return hipGetLastError();
-}
-
-hipError_t launch_lbm_fused_group(Ctx* const* cs, int n, hipStream_t stream) {
-    if (n < 1 || n > MIXED_GROUP_MAX) return hipErrorInvalidValue;
-    LbmGroup G{};
-    G.n = n;
-    long long b0 = 0;
-    bool force = false;
-    for (int k = 0; k < n; ++k) {
-        Ctx& c = *cs[k];
-        if (!lbm_fused_supported(c.nx, c.ny)) return hipErrorInvalidValue;
-        LbmTask& T = G.t[k];
-        T.fin = c.f[c.cur];
-        T.fout = c.f[1 - c.cur];
-        T.force = c.force_xy;
-        T.p = c.lp;
-        T.tiles_x = c.nx / TX;
-        T.tiles_per_member = T.tiles_x * (c.ny / TY);
-        T.tiles = T.tiles_per_member * c.M;
-        T.nb = (T.tiles + 7) / 8 * 8;
-        T.b0 = int(b0);
-        b0 += T.nb;
-        force = force || c.lp.has_force || c.lp.fric_over_dt != 0.f;
-    }
-    if (b0 > (1LL << 31) - 1) return hipErrorInvalidValue;
-    if (force)
-        k_lbm_fused_group<true><<<unsigned(b0), NT, 0, stream>>>(G);
-    else
-        k_lbm_fused_group<false><<<unsigned(b0), NT, 0, stream>>>(G);
     return hipGetLastError();
 }
 

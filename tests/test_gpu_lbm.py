@@ -155,6 +155,24 @@ def test_two_step_strips_are_bit_identical(lb, orc, shape):
     assert _rel(out[1], f) < TOL_F
 
 
+@pytest.mark.parametrize("nx", [512, 1024])
+def test_wide_two_step_strips(lb, nx):
+    """Rows of 512 and 1024 nodes: one thread per column, 80 and 160 KB of LDS
+    per workgroup; the same bits as the tile shape over 5 steps."""
+    cfg = small_cfg(lb, "baseline", n=64)
+    cfg.nx, cfg.ny = nx, 64
+    cfg.variant = lb.VARIANT_MIXED
+    print(f"case: two-step strips {nx}x64 M=2", flush=True)
+    out = []
+    for sh in (1, -116, -132):
+        with _ensemble(lb, cfg, 2) as ctx:
+            ctx.set_tuning(sh, 8)
+            ctx.forecast(5)
+            out.append(ctx.get_all_states())
+    np.testing.assert_array_equal(out[1], out[0])
+    np.testing.assert_array_equal(out[2], out[0])
+
+
 def test_two_step_strips_with_forcing_fall_back(lb):
     """A forced flow (ref preset) steps one step per launch in the two-step shape:
     the state equals the one-step strips' bit for bit."""
