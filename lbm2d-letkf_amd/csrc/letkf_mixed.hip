@@ -684,7 +684,7 @@ __device__ __forceinline__ void finish_body(int bx, int nbx, const float* __rest
         }
     }
     // members: row layout in wave 0, lane = member
-    float xk[Q], dxk[Q];
+    float xk[Q];
     const int member = lane < M ? lane : 0;
     const long long base = (long long)member * mp.member_stride + g;
 #pragma unroll
@@ -702,10 +702,9 @@ __device__ __forceinline__ void finish_body(int bx, int nbx, const float* __rest
         if (lane < N) hv[lane] = h_i;
     }
     __syncthreads();
+    if (w == 0 && lane < N) {
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        dxk[q] = (lane < M) ? float(double(xk[q]) - xbar[q]) : 0.f;
-        if (w == 0 && lane < N) dXs[q * N + lane] = dxk[q];
+        for (int q = 0; q < Q; ++q) dXs[q * N + lane] = (lane < M) ? float(double(xk[q]) - xbar[q]) : 0.f;
     }
     __syncthreads();
     // Z'[q][i] = s_i sum_r dX[q][r] V[r][i], each wave over its rows  (column layout)
@@ -723,10 +722,12 @@ __device__ __forceinline__ void finish_body(int bx, int nbx, const float* __rest
 #pragma unroll
         for (int q = 0; q < Q; ++q) Zp[0][q * N + lane] += Zp[1][q * N + lane];
     }
-    // dXa[q][l] = sum_i Z'[q][i] V[l][i],  w_l = sum_i V[l][i] h_i   (row layout, wave 0)
-    double dxa[Q];
+    // dXa[q][l] = sum_i Z'[q][i] V[l][i],  w_l = sum_i V[l][i] h_i   (row layout, lane =
+    // member l): wave 0 forms q = 0..4 and w, wave 1 q = 5..8
+    constexpr int QW0 = 5;
+    double dxa[QW0];
 #pragma unroll
-    for (int q = 0; q < Q; ++q) dxa[q] = 0.0;
+    for (int k = 0; k < QW0; ++k) dxa[k] = 0.0;
     double wl = 0.0;
 #pragma unroll
     for (int c0 = 0; c0 < N; c0 += TQ) {
@@ -736,30 +737,45 @@ __device__ __forceinline__ void finish_body(int bx, int nbx, const float* __rest
             for (int r = 0; r < H; ++r) Tq[(r0 + r) * TS + (lane - c0)] = ud[r];
         }
         __syncthreads();
-        if (w == 0 && lane < N) {
+        if (lane < N) {
+            if (w == 0) {
 #pragma unroll 2
-            for (int ii = 0; ii < TQ; ++ii) {
-                const double vli = Tq[lane * TS + ii];  // V[lane][c0+ii]
-                wl += vli * hv[c0 + ii];
+                for (int ii = 0; ii < TQ; ++ii) {
+                    const double vli = Tq[lane * TS + ii];  // V[lane][c0+ii]
+                    wl += vli * hv[c0 + ii];
 #pragma unroll
-                for (int q = 0; q < Q; ++q) dxa[q] += Zp[0][q * N + c0 + ii] * vli;
+                    for (int k = 0; k < QW0; ++k) dxa[k] += Zp[0][k * N + c0 + ii] * vli;
+                }
+            } else {
+#pragma unroll 2
+                for (int ii = 0; ii < TQ; ++ii) {
+                    const double vli = Tq[lane * TS + ii];
+#pragma unroll
+                    for (int k = 0; k < Q - QW0; ++k) dxa[k] += Zp[0][(QW0 + k) * N + c0 + ii] * vli;
+                }
             }
         }
     }
-    // Xa = xbar + dXa + (dX w) 1^T
-    if (w == 0) {
-        bool bad = false;
+    // w to both waves; Xa = xbar + dXa + (dX w) 1^T, each wave for its q
+    __syncthreads();
+    if (w == 0) xpart[0][lane] = wl;
+    __syncthreads();
+    const double wlv = xpart[0][lane];
+    const int qb = w == 0 ? 0 : QW0, nq = w == 0 ? QW0 : Q - QW0;
+    bool bad = false;
 #pragma unroll
-        for (int q = 0; q < Q; ++q) {
-            const double dxw = wave_sum_d((lane < M) ? double(dxk[q]) * wl : 0.0);
+    for (int k = 0; k < QW0; ++k) {
+        if (k < nq) {  // uniform per wave
+            const int q = qb + k;
+            const double dxw = wave_sum_d((lane < M) ? double(dXs[q * N + lane]) * wlv : 0.0);
             if (lane < M) {
-                const float xa = float(xbar[q] + dxa[q] + dxw);
+                const float xa = float(xbar[q] + dxa[k] + dxw);
                 bad |= !isfinite(xa);
                 f[base + (long long)q * mp.nodes] = xa;
             }
         }
-        if (bad) atomicOr(status, 1);
     }
+    if (bad) atomicOr(status, 1);
 }
 
 template <int N, bool SWEEP64>
