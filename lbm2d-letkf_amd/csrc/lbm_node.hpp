@@ -68,17 +68,24 @@ __device__ __forceinline__ float smagorinsky(float uE, float uW, float uN, float
 __device__ __forceinline__ void equilibrium9(float rho, float u, float v, const LbmParams& p,
                                              float feq[Q]) {
     LBMNODE_CONTRACT_PER_STATEMENT
+    // opposite directions share A = 1 + 4.5 uc^2 - 1.5 uu and differ in the sign of
+    // 3 uc: feq_q = rho w_q (A + 3 uc), feq_q' = rho w_q (A - 3 uc)
     const float uu = (u * u + v * v) * p.inv_c2;
-    float moving = 0.f;
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        if (q == 4) continue;
-        const int ex = q % 3 - 1, ey = q / 3 - 1;
-        const float wq = (ex == 0 || ey == 0) ? W1 : W2;
-        const float uc = (u * float(ex) + v * float(ey)) * p.inv_c;
-        feq[q] = rho * wq * (1.f + 3.f * uc + 4.5f * uc * uc - 1.5f * uu);
-        moving += feq[q];
-    }
+    const float base = 1.f - 1.5f * uu;
+    const float rw1 = rho * W1, rw2 = rho * W2;
+    const float ux = u * p.inv_c, uy = v * p.inv_c;
+    const float upv = ux + uy, umv = ux - uy;
+    const float ax = base + 4.5f * ux * ux, ay = base + 4.5f * uy * uy;
+    const float ap = base + 4.5f * upv * upv, am = base + 4.5f * umv * umv;
+    feq[5] = rw1 * (ax + 3.f * ux);    // (+1,  0)
+    feq[3] = rw1 * (ax - 3.f * ux);    // (-1,  0)
+    feq[7] = rw1 * (ay + 3.f * uy);    // ( 0, +1)
+    feq[1] = rw1 * (ay - 3.f * uy);    // ( 0, -1)
+    feq[8] = rw2 * (ap + 3.f * upv);   // (+1, +1)
+    feq[0] = rw2 * (ap - 3.f * upv);   // (-1, -1)
+    feq[2] = rw2 * (am + 3.f * umv);   // (+1, -1)
+    feq[6] = rw2 * (am - 3.f * umv);   // (-1, +1)
+    const float moving = ((feq[0] + feq[1]) + (feq[2] + feq[3])) + ((feq[5] + feq[6]) + (feq[7] + feq[8]));
     feq[4] = rho - moving;
 }
 
