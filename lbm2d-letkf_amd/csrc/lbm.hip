@@ -198,8 +198,9 @@ __global__ __launch_bounds__(1024) void k_lbm_rows(const float* __restrict__ fin
 //   D (row k-3): Smagorinsky from u2/v2, collide -> global (t+2)
 // Stage A covers strip rows -3 .. H+2 (it reads rows -4 .. H+3; the outer rows
 // are also read by the neighbouring strips, mostly from L2), B rows -2 .. H+1,
-// C rows -1 .. H and D rows 0 .. H-1.  Every ring holds 4 rows; a slot is
-// rewritten only after the stage reading it has passed two barriers.  The ring p1
+// C rows -1 .. H and D rows 0 .. H-1.  Every ring holds 3 rows (a stage reads rows
+// r-1, r, r+1 of its ring); a slot is rewritten only after the stage that read it
+// last has passed two barriers.  The ring p1
 // holds the six populations that stream across columns (ex != 0); the three with
 // ex = 0 (q = 1, 4, 7) stream within the column, so each thread keeps its own in
 // registers (one and two rows back).  Each node runs the arithmetic of the
@@ -211,14 +212,14 @@ __device__ __forceinline__ constexpr int p1_slot(int q) { return (q / 3) * 2 + (
 template <int H>
 __global__ __launch_bounds__(1024) void k_lbm_rows2(const float* __restrict__ fin, float* __restrict__ fout,
                                                    LbmParams p, int strips_per_member) {
-    extern __shared__ float lds2[];  // u1, v1, u2, v2: [4][nx] each; p1: [4][P1Q][nx]
+    extern __shared__ float lds2[];  // u1, v1, u2, v2: [3][nx] each; p1: [3][P1Q][nx]
     const int nx = p.nx, ny = p.ny;
     const int nodes = int(p.nodes);  // offsets within one member fit in 32 bits
     float* u1 = lds2;
-    float* v1 = u1 + 4 * nx;
-    float* u2 = v1 + 4 * nx;
-    float* v2 = u2 + 4 * nx;
-    float* p1 = v2 + 4 * nx;
+    float* v1 = u1 + 3 * nx;
+    float* u2 = v1 + 3 * nx;
+    float* v2 = u2 + 3 * nx;
+    float* p1 = v2 + 3 * nx;
     const int nb = gridDim.x, b = blockIdx.x;
     const int bb = (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
     const int member = bb / strips_per_member;
@@ -227,7 +228,7 @@ __global__ __launch_bounds__(1024) void k_lbm_rows2(const float* __restrict__ fi
     float* fo = fout + (long long)member * p.member_stride;
     const int i = threadIdx.x;
     const int im = i == 0 ? nx - 1 : i - 1, ip = i == nx - 1 ? 0 : i + 1;
-    auto sl = [](int r) { return (r + 8) & 3; };  // ring slot of strip row r >= -8
+    auto sl = [](int r) { return (r + 9) % 3; };  // ring slot of strip row r >= -9
     // unsigned 32-bit offsets from the member's base: the loads and stores address
     // as uniform base + per-lane offset
     const unsigned un = unsigned(nodes), ui = unsigned(i), uim = unsigned(im), uip = unsigned(ip);
@@ -604,8 +605,8 @@ static hipError_t launch_rows1(Ctx& c, const float* fin, float* fout, int h, boo
     return hipGetLastError();
 }
 
-// LDS of the two-step strips: 160 B per column (16 velocity rows + 4 x 6 population
-// rows of 4 B), 160 KB at nx = 1024, the MI355X limit per workgroup
+// LDS of the two-step strips: 120 B per column (12 velocity rows + 3 x 6 population
+// rows of 4 B), 120 KB at nx = 1024 (the MI355X limit per workgroup is 160 KB)
 bool lbm_rows2_supported(int nx, int ny, int h) {
     return nx % 64 == 0 && nx <= 1024 && (h == 16 || h == 32) && ny % h == 0 && ny >= 2;
 }
@@ -626,7 +627,7 @@ hipError_t launch_lbm_fused(Ctx& c, const float* fin, float* fout, int steps) {
             if (!lbm_rows2_supported(c.nx, c.ny, h)) return hipErrorInvalidValue;
             const int strips = c.ny / h;
             const unsigned nb = unsigned(strips * c.M);
-            const size_t lds = sizeof(float) * (16 + 4 * P1Q) * size_t(c.nx);
+            const size_t lds = sizeof(float) * (12 + 3 * P1Q) * size_t(c.nx);
             const void* fn = h == 16 ? reinterpret_cast<const void*>(&k_lbm_rows2<16>)
                                      : reinterpret_cast<const void*>(&k_lbm_rows2<32>);
             if (lds > 65536) {  // beyond the default dynamic LDS of a workgroup
