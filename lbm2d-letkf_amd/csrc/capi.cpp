@@ -1527,7 +1527,7 @@ std::string tune_key(const Ctx* c) {
     return dev + "/" + std::to_string(c->nx) + "x" + std::to_string(c->ny) + "/M" + std::to_string(c->M) + "/v" +
            std::to_string(g.variant) + "/p" + std::to_string(g.gram_passes) + "/f" +
            std::to_string(g.jacobi_fp64_sweeps > 0) + "/s" + std::to_string(c->obs.n_sites) + "/e" +
-           std::to_string(c->obs.n_entries);
+           std::to_string(c->obs.n_entries) + "/t2";  // t2: the candidate set (two-step strips, chain 32)
 }
 
 }  // namespace
@@ -1611,7 +1611,7 @@ int lbmletkf_autotune(lbmletkf_ctx* ctx, const char* cache_path, char* report, i
         HIP_TRY(c, launch_obs_prepare(*c));
         double best = 1e300;
         rep += std::string(rep.empty() ? "" : "; ") + "solve us:";
-        for (int ch : {1, 2, 4, 8, 16}) {
+        for (int ch : {1, 2, 4, 8, 16, 32}) {
             c->mix_chain = ch;
             double us = 0;
             if (int st = time_launches(c, 1, 3, [&] {
