@@ -416,6 +416,59 @@ __device__ inline void gram_to_lds(int g, int lane, const int* __restrict__ loc_
         }
 }
 
+// Warm start: U0 = C V_prev on the FP32 matrix cores (v_mfma_f32_16x16x4f32), where
+// V_prev is the previous gridpoint's U (row-major in the workspace, just written by
+// this wave) with unit columns.  The product takes U_prev as it is and scales column
+// n of the result by 1 / ||u_n|| afterwards.  MFMA lane layout: A[m][k] from lane
+// (m = lane % 16, k = lane / 16), B[k][n] from lane (k = lane / 16, n = lane % 16),
+// D[4 (lane / 16) + r][lane % 16].  Each lane's B values are a quarter of the rows
+// of columns 16 tj + lane % 16, so the column norms come from its squares summed
+// over the four lanes of a column (the same sum in all four).  U0 replaces C in
+// cfull (row-major) after every read of C is done.
+template <int N>
+__device__ inline void warm_start(const float* __restrict__ up, float* cfull, int lane) {
+    constexpr int NT = N / 16;
+    const int col = lane & 15, kq = lane >> 4;
+    f32x4 acc[NT * NT];
+#pragma unroll
+    for (int a = 0; a < NT * NT; ++a) acc[a] = f32x4{0.f, 0.f, 0.f, 0.f};
+    float nrm[NT];
+#pragma unroll
+    for (int tj = 0; tj < NT; ++tj) nrm[tj] = 0.f;
+#pragma unroll
+    for (int ks = 0; ks < N / 4; ++ks) {
+        const int k = 4 * ks + kq;
+        float a[NT], b[NT];
+#pragma unroll
+        for (int ti = 0; ti < NT; ++ti) a[ti] = cfull[k * N + 16 * ti + col];  // C symmetric: no bank conflicts
+#pragma unroll
+        for (int tj = 0; tj < NT; ++tj) {
+            b[tj] = up[k * N + 16 * tj + col];
+            nrm[tj] = fmaf(b[tj], b[tj], nrm[tj]);
+        }
+#pragma unroll
+        for (int ti = 0; ti < NT; ++ti)
+#pragma unroll
+            for (int tj = 0; tj < NT; ++tj)
+                acc[ti * NT + tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[ti], b[tj], acc[ti * NT + tj], 0, 0, 0);
+    }
+    float inv[NT];
+#pragma unroll
+    for (int tj = 0; tj < NT; ++tj) {
+        float nn = nrm[tj] + __shfl_xor(nrm[tj], 16);
+        nn = nn + __shfl_xor(nn, 32);
+        inv[tj] = nn > 0.f ? rot_rsq(nn) : 0.f;
+    }
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int ti = 0; ti < NT; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < NT; ++tj)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                cfull[(16 * ti + 4 * kq + r) * N + 16 * tj + col] = acc[ti * NT + tj][r] * inv[tj];
+}
+
 // ---------------------------------------------------------------------------------
 // kernel 1: gather, MFMA Gram, FP32 sweeps for a chain of mp.chain consecutive
 // gridpoints.  The first starts from U = C; each later one from U = C V_prev, the
@@ -443,35 +496,7 @@ __device__ __forceinline__ void solve_body(int bx, int nbx, const int* __restric
         __syncthreads();
         float u[N];
         if (g > g_begin) {
-            // V_prev: the previous gridpoint's U (written by this lane) with unit columns
-            float v[N];
-            const float* up = ws_u + (long long)(g - 1) * N * N;
-            float n4[4] = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int i = 0; i < N; ++i) {
-                v[i] = (lane < N) ? up[i * N + l] : 0.f;
-                n4[i & 3] = fmaf(v[i], v[i], n4[i & 3]);
-            }
-            const float nn = (n4[0] + n4[1]) + (n4[2] + n4[3]);
-            const float inv = nn > 0.f ? rot_rsq(nn) : 0.f;
-#pragma unroll
-            for (int i = 0; i < N; ++i) v[i] *= inv;
-            // U0 = C V_prev row by row: row i of U0 replaces row i of C, which no later
-            // row reads (each row is read in full, as a broadcast, before it is written)
-            for (int i = 0; i < N; ++i) {
-                const float4* crow = reinterpret_cast<const float4*>(cfull + i * N);
-                float a4[4] = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-                for (int k = 0; k < N / 4; ++k) {
-                    const float4 c4 = crow[k];
-                    a4[0] = fmaf(c4.x, v[4 * k], a4[0]);
-                    a4[1] = fmaf(c4.y, v[4 * k + 1], a4[1]);
-                    a4[2] = fmaf(c4.z, v[4 * k + 2], a4[2]);
-                    a4[3] = fmaf(c4.w, v[4 * k + 3], a4[3]);
-                }
-                __builtin_amdgcn_wave_barrier();
-                if (lane < N) cfull[i * N + lane] = (a4[0] + a4[1]) + (a4[2] + a4[3]);
-            }
+            warm_start<N>(ws_u + (long long)(g - 1) * N * N, cfull, lane);
             __syncthreads();
         }
         // column `lane` (C symmetric, so row and column read alike)
