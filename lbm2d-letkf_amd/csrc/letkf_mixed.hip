@@ -338,25 +338,46 @@ __device__ inline void gram_to_lds(int g, int lane, const int* __restrict__ loc_
     const int P = (e1 - e0) * nv;
     const int col = lane & 15;   // member within a tile (A rows / B columns)
     const int kq = lane >> 4;    // which 8 of the 32 k-values this lane holds
+    // Observation index j of the local set = entry e0 + j / nv, variable j % nv.  A
+    // lane's 8 consecutive k-values step (e, va) without dividing; values beyond P
+    // and members beyond M read a valid element and contribute zero, so the loop has
+    // no branches and all its loads are in flight together.
+    // the R^-1 weights as uniform values (a select between them, not an indexed load)
+    const float cf0 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(mp.coef[0])));
+    const float cf1 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(mp.coef[1])));
+    const float cf2 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(mp.coef[2])));
     for (int base = 0; base < P; base += 32) {
         float x[NT][8], inn[8];
+        const int jj0 = base + 8 * kq;
+        int e = e0 + jj0 / nv, va = jj0 % nv;
+        unsigned oo[8];
+        float sw[8];
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
-            const int jj = base + 8 * kq + t;
+            const bool ok = jj0 + t < P;
+            const int ec = ok ? e : e0, vc = ok ? va : 0;
+            const float cf = vc == 0 ? cf0 : (vc == 1 ? cf1 : cf2);
+            oo[t] = unsigned(loc_site[ec]) * unsigned(nv) + unsigned(vc);
+            const float w = loc_w[ec];
+            sw[t] = __builtin_amdgcn_sqrtf(w * cf) * (ok ? 1.f : 0.f);  // 1 ulp; 0 beyond P
+            const bool wrapv = va + 1 == nv;
+            va = wrapv ? 0 : va + 1;
+            e += wrapv ? 1 : 0;
+        }
+        // masks as factors, so the loads stay unconditional
+        float mmask[NT];
 #pragma unroll
-            for (int I = 0; I < NT; ++I) x[I][t] = 0.f;
-            inn[t] = 0.f;
-            if (jj < P) {
-                const int e = e0 + jj / nv, va = jj % nv;
-                const long long o = (long long)loc_site[e] * nv + va;
-                const float sw = sqrtf(loc_w[e] * mp.coef[va]);
+        for (int I = 0; I < NT; ++I) mmask[I] = (16 * I + col < M) ? 1.f : 0.f;
+        const float cmask = col == 0 ? 1.f : 0.f;
 #pragma unroll
-                for (int I = 0; I < NT; ++I) {
-                    const int mm = 16 * I + col;
-                    x[I][t] = (mm < M) ? dy[o * M + mm] * sw : 0.f;
-                }
-                inn[t] = (col == 0) ? dyo[o] * sw : 0.f;
+        for (int t = 0; t < 8; ++t) {
+#pragma unroll
+            for (int I = 0; I < NT; ++I) {
+                const int mm = 16 * I + col;
+                const float d = dy[(size_t)oo[t] * M + (mm < M ? mm : M - 1)];
+                x[I][t] = d * (sw[t] * mmask[I]);
             }
+            inn[t] = dyo[oo[t]] * (sw[t] * cmask);
         }
         bf16x8 hi[NT + 1], lo[NT + 1];
 #pragma unroll
