@@ -37,9 +37,9 @@ __device__ __forceinline__ void pull9(const float* __restrict__ f, long long nod
 // quotients through the hardware reciprocal of rho (1 ulp; rho ~ 1)
 __device__ __forceinline__ void moments(const float fc[Q], float c, float& rho, float& u, float& v) {
     LBMNODE_CONTRACT_PER_STATEMENT
-    float r = 0.f;
+    float r = fc[0];
 #pragma unroll
-    for (int q = 0; q < Q; ++q) r += fc[q];
+    for (int q = 1; q < Q; ++q) r += fc[q];
     const float jx = (fc[2] + fc[5] + fc[8]) - (fc[0] + fc[3] + fc[6]);
     const float jy = (fc[6] + fc[7] + fc[8]) - (fc[0] + fc[1] + fc[2]);
     const float ir = __builtin_amdgcn_rcpf(r);
