@@ -332,11 +332,12 @@ int lbmletkf_autotune(lbmletkf_ctx* ctx, const char* cache_path, char* report, i
 /* ---- groups: several OSSEs in shared launches ----------------------------------
  * No counterpart in the reference (one OSSE per run).  A group of 1-8 contexts of
  * different grids and ensemble sizes (MIXED variant, grids multiples of 64 x 16,
- * Smagorinsky closure, one gram_passes / FP64-sweep setting) is stepped by ONE
- * LBM launch per step over every member of every context, and analysed by ONE
- * launch of each analysis kernel over every gridpoint of every context; the
- * group cycle (analysis, then da_interval steps, equal in all contexts) is
- * captured as one hipGraph.  The group works on its own stream after draining
+ * one gram_passes / FP64-sweep setting) is analysed by ONE launch of each
+ * analysis kernel over every gridpoint of every context, and stepped on the
+ * group's stream by each context's own tuned LBM launches (a context with the
+ * learned closure by its closure kernel, which needs lbmletkf_set_closure first;
+ * the group calls check that before any launch); the group cycle (analysis,
+ * then da_interval steps, equal in all contexts) is captured as one hipGraph.  The group works on its own stream after draining
  * the contexts' streams; do not use a member context concurrently with a group
  * call.  yo_device[k] is the observation vector (device) of the k-th context as
  * passed to create.  Errors of the group calls read from lbmletkf_last_error(NULL). */

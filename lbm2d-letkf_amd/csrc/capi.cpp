@@ -1251,6 +1251,14 @@ int gfail(Group* g, int code, const std::string& msg) {
         if (e_ != hipSuccess) return gfail((g), LBMLETKF_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+// every context with the learned closure has its network (checked before any launch)
+int group_closures_ready(Group* g) {
+    for (Ctx* c : g->cs)
+        if (c->cfg.closure_kind == LBMLETKF_CLOSURE_MLP && !c->closure_ready)
+            return gfail(g, LBMLETKF_ERR_STATE, "a context with closure_kind = MLP has no network: call lbmletkf_set_closure first");
+    return LBMLETKF_OK;
+}
+
 int group_drain(Group* g) {
     for (Ctx* c : g->cs) G_HIP(g, hipStreamSynchronize(c->stream));
     return LBMLETKF_OK;
@@ -1281,13 +1289,20 @@ int group_analysis_launches(Group* g) {
 // the large grids fill the device by themselves, and the tuned shapes (one- or
 // two-step row strips) beat one tile launch over all contexts, which measured the
 // same as the sum of the contexts' one-step launches.
+// A context with the learned closure steps with the closure kernel, one step per
+// launch, as it does on its own.
 int group_forecast_launches(Group* g, int n_steps) {
     for (Ctx* c : g->cs) {
-        const int two = lbm_launch_steps(*c);
+        const bool mlp = c->cfg.closure_kind == LBMLETKF_CLOSURE_MLP;
+        const int two = mlp ? 1 : lbm_launch_steps(*c);
         for (int s = 0; s < n_steps;) {
             const int k = (two == 2 && s + 2 <= n_steps) ? 2 : 1;
             if (c->lp.has_force) G_HIP(g, launch_force_field(*c, s));
-            G_HIP(g, launch_lbm_fused(*c, c->f[c->cur], c->f[1 - c->cur], k));
+            if (mlp)
+                G_HIP(g, launch_lbm_mlp(*c, c->f[c->cur], c->f[1 - c->cur], 0, (c->nx / 64) * (c->ny / 16) * c->M,
+                                        nullptr));
+            else
+                G_HIP(g, launch_lbm_fused(*c, c->f[c->cur], c->f[1 - c->cur], k));
             c->cur = 1 - c->cur;
             s += k;
         }
@@ -1313,8 +1328,6 @@ int lbmletkf_group_create(lbmletkf_ctx* const* ctxs, int n, lbmletkf_group** out
             if (ctxs[j] == ctxs[k]) return fail(nullptr, LBMLETKF_ERR_ARG, "a context appears twice");
         if (c->cfg.variant != LBMLETKF_VARIANT_MIXED) return fail(nullptr, LBMLETKF_ERR_ARG, "group contexts must use the MIXED variant");
         if (!lbm_fused_supported(c->nx, c->ny)) return fail(nullptr, LBMLETKF_ERR_ARG, "group grids must be multiples of 64 x 16");
-        if (c->cfg.closure_kind != LBMLETKF_CLOSURE_SMAGORINSKY)
-            return fail(nullptr, LBMLETKF_ERR_ARG, "group contexts must use the Smagorinsky closure");
         if (c->M < 2) return fail(nullptr, LBMLETKF_ERR_ARG, "group contexts need at least two members");
         Ctx* c0 = reinterpret_cast<Ctx*>(ctxs[0]);
         if (c->cfg.gram_passes != c0->cfg.gram_passes || (c->cfg.jacobi_fp64_sweeps > 0) != (c0->cfg.jacobi_fp64_sweeps > 0))
@@ -1352,6 +1365,7 @@ int lbmletkf_group_forecast(lbmletkf_group* grp, int n_steps) {
     Group* g = reinterpret_cast<Group*>(grp);
     if (!g) return fail(nullptr, LBMLETKF_ERR_ARG, "group is NULL");
     if (n_steps < 0) return gfail(g, LBMLETKF_ERR_ARG, "n_steps must be >= 0");
+    if (int st = group_closures_ready(g)) return st;
     if (int st = group_drain(g)) return st;
     StreamLend lend(g);
     return group_forecast_launches(g, n_steps);
@@ -1378,6 +1392,7 @@ int lbmletkf_group_cycle(lbmletkf_group* grp, const float* const* yo_device) {
     const int steps = g->cs[0]->cfg.da_interval;
     for (Ctx* c : g->cs)
         if (c->cfg.da_interval != steps) return gfail(g, LBMLETKF_ERR_ARG, "group cycles need one da_interval");
+    if (int st = group_closures_ready(g)) return st;
     if (int st = group_drain(g)) return st;
     if (int st = group_prepare(g, yo_device)) return st;
     StreamLend lend(g);

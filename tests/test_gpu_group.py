@@ -22,11 +22,15 @@ def _cfg(lb, nx, ny, uneven):
     return cfg
 
 
-def _make(lb, uneven):
+def _make(lb, uneven, mlp=False):
     out = []
     for k, (nx, ny, M) in enumerate(SHAPES):
         cfg = _cfg(lb, nx, ny, uneven)
+        if mlp and k == 1:  # one context with the learned closure
+            cfg.closure_kind = lb.CLOSURE_MLP
         ens = lb.Context(cfg, M)
+        if mlp and k == 1:
+            ens.set_closure(lb.closure_init_params(7))
         nat = lb.Context(cfg, 1)
         for m in range(M):
             ens.init_member(m, 100 * k + m)
@@ -43,10 +47,11 @@ def _obs(pairs, counter):
     return [nat.observe(0, seed=3, counter=counter, noise=True) for _, nat in pairs]
 
 
-@pytest.mark.parametrize("uneven", [False, True])
-def test_group_matches_contexts(lb, hipbuf, uneven):
-    print(f"case: group of {len(SHAPES)} contexts, uneven={uneven}", flush=True)
-    a, b = _make(lb, uneven), _make(lb, uneven)  # a: one context at a time; b: the group
+@pytest.mark.parametrize("mode", ["lattice", "uneven", "mlp"])
+def test_group_matches_contexts(lb, hipbuf, mode):
+    uneven, mlp = mode == "uneven", mode == "mlp"
+    print(f"case: group of {len(SHAPES)} contexts, {mode}", flush=True)
+    a, b = _make(lb, uneven, mlp), _make(lb, uneven, mlp)  # a: one context at a time; b: the group
     try:
         # forecast
         for ens, _ in a:
@@ -98,7 +103,12 @@ def test_group_rejects(lb):
         q = cfg.copy()
         q.closure_kind = lb.CLOSURE_MLP
         with lb.Context(q, 4) as c3:
-            with pytest.raises(lb.LbmLetkfError):
-                lb.Group([c1, c3])
+            # a context with the learned closure joins, but steps only with its network
+            with lb.Group([c1, c3]) as grp:
+                s1, s3 = c1.get_all_states(), c3.get_all_states()
+                with pytest.raises(lb.LbmLetkfError, match="set_closure"):
+                    grp.forecast(1)
+                np.testing.assert_array_equal(c1.get_all_states(), s1)  # nothing launched
+                np.testing.assert_array_equal(c3.get_all_states(), s3)
         with pytest.raises(lb.LbmLetkfError):
             lb.Group([])
