@@ -1,0 +1,47 @@
+"""Host logic of bench.py (CPU): the one-GPU contract and the committed HBM-traffic
+summary that the bench line's roofline reads."""
+import importlib.util
+import os
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _bench():
+    spec = importlib.util.spec_from_file_location("bench_host", os.path.join(ROOT, "bench.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "not supported" in r.stderr
+    assert r.stdout == ""  # no JSON line
+
+
+def test_steps_per_launch_of_the_shape_codes():
+    b = _bench()
+    assert b.lbm_steps_per_launch([-116, 32]) == 2
+    assert b.lbm_steps_per_launch([-132, 16]) == 2
+    assert b.lbm_steps_per_launch([-16, 32]) == 1
+    assert b.lbm_steps_per_launch([2, 8]) == 1
+    assert b.lbm_steps_per_launch(None) == 1
+
+
+def test_traffic_summary_holds_the_benchmark_shapes():
+    """The rows the roofline looks up at 256^2 x 64 (solve at chain 32 = 2,048
+    workgroups, two-step strips of 16 rows = 1,024 workgroups) are in the committed
+    summary, with byte counts no smaller than what the kernels must move."""
+    b = _bench()
+    nodes, M = 256 * 256, 64
+    solve = b.measured_traffic("k_mixed_solve<64, 3>", nodes // 32)
+    lbm = b.measured_traffic("k_lbm_rows2<16>", (256 // 16) * M)
+    assert solve is not None and lbm is not None
+    assert solve >= nodes * M * M * 4  # U written: 16 KB per gridpoint
+    algorithmic = b.LBM_BYTES_PER_NODE_UPDATE * nodes * M
+    assert algorithmic <= lbm <= 2 * algorithmic
+    assert b.measured_traffic("k_mixed_solve<64, 3>", 12345) is None
