@@ -1213,6 +1213,10 @@ struct Group {
     };
     std::vector<Exec> execs;
     std::string last_error;
+    // the forecast of cs[k], k >= 1, runs on side[k-1], forked from `stream` by
+    // ev[0] and joined back by ev[k] (default priority, like `stream`)
+    std::vector<hipStream_t> side;
+    std::vector<hipEvent_t> ev;
     void drop() {
         if (!execs.empty() && stream) (void)hipStreamSynchronize(stream);
         for (Exec& e : execs) (void)hipGraphExecDestroy(e.x);
@@ -1220,6 +1224,9 @@ struct Group {
     }
     ~Group() {
         drop();
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (hipStream_t s : side) (void)hipStreamDestroy(s);
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -1285,31 +1292,56 @@ int group_analysis_launches(Group* g) {
     return LBMLETKF_OK;
 }
 
-// Each context steps in its own tuned shape on the group's stream: the launches of
-// the large grids fill the device by themselves, and the tuned shapes (one- or
-// two-step row strips) beat one tile launch over all contexts, which measured the
-// same as the sum of the contexts' one-step launches.
-// A context with the learned closure steps with the closure kernel, one step per
-// launch, as it does on its own.
-int group_forecast_launches(Group* g, int n_steps) {
-    for (Ctx* c : g->cs) {
-        const bool mlp = c->cfg.closure_kind == LBMLETKF_CLOSURE_MLP;
-        const int two = mlp ? 1 : lbm_launch_steps(*c);
-        for (int s = 0; s < n_steps;) {
-            const int k = (two == 2 && s + 2 <= n_steps) ? 2 : 1;
-            if (c->lp.has_force) G_HIP(g, launch_force_field(*c, s));
-            if (mlp)
-                G_HIP(g, launch_lbm_mlp(*c, c->f[c->cur], c->f[1 - c->cur], 0, (c->nx / 64) * (c->ny / 16) * c->M,
-                                        nullptr));
-            else
-                G_HIP(g, launch_lbm_fused(*c, c->f[c->cur], c->f[1 - c->cur], k));
-            c->cur = 1 - c->cur;
-            s += k;
-        }
-        if (n_steps > 0) G_HIP(g, launch_step_advance(*c, n_steps));
-        c->step += n_steps;
+// Each context steps in its own tuned shape: the tuned shapes (one- or two-step row
+// strips) beat one tile launch over all contexts, which measured the same as the
+// sum of the contexts' one-step launches.  A context with the learned closure steps
+// with the closure kernel, one step per launch, as it does on its own.
+int group_context_steps(Group* g, Ctx* c, int n_steps) {
+    const bool mlp = c->cfg.closure_kind == LBMLETKF_CLOSURE_MLP;
+    const int two = mlp ? 1 : lbm_launch_steps(*c);
+    for (int s = 0; s < n_steps;) {
+        const int k = (two == 2 && s + 2 <= n_steps) ? 2 : 1;
+        if (c->lp.has_force) G_HIP(g, launch_force_field(*c, s));
+        if (mlp)
+            G_HIP(g, launch_lbm_mlp(*c, c->f[c->cur], c->f[1 - c->cur], 0, (c->nx / 64) * (c->ny / 16) * c->M,
+                                    nullptr));
+        else
+            G_HIP(g, launch_lbm_fused(*c, c->f[c->cur], c->f[1 - c->cur], k));
+        c->cur = 1 - c->cur;
+        s += k;
     }
+    if (n_steps > 0) G_HIP(g, launch_step_advance(*c, n_steps));
+    c->step += n_steps;
     return LBMLETKF_OK;
+}
+
+// The first context (the widest analysis, launched first) steps on the group's
+// stream; the others step on side streams forked from it and joined back, so the
+// launches of small grids, which do not fill the device, overlap with it.  The
+// same fork and join are captured into the cycle graph as parallel branches.
+int group_forecast_launches(Group* g, int n_steps) {
+    const size_t n = g->cs.size();
+    const bool fork = n > 1 && n_steps > 0;
+    if (fork) {
+        G_HIP(g, hipEventRecord(g->ev[0], g->stream));
+        for (size_t k = 1; k < n; ++k) {
+            G_HIP(g, hipStreamWaitEvent(g->side[k - 1], g->ev[0], 0));
+            g->cs[k]->stream = g->side[k - 1];  // StreamLend restores the context's own stream
+        }
+    }
+    int st = LBMLETKF_OK;
+    for (Ctx* c : g->cs)
+        if ((st = group_context_steps(g, c, n_steps)) != LBMLETKF_OK) break;
+    if (fork) {
+        for (size_t k = 1; k < n; ++k) {
+            g->cs[k]->stream = g->stream;
+            if (st == LBMLETKF_OK) {
+                G_HIP(g, hipEventRecord(g->ev[k], g->side[k - 1]));
+                G_HIP(g, hipStreamWaitEvent(g->stream, g->ev[k], 0));
+            }
+        }
+    }
+    return st;
 }
 
 }  // namespace
@@ -1347,10 +1379,20 @@ int lbmletkf_group_create(lbmletkf_ctx* const* ctxs, int n, lbmletkf_group** out
         g->cs.push_back(reinterpret_cast<Ctx*>(ctxs[k]));
         g->user_pos.push_back(k);
     }
-    const hipError_t e = hipStreamCreate(&g->stream);
+    hipError_t e = hipStreamCreate(&g->stream);
+    for (int k = 1; k < n && e == hipSuccess; ++k) {
+        hipStream_t s = nullptr;
+        e = hipStreamCreate(&s);
+        if (e == hipSuccess) g->side.push_back(s);
+    }
+    for (int k = 0; k < n && e == hipSuccess; ++k) {
+        hipEvent_t ev = nullptr;
+        e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (e == hipSuccess) g->ev.push_back(ev);
+    }
     if (e != hipSuccess) {
         delete g;
-        return fail(nullptr, LBMLETKF_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+        return fail(nullptr, LBMLETKF_ERR_HIP, std::string("group streams and events: ") + hipGetErrorString(e));
     }
     *out = reinterpret_cast<lbmletkf_group*>(g);
     return LBMLETKF_OK;
