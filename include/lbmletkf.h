@@ -35,11 +35,11 @@ extern "C" {
 
 /* forcing kinds */
 #define LBMLETKF_FORCE_NONE          0
-#define LBMLETKF_FORCE_WATANABE97    1   /* src/data.cu:227-350 */
+#define LBMLETKF_FORCE_WATANABE97    1   /* src/data.cu:161-350 */
 /* initial-condition kinds */
 #define LBMLETKF_IC_REFERENCE        0   /* src/data.cu:25-141: mt19937 phases, quadrant modes, max-speed scaling */
 #define LBMLETKF_IC_K4_SPECTRUM      1   /* E(k) ~ k^4 exp(-2 (k/k0)^2), half-plane modes, rms-speed scaling */
-/* observed variables (bit mask) -- src/letkf.cu:125, DA_OBS_RUV src/config.h:44 */
+/* observed variables (bit mask) -- src/letkf.cu:41,183-185, DA_OBS_RUV src/config.h:44 */
 #define LBMLETKF_OBS_U   1
 #define LBMLETKF_OBS_V   2
 #define LBMLETKF_OBS_RHO 4
@@ -66,13 +66,13 @@ typedef struct lbmletkf_config {
     double cs_smag;             /* Smagorinsky constant (literal 0.2 at lbm.cu:59)          */
     double u_ref;               /* velocity scale multiplying e_q in the force term (lbm.h:47) */
     double friction_rate;       /* linear friction per step (config.h:180, lbm.h:52)        */
-    /* stochastic forcing, Watanabe (1997) -- data.cu:227-350 */
+    /* stochastic forcing, Watanabe (1997) -- data.cu:161-350 */
     int    force_kind;
     double force_kf;            /* injected wavenumber (config.h:161)                       */
-    double force_amp_total;     /* fkf * u_ref / t0 (config.h:171, data.cu:284-285)         */
-    double force_sigma_k;       /* Gaussian width in |k| (data.cu:236)                      */
-    double force_dk;            /* clipping half-width in |k| (data.cu:237)                 */
-    double force_rand_stdev;    /* N(0, s^2) draws, s = 2^-1/4 (data.cu:309)                */
+    double force_amp_total;     /* fkf * u_ref / t0 (config.h:171, data.cu:218-223)         */
+    double force_sigma_k;       /* Gaussian width in |k| (data.cu:170)                      */
+    double force_dk;            /* clipping half-width in |k| (data.cu:171)                 */
+    double force_rand_stdev;    /* N(0, s^2) draws, s = 2^-1/4 (data.cu:243)                */
     unsigned long long force_seed;
     /* initial condition -- data.cu:25-141 */
     int    ic_kind;
@@ -92,7 +92,7 @@ typedef struct lbmletkf_config {
     int    da_interval;         /* LBM steps per DA cycle (iiter*DAPRUNE, config.h:136,142) */
     int    variant;             /* LBMLETKF_VARIANT_*                                       */
     double jacobi_tol;          /* relative off-diagonal tolerance of the eigensolver       */
-    int    jacobi_max_sweeps;   /* cusolver syevj uses tol 1e-6, 100 sweeps (eigensolver_wrapper.hpp:192-193) */
+    int    jacobi_max_sweeps;   /* cusolver syevj uses tol 1e-6, 100 sweeps (cu_matrix_eigensolver_wrapper.hpp:192-193) */
     int    gram_passes;         /* MIXED: 1 = bf16 operands, 3 = split bf16 (hi*hi + hi*lo + lo*hi) */
     int    jacobi_fp64_sweeps;  /* MIXED: at most this many FP64 sweeps after the FP32 phase (presets: 0) */
     /* eddy-viscosity closure of the fused LBM kernel (MIXED variant) */
@@ -149,7 +149,7 @@ int lbmletkf_get_macro_device(lbmletkf_ctx* ctx, int member, float* rho, float* 
 int lbmletkf_forecast(lbmletkf_ctx* ctx, int n_steps);
 
 /* Step counter that indexes the forcing random stream (the reference consumes
- * its cuRAND buffer once per update(), src/data.cu:305-313). */
+ * its cuRAND buffer once per update(), src/data.cu:240-253). */
 int lbmletkf_set_step(lbmletkf_ctx* ctx, long long step);
 int lbmletkf_get_step(lbmletkf_ctx* ctx, long long* step);
 
@@ -175,7 +175,7 @@ int lbmletkf_obs_count(lbmletkf_ctx* ctx, int* n_sites, int* n_obs);
 /* Synthetic observations of one member (normally the nature run, member 0 of
  * a one-member context): y = H(x) + sigma*N(0,1), noise from Philox4x32-10 keyed
  * by (seed, counter); noise=0 gives H(x).  Output has n_obs floats; device or host.
- * Replaces the OBSERVE branch of output() (src/io.cu:160-246). */
+ * Replaces the OBSERVE branch of output() (src/io.cu:159-246). */
 int lbmletkf_observe(lbmletkf_ctx* ctx, int member, unsigned long long seed,
                      unsigned long long counter, int noise, float* yo, int yo_on_device);
 
@@ -218,7 +218,7 @@ int lbmletkf_get_dump(lbmletkf_ctx* ctx, double* W, double* w, double* evals);
 
 /* Phase timers on the context's stream (HIP events around each phase), the
  * counterpart of the reference's util::timer tags (src/util/timer.hpp:17-85,
- * src/main.cu:95-115, src/letkf.cu:217-403).  enable(1) clears and starts them.
+ * src/main.cu:95-115, src/letkf.cu:120-360).  enable(1) clears and starts them.
  * Phases: "forecast" (all LBM kernels incl. forcing), "obs" (H(x), innovations),
  * "gram" (PLAIN: C and b), "solve" (PLAIN: eigensolve, transform, state update;
  * MIXED: gather, MFMA Gram and the FP32 Jacobi sweeps), "finish" (MIXED: FP64

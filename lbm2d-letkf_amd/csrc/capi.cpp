@@ -55,7 +55,7 @@ hipError_t dalloc(T** p, size_t n) {
 
 
 // ---------------------------------------------------------------------------
-// forcing modes, src/data.cu:227-290 (kx, ky in [0, kf+dk), kf-dk <= |k| <= kf+dk,
+// forcing modes, src/data.cu:161-223 (kx, ky in [0, kf+dk), kf-dk <= |k| <= kf+dk,
 // amp ~ exp(-(|k|-kf)^2/sigma) normalised so that sum(amp) = fkf*u_ref/t0)
 // ---------------------------------------------------------------------------
 void build_force_table(const lbmletkf_config& cfg, ForceTable& t) {
@@ -90,13 +90,13 @@ void build_force_table(const lbmletkf_config& cfg, ForceTable& t) {
 // initial velocity fields.  Both kinds are the velocity of a random-phase
 // streamfunction psi = sum_k p(k) sin(kx i + ky j + theta_k):
 //   u = sum p ky cos(kx i + ky j + theta),  v = -sum p kx cos(...)
-// (src/data.cu:142-176), evaluated here by two separable passes in FP64.
+// (src/data.cu:76-110), evaluated here by two separable passes in FP64.
 // ---------------------------------------------------------------------------
 struct Mode { int ki, kj; double p, theta; };
 
 void modes_reference(const lbmletkf_config& cfg, unsigned long long seed, int nx,
                      std::vector<Mode>& modes) {
-    // src/data.cu:107-139: std::mt19937(seed), uniform_real_distribution<float>(-1,1),
+    // src/data.cu:41-73: std::mt19937(seed), uniform_real_distribution<float>(-1,1),
     // p(k) = exp(-(k-k0)^2/sigma) on k0-dk <= k <= k0+dk, quadrant 0 <= ki,kj < nx/2
     const int nk = nx / 2;
     std::mt19937 eng(static_cast<std::mt19937::result_type>(seed));
@@ -204,7 +204,7 @@ int check_config(const lbmletkf_config* cfg, int n_members, std::string& why) {
     if (cfg->ic_kind != LBMLETKF_IC_REFERENCE && cfg->ic_kind != LBMLETKF_IC_K4_SPECTRUM)
         return bad("unknown ic_kind");
     if (cfg->obs_stride < 1 || cfg->nx % cfg->obs_stride || cfg->ny % cfg->obs_stride)
-        return bad("obs_stride must divide nx and ny (src/letkf.cu:114)");
+        return bad("obs_stride must divide nx and ny (src/letkf.cu:30)");
     if (cfg->obs_offset < 0 || cfg->obs_offset >= cfg->obs_stride) return bad("obs_offset must be in [0, obs_stride)");
     if (cfg->obs_vars <= 0 || cfg->obs_vars > 7) return bad("obs_vars must be a non-empty mask of u|v|rho");
     if (!(cfg->obs_sigma_u > 0) || !(cfg->obs_sigma_rho > 0)) return bad("observation errors must be positive");
@@ -566,7 +566,7 @@ int lbmletkf_config_preset(const char* name, lbmletkf_config* cfg) {
     lbmletkf_config c{};
     const std::string n(name);
     if (n == "ref") {
-        // src/config.h at SCALER=4, src/lbm.cu:170 (Cs=0.2), src/data.cu:227-350
+        // src/config.h at SCALER=4, src/lbm.cu:59 (Cs=0.2), src/data.cu:161-350
         const double u_ref = 1.0, cfl = 0.05, h_ref = 2 * PI;
         c.nx = c.ny = 64 * 4;
         c.c = u_ref / cfl;
@@ -780,7 +780,7 @@ int lbmletkf_perturb(lbmletkf_ctx* ctx, int member, unsigned long long ensemble_
     CHECK_MEMBER(c, member);
     if (!std::isfinite(epsilon)) return fail(c, LBMLETKF_ERR_ARG, "epsilon must be finite");
     if (int st = ensure_scratch(c)) return st;
-    // src/data.cu:432-443: std::mt19937(id + 132333), normal_distribution<float>(0, 1),
+    // src/data.cu:432-447: std::mt19937(id + 132333), normal_distribution<float>(0, 1),
     // u then v per node, rows j outer, scaled by u_ref
     std::mt19937 eng(uint32_t(ensemble_id + 132333ull));
     std::normal_distribution<float> nd(0.f, 1.f);
@@ -810,7 +810,7 @@ int lbmletkf_init_member(lbmletkf_ctx* ctx, int member, unsigned long long seed)
     const size_t n = size_t(nx) * ny;
     double scale = 0.0;
     if (c->cfg.ic_kind == LBMLETKF_IC_REFERENCE) {
-        // src/data.cu:178-186: scale to max speed p_amp*u_ref
+        // src/data.cu:112-120: scale to max speed p_amp*u_ref
         double vmax = 1e-30 * c->cfg.u_ref;
         for (size_t a = 0; a < n; ++a) vmax = std::max(vmax, std::sqrt(u[a] * u[a] + v[a] * v[a]));
         scale = c->cfg.ic_amp / vmax;

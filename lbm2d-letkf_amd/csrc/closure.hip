@@ -315,7 +315,7 @@ __device__ __forceinline__ float mlp_nu(const bf16x8 (&A1)[4], const bf16x8 (&A2
 // streamed velocity on the tile plus a 1-node ring (LDS), the network on MFMA for
 // 16 nodes at a time (a wave takes 4 rows of 64 nodes), then the BGK collision with
 // that nu_t.  Where the network gives a non-finite value or one outside
-// [0, nu_max] the node falls back to the Smagorinsky nu_t of src/lbm.cu:147-171.
+// [0, nu_max] the node falls back to the Smagorinsky nu_t of src/lbm.cu:24-65.
 template <bool FORCE>
 __global__ __launch_bounds__(MNT) void k_lbm_mlp(const float* __restrict__ fin, float* __restrict__ fout,
                                                   const float* __restrict__ force, LbmParams p,

@@ -12,7 +12,7 @@ namespace lbmletkf {
 
 constexpr int Q = 9;
 constexpr int MAX_MEMBERS = 64;      // one wavefront of lanes per ensemble matrix
-constexpr int MAX_FORCE_MODES = 100; // src/data.cu:258 prunes to 100 modes
+constexpr int MAX_FORCE_MODES = 100; // src/data.cu:191-212 prunes to 100 modes
 constexpr int MIXED_GROUP_MAX = 8;   // contexts in one group launch
 constexpr int LBM_TWO_STEP = 100;    // shape code offset of the two-step strips
 
@@ -22,17 +22,17 @@ struct LbmParams {
     long long nodes;          // nx*ny
     long long member_stride;  // Q*nx*ny floats
     float c, inv_c, inv_c2;   // c, 1/c, 1/c^2
-    float k_tau;              // tau = 0.5 + k_tau*(nu + nu_t), k_tau = 3/(c^2 dt)   (lbm.cu:144-148)
+    float k_tau;              // tau = 0.5 + k_tau*(nu + nu_t), k_tau = 3/(c^2 dt)   (lbm.cu:133-138)
     float nu;
-    float les_coef;           // Cs^2 dx^2                                           (lbm.cu:170-171)
-    float inv_2dx;            // central difference 1/(2 dx)                         (lbm.cu:155-158)
+    float les_coef;           // Cs^2 dx^2                                           (lbm.cu:59-60)
+    float inv_2dx;            // central difference 1/(2 dx)                         (lbm.cu:44-47)
     float force_scale;        // u_ref * 3/c^2: e_q*u_ref/cs2 of force_acc            (lbm.h:47-54)
     float fric_over_dt;       // friction_rate/dt                                   (lbm.h:52-53)
     float dt;
     int has_force;
 };
 
-// Forcing modes of Watanabe (1997) as built by init_forces (src/data.cu:227-290).
+// Forcing modes of Watanabe (1997) as built by init_forces (src/data.cu:161-223).
 struct ForceTable {
     int n = 0;
     float kx[MAX_FORCE_MODES], ky[MAX_FORCE_MODES], amp[MAX_FORCE_MODES];
@@ -44,7 +44,7 @@ struct ObsNet {
     int n_sx = 0, n_sy = 0, n_sites = 0;
     int n_vars = 0;
     int var_id[3] = {0, 0, 0};          // 0=u 1=v 2=rho, in observation order
-    float inv_sigma2[3] = {0, 0, 0};    // R^-1 diagonal per variable (letkf.cu:174-178)
+    float inv_sigma2[3] = {0, 0, 0};    // R^-1 diagonal per variable (letkf.cu:89-94)
     // sites: a lattice (stride, offset) or, when `custom`, an explicit node list
     bool custom = false;
     std::vector<int> site_nodes;        // node of each site (filled by setup for both kinds)

@@ -372,8 +372,8 @@ __global__ void k_collide(float* __restrict__ fn, const float* __restrict__ ruv,
 // ----------------------------------------------------------------------------
 // stochastic forcing of Watanabe (1997): one field per step, shared by all
 // members (the reference seeds every member's generator identically,
-// src/data.cu:293-294).  f_x = sum_n amp_n 2 pi ky_n [(r0 kx + r1 ky) sin(th)
-// + (r2 kx + r3 ky) cos(th)], f_y likewise with -kx   (src/data.cu:383-414)
+// src/data.cu:227-228).  f_x = sum_n amp_n 2 pi ky_n [(r0 kx + r1 ky) sin(th)
+// + (r2 kx + r3 ky) cos(th)], f_y likewise with -kx   (src/data.cu:305-349)
 // ----------------------------------------------------------------------------
 __global__ void k_force_field(const ForceTable* __restrict__ tab, uint64_t seed, float stdev,
                               const long long* __restrict__ d_step, int step_offset,

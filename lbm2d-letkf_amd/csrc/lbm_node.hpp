@@ -17,7 +17,7 @@ constexpr float W0 = 4.f / 9.f, W1 = 1.f / 9.f, W2 = 1.f / 36.f;
 __device__ __forceinline__ int wrap(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
 
 // Pull streaming of the 9 populations into node (i,j): fc[q] = f[q](i-ex_q, j-ey_q)
-// with ex_q = q%3-1, ey_q = q/3-1 (src/config.h:256-258, src/lbm.cu:193-198).
+// with ex_q = q%3-1, ey_q = q/3-1 (src/config.h:256-258, src/lbm.cu:81-89).
 // im/ip are the wrapped columns i-1/i+1, rm/r0/rp the row offsets of j-1/j/j+1.
 __device__ __forceinline__ void pull9(const float* __restrict__ f, long long nodes, int im, int i,
                                       int ip, long long rm, long long r0, long long rp,
@@ -48,7 +48,7 @@ __device__ __forceinline__ void moments(const float fc[Q], float c, float& rho, 
     v = (c * jy) * ir;
 }
 
-// nu_t = Cs^2 dx^2 sqrt(2 S:S) from central differences (src/lbm.cu:147-171).
+// nu_t = Cs^2 dx^2 sqrt(2 S:S) from central differences (src/lbm.cu:36-60).
 __device__ __forceinline__ float smagorinsky(float uE, float uW, float uN, float uS, float vE,
                                              float vW, float vN, float vS, const LbmParams& p) {
     LBMNODE_CONTRACT_PER_STATEMENT

@@ -1,7 +1,7 @@
 // Philox4x32-10 counter-based generator (Salmon, Moraes, Dror, Shaw, "Parallel
 // random numbers: as easy as 1, 2, 3", SC'11), used for every random draw of the
 // library: forcing phases, initial spectra of the K4 kind and observation noise.
-// It replaces cuRAND MTGP32 (src/data.cu:293-311 of the reference), whose stream
+// It replaces cuRAND MTGP32 (src/data.cu:227-247 of the reference), whose stream
 // cannot be reproduced without cuRAND.  Being counter based, the same draw is
 // obtained on the host and on the device from (key, counter) alone.
 #pragma once

@@ -63,7 +63,7 @@ inline void read_field(const std::string& fname, float* data, size_t n) {
 }
 
 // Observation noise of the reference's OBSERVE mode with OBSERVE_ERROR_RUV
-// (src/io.cu:158-174): per node in order rho, u, v, the truth plus sigma times a draw
+// (src/io.cu:165-179): per node in order rho, u, v, the truth plus sigma times a draw
 // of std::normal_distribution<float>(0, 1) on one std::mt19937 engine (seeded
 // 334 + ensemble id, alive for the whole run).  The draws are taken at every node,
 // observed or not.

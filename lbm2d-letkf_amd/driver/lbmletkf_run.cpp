@@ -29,7 +29,7 @@
 //   --loc-scale L           Gaussian localization scale (cells)
 //   --loc-cutoff d          Gaussian localization cutoff (cells)
 //   --inflation b           multiplicative covariance inflation (LETKF_COVINF)
-//   --obs-seed S            seed of the observation noise (default 334, src/io.cu:162)
+//   --obs-seed S            seed of the observation noise (default 334, src/io.cu:166)
 //   --no-nan                keep values at unobserved nodes (default: NaN, OBS_XYPRUNE_NAN)
 //   --stats                 also write the ensemble mean/std files (ENSEMBLE_STAT)
 //   --timing                print the phase timers at the end
@@ -189,7 +189,7 @@ void write_fields(const std::string& dir, long long step, const Fields& f) {
 }
 
 // Observed fields of the reference's OBSERVE mode with OBSERVE_ERROR_RUV and
-// OBS_XYPRUNE_NAN (src/io.cu:158-235): per node in order rho, u, v, each truth plus
+// OBS_XYPRUNE_NAN (src/io.cu:159-246): per node in order rho, u, v, each truth plus
 // sigma times a draw of std::normal_distribution<float>(0, 1) on one
 // std::mt19937(seed) engine that lives for the whole run; vorticity from the noisy
 // velocities; NaN off the observation lattice (i or j not a multiple of the stride).

@@ -134,7 +134,7 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 }
 void orc_philox_normal4(uint64_t key, const uint32_t ctr[4], double out[4]) { normal4(key, ctr, out); }
 
-// src/data.cu:227-290
+// src/data.cu:161-223 (mode search :172-189, amplitudes normalised to fkf*u_ref :218-223)
 int orc_force_modes(const orc_config* cfg, double* kx, double* ky, double* amp, int cap) {
     const orc_config& c = *cfg;
     if (c.force_kind != 1) return 0;
@@ -163,7 +163,7 @@ int orc_force_modes(const orc_config* cfg, double* kx, double* ky, double* amp, 
     return n;
 }
 
-// src/data.cu:383-414, randoms from Philox instead of cuRAND MTGP32
+// src/data.cu:239-350 (field :305-349), randoms from Philox instead of cuRAND MTGP32
 void orc_force_field(const orc_config* cfg, long long step, double* fx, double* fy) {
     const orc_config& c = *cfg;
     double kx[100], ky[100], amp[100];
@@ -213,7 +213,7 @@ void orc_macro(const orc_config* cfg, const double* f, double* rho, double* u, d
     }
 }
 
-// src/lbm.cu:187-200 (pull streaming)
+// src/lbm.cu:67-103 (pull streaming :81-89; the moments :92-100 are orc_macro)
 void orc_stream(const orc_config* cfg, const double* f, double* fn) {
     const int nx = cfg->nx, ny = cfg->ny;
     const long long N = (long long)nx * ny;
@@ -226,7 +226,7 @@ void orc_stream(const orc_config* cfg, const double* f, double* fn) {
             }
 }
 
-// src/lbm.cu:135-176 (Smagorinsky, LES_CSM not defined)
+// src/lbm.cu:24-65 (Smagorinsky :36-60, LES_CSM not defined)
 void orc_les(const orc_config* cfg, const double* u, const double* v, double* nus) {
     const int nx = cfg->nx, ny = cfg->ny;
     const double dx = cfg->dx, cs = cfg->cs_smag;
@@ -243,34 +243,41 @@ void orc_les(const orc_config* cfg, const double* u, const double* v, double* nu
         }
 }
 
-void orc_lbm_step(const orc_config* cfg, int M, const double* fin, double* fout, long long step) {
+// src/lbm.h:16-29: stream + moments, LES, collide + force, for every member
+void orc_lbm_step_forced(const orc_config* cfg, int M, const double* fin, double* fout, const double* Fx,
+                         const double* Fy) {
     const orc_config& c = *cfg;
     const long long N = (long long)c.nx * c.ny;
-    std::vector<double> Fx(N, 0.0), Fy(N, 0.0);
-    if (c.force_kind == 1) orc_force_field(cfg, step, Fx.data(), Fy.data());
     std::vector<double> r(N), u(N), v(N), nus(N);
     for (int k = 0; k < M; ++k) {
         const double* f = fin + (long long)k * Q * N;
         double* fn = fout + (long long)k * Q * N;
-        orc_stream(cfg, f, fn);                               // src/lbm.cu:178-214
+        orc_stream(cfg, f, fn);                               // src/lbm.cu:67-103
         orc_macro(cfg, fn, r.data(), u.data(), v.data());
-        orc_les(cfg, u.data(), v.data(), nus.data());         // src/lbm.cu:135-176
+        orc_les(cfg, u.data(), v.data(), nus.data());         // src/lbm.cu:24-65
 #pragma omp parallel for
-        for (long long n = 0; n < N; ++n) {                    // src/lbm.cu:216-271
+        for (long long n = 0; n < N; ++n) {                    // src/lbm.cu:105-160
             const double nu = c.nu + nus[n];
-            const double tau = 0.5 + 3.0 * nu / c.c / c.c / c.dt;
+            const double tau = 0.5 + 3.0 * nu / c.c / c.c / c.dt;   // src/lbm.cu:134-138
             const double omega = 1.0 / tau;
             for (int q = 0; q < Q; ++q) {
                 double fc = fn[q * N + n];
                 fc -= omega * (fc - feq_q(c, r[n], u[n], v[n], q));   // lbm_collision_srt.h:8-16
-                fc += force_q(c, Fx[n], Fy[n], q, r[n], u[n], v[n]);
+                fc += force_q(c, Fx[n], Fy[n], q, r[n], u[n], v[n]);  // src/lbm.cu:147-152
                 fn[q * N + n] = fc;
             }
         }
     }
 }
 
-// src/data.cu:91-186 (REFERENCE kind); the K4 kind uses the same streamfunction
+void orc_lbm_step(const orc_config* cfg, int M, const double* fin, double* fout, long long step) {
+    const long long N = (long long)cfg->nx * cfg->ny;
+    std::vector<double> Fx(N, 0.0), Fy(N, 0.0);
+    if (cfg->force_kind == 1) orc_force_field(cfg, step, Fx.data(), Fy.data());   // src/lbm.cu:20-22
+    orc_lbm_step_forced(cfg, M, fin, fout, Fx.data(), Fy.data());
+}
+
+// src/data.cu:25-141 (REFERENCE kind); the K4 kind uses the same streamfunction
 // form over the half plane with Philox phases.  Direct sums over all modes.
 void orc_initial_velocity(const orc_config* cfg, unsigned long long seed, double* u, double* v) {
     const orc_config& c = *cfg;
@@ -369,7 +376,7 @@ int orc_local_obs(const orc_config* cfg, int gridpoint, int* sites, double* weig
         ++n;
     };
     if (c.loc_kind == 0) {
-        // window geometry, src/letkf.cu:129-140
+        // window geometry, src/letkf.cu:45-55
         int n_obs_local, obs_offset, n_obs_x;
         double d_local;
         if (c.loc_rloc_len <= 0) {
@@ -391,11 +398,11 @@ int orc_local_obs(const orc_config* cfg, int gridpoint, int* sites, double* weig
                 const int di = std::abs(i - io), dj = std::abs(j - jo);
                 const int iop = (io + c.nx) % c.nx, jop = (jo + c.ny) % c.ny;
                 double w = 1.0;
-                if (p > 1) {  // src/letkf.cu:182-197
+                if (p > 1) {  // src/letkf.cu:97-113 (R-localization)
                     const double cutoff = d_local + 1e-6;
                     w = orc_gaspari_cohn(std::sqrt(double(di) * di + double(dj) * dj) / cutoff);
                 }
-                if (di > n_obs_local * p || dj > n_obs_local * p) w = 0.0;  // src/letkf.cu:265
+                if (di > n_obs_local * p || dj > n_obs_local * p) w = 0.0;  // src/letkf.cu:181,264
                 put(iop / p + (jop / p) * o.n_sx, w);
             }
     } else if (g_custom) {
@@ -699,7 +706,7 @@ void mlp_forward(const double* P, const double* x_in, int bf16, Fwd& F) {
     F.z = z;
 }
 
-// strain (a, b) of node (i, j) by the central differences of src/lbm.cu:155-158
+// strain (a, b) of node (i, j) by the central differences of src/lbm.cu:44-47
 inline void strain_ab(int nx, int ny, double dx, const double* u, const double* v, int i, int j, double& a,
                       double& b) {
     const long long e = pmod(i + 1, nx) + (long long)nx * j, w = pmod(i - 1, nx) + (long long)nx * j;

@@ -49,7 +49,7 @@ void orc_set_threads(int n);
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 void orc_philox_normal4(uint64_t key, const uint32_t ctr[4], double out[4]);
 
-/* forcing: src/data.cu:227-290 (modes) and :383-414 (field) */
+/* forcing: src/data.cu:161-223 (modes) and :239-350 (field) */
 int  orc_force_modes(const orc_config* cfg, double* kx, double* ky, double* amp, int cap);
 void orc_force_field(const orc_config* cfg, long long step, double* fx, double* fy);
 
@@ -60,8 +60,11 @@ void orc_stream(const orc_config* cfg, const double* f, double* fn);
 void orc_les(const orc_config* cfg, const double* u, const double* v, double* nus);
 /* one full step of M members, f[M][9][ny][nx]; step indexes the forcing stream */
 void orc_lbm_step(const orc_config* cfg, int M, const double* fin, double* fout, long long step);
+/* the same step with a given body-force field Fx, Fy [ny][nx] (shared by the members) */
+void orc_lbm_step_forced(const orc_config* cfg, int M, const double* fin, double* fout, const double* Fx,
+                         const double* Fy);
 
-/* initial velocity of one member, src/data.cu:91-186 (direct mode sums) */
+/* initial velocity of one member, src/data.cu:25-141 (direct mode sums) */
 void orc_initial_velocity(const orc_config* cfg, unsigned long long seed, double* u, double* v);
 
 /* observation network and local sets: src/letkf.cu:29-56, 97-113, 150-187 */
@@ -94,7 +97,7 @@ void orc_ensemble_stats(const orc_config* cfg, int M, const double* f, const dou
 /* ---- learned eddy-viscosity closure (a north-star extension; no reference code) ----
  * features at node n: for the 3x3 stencil s = (dj+1)*3 + (di+1), the deviatoric
  * strain a = (ux - vy)/2 and shear b = (uy + vx)/2 at node (i+di, j+dj) from the
- * central differences of src/lbm.cu:155-158, divided by strain_scale:
+ * central differences of src/lbm.cu:44-47, divided by strain_scale:
  * feat[n][2s] = a, feat[n][2s+1] = b.  ab[n] = (a, b) at the node itself, unscaled.
  * MLP 18 -> 64 -> 64 -> 1: h1 = gelu(W1 x + b1), h2 = gelu(W2 h1 + b2),
  * z = w3 . h2 + b3, nu_t = nu_scale * softplus(z); gelu(x) = x/2 (1 + tanh(k (x +
@@ -131,10 +134,10 @@ void   orc_set_obs_sites(int n, const int* nodes);
  * rho may be NULL (rho_o = the member's own density) */
 void   orc_nudge(const orc_config* cfg, int M, double* f, const double* rho, const double* u, const double* v,
                  double alpha, int xyprune, int quadra);
-/* src/data.cu:432-443: the random velocities of ensemble id (std::mt19937(id + 132333),
+/* src/data.cu:432-447: the random velocities of ensemble id (std::mt19937(id + 132333),
  * std::normal_distribution<float>, u then v per node, rows outer, times u_ref) */
 void   orc_perturb_draws(const orc_config* cfg, unsigned long long ensemble_id, double* ur, double* vr);
-/* src/data.cu:445-453: f [9][ny][nx] := eps feq(rho, ur, vr) + (1 - eps) f */
+/* src/data.cu:443-454: f [9][ny][nx] := eps feq(rho, ur, vr) + (1 - eps) f */
 void   orc_perturb(const orc_config* cfg, double* f, const double* ur, const double* vr, double eps);
 
 #ifdef __cplusplus

@@ -31,6 +31,7 @@ _sig = {
     "orc_stream": ([P, P, P], None),
     "orc_les": ([P, P, P, P], None),
     "orc_lbm_step": ([P, I, P, P, C.c_longlong], None),
+    "orc_lbm_step_forced": ([P, I, P, P, P, P], None),
     "orc_initial_velocity": ([P, C.c_ulonglong, P, P], None),
     "orc_obs_count": ([P, C.POINTER(I), C.POINTER(I)], None),
     "orc_local_obs": ([P, I, P, P, I], I),
@@ -138,6 +139,15 @@ def lbm_step(cfg, f, step: int):
     M = fin.shape[0]
     fout = np.empty_like(fin)
     _o.orc_lbm_step(_c(cfg), M, _p(fin), _p(fout), step)
+    return fout
+
+
+def lbm_step_forced(cfg, f, Fx, Fy):
+    """One step with the given body-force field [ny][nx] in place of the configured forcing."""
+    fin = _f64(f)
+    M = fin.shape[0]
+    fout = np.empty_like(fin)
+    _o.orc_lbm_step_forced(_c(cfg), M, _p(fin), _p(fout), _p(_f64(Fx)), _p(_f64(Fy)))
     return fout
 
 

@@ -39,7 +39,7 @@ def test_presets_and_validation(lb):
     with pytest.raises(lb.LbmLetkfError):
         lb.config_check(cfg, 65)  # more members than lanes of a wavefront
     bad = cfg.copy()
-    bad.obs_stride = 7  # does not divide 256 (src/letkf.cu:114 static_assert)
+    bad.obs_stride = 7  # does not divide 256 (src/letkf.cu:30 static_assert)
     with pytest.raises(lb.LbmLetkfError, match="obs_stride"):
         lb.config_check(bad, 16)
     bad = cfg.copy()

@@ -110,6 +110,148 @@ def test_uniform_equilibrium_is_fixed_point(lb, orc):
     np.testing.assert_allclose(orc.lbm_step(cfg, f, 0), f, rtol=1e-14)
 
 
+def _shear_wave_rate(cfg, orc, ny, step_fn, t0=100, t1=400):
+    """Measured per-step decay rate of u = A sin(2 pi j/ny) (nx = 4, v = 0, rho = 1)
+    from the projection of u on the mode at steps t0 and t1 (the non-hydrodynamic
+    modes excited by starting from feq have died out by t0 for tau >= 0.6)."""
+    cfg.nx, cfg.ny = 4, ny
+    j = np.arange(ny)[:, None] * np.ones((1, 4))
+    s = np.sin(2 * np.pi * np.arange(ny) / ny)
+    u = 1e-4 * cfg.c * np.sin(2 * np.pi * j / ny)
+    f = orc.feq(cfg, np.ones_like(u), u, np.zeros_like(u))[None]
+    amp = {}
+    for t in range(t1 + 1):
+        if t in (t0, t1):
+            _, uu, _ = orc.macro(cfg, f[0])
+            amp[t] = (uu[:, 0] @ s) / (s @ s)
+        f = step_fn(f, t)
+    return -math.log(amp[t1] / amp[t0]) / (t1 - t0)
+
+
+@pytest.mark.parametrize("preset", ["baseline", "ref"])
+@pytest.mark.parametrize("tau", [0.6, 0.8])
+def test_shear_wave_decays_at_the_viscous_rate(lb, orc, preset, tau):
+    # Navier-Stokes: a shear wave u = A sin(k y) decays as exp(-nu k^2 t) with the
+    # kinematic viscosity nu of the configuration.  BGK reproduces it with
+    # tau = 1/2 + 3 nu/(c^2 dt); the lattice's own correction is O((k dx)^2) and
+    # below 2e-4 relative at 128 cells.  A relaxation time built with c instead of
+    # c^2 (the ref preset has c = 20), without the factor 3 or without the 1/2 moves
+    # the rate by a factor 3 to 20; Cs = 0 removes the eddy viscosity.
+    cfg = lb.preset(preset)
+    cfg.cs_smag = 0.0
+    cfg.force_kind = 0
+    cfg.friction_rate = 0.0
+    cfg.nu = (tau - 0.5) * cfg.c**2 * cfg.dt / 3.0
+    ny = 128
+    rate = _shear_wave_rate(cfg, orc, ny, lambda f, t: orc.lbm_step(cfg, f, t))
+    k = 2 * np.pi / (ny * cfg.dx)
+    assert rate == pytest.approx(cfg.nu * k * k * cfg.dt, rel=1e-3)
+
+
+@pytest.mark.parametrize("preset", ["baseline", "ref"])
+def test_shear_wave_with_constant_eddy_viscosity(lb, orc, preset):
+    # A closure network with zero weights outputs nu_t = nu_scale softplus(b3): a
+    # constant eddy viscosity, which must add to the molecular one in the decay rate.
+    cfg = lb.preset(preset)
+    cfg.cs_smag = 0.0
+    cfg.force_kind = 0
+    cfg.friction_rate = 0.0
+    cfg.nu = 0.1 * cfg.c**2 * cfg.dt / 3.0  # tau 0.6 alone
+    nut = 0.2 * cfg.c**2 * cfg.dt / 3.0  # tau 0.8 with it
+    cfg.closure_nu_scale = nut / math.log(2.0)  # softplus(0) = ln 2
+    cfg.closure_nu_max = 10 * nut
+    params = np.zeros(orc.CLOSURE_NPARAM)
+    ny = 128
+    rate = _shear_wave_rate(cfg, orc, ny, lambda f, t: orc.lbm_step_closure(cfg, f, t, params))
+    k = 2 * np.pi / (ny * cfg.dx)
+    assert rate == pytest.approx((cfg.nu + nut) * k * k * cfg.dt, rel=1e-3)
+
+
+@pytest.mark.parametrize("rho0", [1.0, 1.02])
+def test_uniform_force_momentum_step(lb, orc, rho0):
+    # Fluid at rest under a uniform body force F: one step changes the momentum of
+    # every node by exactly rho (u_ref / c) F dt.  This is the first-order force term
+    # w_q rho (u_ref e_q . F dt)/(c^2/3) of the reference, whose e_q is scaled by
+    # u_ref rather than c: with u_ref = 1 and c = 20 (ref preset) the step is 1/20 of
+    # the textbook rho F dt / rho.  u_ref = 2.5 separates the two factors.
+    cfg = small_cfg(lb, "ref", n=8)
+    cfg.force_kind = 0
+    cfg.u_ref = 2.5
+    Fx, Fy = 0.7, -0.3
+    one = np.ones((8, 8))
+    f = orc.feq(cfg, rho0 * one, 0 * one, 0 * one)[None]
+    f1 = orc.lbm_step_forced(cfg, f, Fx * one, Fy * one)
+    r, u, v = orc.macro(cfg, f1[0])
+    np.testing.assert_allclose(r, rho0, rtol=1e-14)
+    # (1e-10: the momentum, ~1e-4, is a sum of populations ~0.1)
+    np.testing.assert_allclose(r * u, rho0 * cfg.u_ref / cfg.c * Fx * cfg.dt, rtol=1e-10)
+    np.testing.assert_allclose(r * v, rho0 * cfg.u_ref / cfg.c * Fy * cfg.dt, rtol=1e-10)
+
+
+def test_friction_decay_factor(lb, orc):
+    # A uniform flow without force loses the fraction friction_rate * u_ref / c of
+    # its velocity per step (F' = -(friction/dt) u dt, momentum rho (u_ref/c) F').
+    cfg = small_cfg(lb, "ref", n=8)
+    cfg.force_kind = 0
+    cfg.friction_rate = 0.02
+    cfg.u_ref = 2.5
+    one = np.ones((8, 8))
+    u0, v0 = 0.3, -0.1
+    f = orc.feq(cfg, one, u0 * one, v0 * one)[None]
+    zero = np.zeros((8, 8))
+    for _ in range(50):
+        f = orc.lbm_step_forced(cfg, f, zero, zero)
+    _, u, v = orc.macro(cfg, f[0])
+    g = (1 - cfg.friction_rate * cfg.u_ref / cfg.c) ** 50
+    np.testing.assert_allclose(u, u0 * g, rtol=1e-12)
+    np.testing.assert_allclose(v, v0 * g, rtol=1e-12)
+    # the ref preset's own constants: 5e-4, u_ref = 1, c = 20 -> 1 - 2.5e-5 per step
+    ref = lb.preset("ref")
+    assert 1 - ref.friction_rate * ref.u_ref / ref.c == pytest.approx(1 - 2.5e-5, rel=1e-15)
+
+
+def test_force_and_friction_reach_terminal_velocity(lb, orc):
+    # u_{n+1} = u_n + (u_ref/c)(F - (friction/dt) u_n) dt: geometric approach to
+    # u* = F dt / friction with ratio 1 - friction u_ref / c.
+    cfg = small_cfg(lb, "ref", n=4)
+    cfg.force_kind = 0
+    cfg.friction_rate = 0.05
+    cfg.u_ref = 4.0
+    one = np.ones((4, 4))
+    F = 0.02
+    f = orc.feq(cfg, one, 0 * one, 0 * one)[None]
+    n = 30
+    for _ in range(n):
+        f = orc.lbm_step_forced(cfg, f, F * one, 0 * one)
+    _, u, _ = orc.macro(cfg, f[0])
+    ustar = F * cfg.dt / cfg.friction_rate
+    g = 1 - cfg.friction_rate * cfg.u_ref / cfg.c
+    np.testing.assert_allclose(u, ustar * (1 - g**n), rtol=1e-9)  # 30 steps of FP64 rounding on u ~ 1e-4
+
+
+def test_force_field_fourier_coefficients(lb, orc):
+    # Each mode of the Watanabe-97 field is, in Fourier space, one coefficient at
+    # (ky, kx) (and its conjugate): with x = (i - n/2)/n, theta = 2 pi (kx x + ky y),
+    # P = r0 kx + r1 ky, Q = r2 kx + r3 ky (r ~ N(0, s^2), s = 2^-1/4), the field
+    # amp 2 pi ky (P sin theta + Q cos theta) has the DFT coefficient
+    # n^2 (-1)^(kx+ky) amp 2 pi ky (Q - i P)/2.  A missing 2 pi, a wrong amplitude or
+    # stdev, swapped draws or a shifted origin (i/n instead of (i - n/2)/n) fail.
+    cfg = small_cfg(lb, "ref", n=32)
+    assert cfg.force_rand_stdev == pytest.approx(2**-0.25, rel=1e-10)  # src/data.cu:243
+    n, step = 32, 5
+    fx, fy = orc.force_field(cfg, step)
+    FX, FY = np.fft.fft2(fx), np.fft.fft2(fy)
+    kx, ky, amp = orc.force_modes(cfg)
+    stream_force = 0x464F5243  # the forcing stream's counter word of the oracle's Philox draws
+    for m, (a, b, A) in enumerate(zip(kx.astype(int), ky.astype(int), amp)):
+        r = orc.philox_normal4(cfg.force_seed, (step, 0, m, stream_force)) * cfg.force_rand_stdev
+        P, Qc = r[0] * a + r[1] * b, r[2] * a + r[3] * b
+        base = n * n * (-1) ** (a + b) * A * 2 * np.pi * (Qc - 1j * P) / 2
+        np.testing.assert_allclose(FX[b, a], base * b, rtol=1e-9, atol=1e-9 * abs(base))
+        np.testing.assert_allclose(FY[b, a], base * -a, rtol=1e-9, atol=1e-9 * abs(base))
+        np.testing.assert_allclose(FX[-b % n, -a % n], np.conj(base * b), rtol=1e-9, atol=1e-9 * abs(base))
+
+
 def test_smagorinsky_closed_form(lb, orc):
     cfg = small_cfg(lb, "ref", n=32)
     n, A = 32, 0.7

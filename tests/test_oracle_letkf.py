@@ -29,7 +29,7 @@ def test_gaspari_cohn_published_values(orc):
 
 
 def _brute_window_sites(cfg, i, j):
-    """Sites of the reference window (src/letkf.cu:129-140,164-165) found by scanning
+    """Sites of the reference window (src/letkf.cu:45-55,164-165) found by scanning
     every lattice site and keeping those inside the window's box."""
     p = cfg.obs_stride
     if cfg.loc_rloc_len == 0 and p == 1:
