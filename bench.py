@@ -7,10 +7,13 @@ LBM steps of all 64 members + one LETKF analysis at all 65,536 gridpoints.
 
 The nature run and its observations for every cycle are generated first and kept
 in HBM; the timed region covers K cycles of the ensemble only (the reference's
-`forecast` + `DA` timers, src/main.cu:95-115).  Both variants are measured:
-plain (per-member LBM launches, FP32 Gram, FP64 two-sided Jacobi, eager launches)
-and mixed (batched LBM, fused MFMA-Gram / FP32+FP64 one-sided Jacobi analysis, each
-cycle replayed from a hipGraph).  `value` is the mixed variant's DA cycles/s.  The
+`forecast` + `DA` timers, src/main.cu:95-115).  The variants:
+plain (per-member LBM launches, FP32 Gram, FP64 two-sided Jacobi, eager launches) at
+the preset's eigensolver tolerance (1e-10) and at the reference's (syevj tolerance
+1e-6, src/util/cu_matrix_eigensolver_wrapper.hpp:192,202), and mixed (all members'
+LBM steps in one launch, one fused analysis kernel -- split-bf16 MFMA Gram, FP32
+one-sided Jacobi, FP64 eigenvalue coefficients, FP32 MFMA update -- each cycle
+replayed from a hipGraph).  `value` is the mixed variant's DA cycles/s.  The
 analysis RMSE of each variant is taken at the observation time after the timed
 cycles; a few further cycles with phase timing on give the per-kernel breakdown.
 
@@ -36,7 +39,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
-FP32_PEAK_TFLOPS = 157.3
+FP32_PEAK_TFLOPS = 157.3      # published FP32 vector peak (counts packed FP32 issue)
+FP32_UNPACKED_TFLOPS = 78.6   # the same without packed FP32, which this build never emits
 LBM_BYTES_PER_NODE_UPDATE = 72  # 9 x 4 B read + 9 x 4 B written
 
 
@@ -101,7 +105,8 @@ class Hip:
             return "unknown"
 
 
-def bench_variant(lb, hip, cfg, variant, M, K, W, P, obs, truth, n_obs, log, repeats=1):
+def bench_variant(lb, hip, cfg, variant, M, K, W, P, obs, truth, n_obs, log, repeats=1, label=None,
+                  fixed_tuning=None):
     """K timed DA cycles (after W untimed ones) through lbmletkf_cycle -- for the
     mixed variant the captured graph is replayed -- then the analysis RMSE at the
     next observation time; then repeats-1 further timed runs of K cycles (their
@@ -121,7 +126,11 @@ def bench_variant(lb, hip, cfg, variant, M, K, W, P, obs, truth, n_obs, log, rep
     ens.forecast(steps)  # to the time of the first observation
     tuning = None
     if variant == lb.VARIANT_MIXED:
-        log("[mixed] autotune: " + ens.autotune())
+        if fixed_tuning:
+            ens.set_tuning(*fixed_tuning)
+            log(f"[mixed] tuning fixed at {fixed_tuning}")
+        else:
+            log("[mixed] autotune: " + ens.autotune())
         tuning = list(ens.get_tuning())
     for c in range(W):
         ens.cycle_device(yo(c))
@@ -163,7 +172,7 @@ def bench_variant(lb, hip, cfg, variant, M, K, W, P, obs, truth, n_obs, log, rep
     ens.synchronize()
     phases = {ph: ens.profile_read(ph) for ph in ("forecast", "obs", "gram", "solve", "finish", "solve_fp64")}
     ens.close()
-    name = "plain" if variant == lb.VARIANT_PLAIN else "mixed"
+    name = label or ("plain" if variant == lb.VARIANT_PLAIN else "mixed")
     ws = sorted(walls)
     wall = ws[len(ws) // 2]
     q1, q3 = (np.percentile(ws, 25), np.percentile(ws, 75)) if len(ws) > 1 else (ws[0], ws[0])
@@ -173,7 +182,25 @@ def bench_variant(lb, hip, cfg, variant, M, K, W, P, obs, truth, n_obs, log, rep
         log(f"    {ph:10s} {sec * 1e3 / P:9.3f} ms/cycle  launches {n}  work {wk}   (phase-timed pass, {P} cycles)")
     return dict(wall=wall, walls=walls, iqr_cycles_per_s=[K / q3, K / q1], event_s=float(np.median(evs)) / 1e3,
                 phases=phases, stats=stats, steps=steps, state=final_state, yo_index=ca, prof_cycles=P,
-                variant=name, tuning=tuning)
+                variant="plain" if variant == lb.VARIANT_PLAIN else "mixed", tuning=tuning, K=K,
+                jacobi_tol=cfg.jacobi_tol)
+
+
+def rmse_probe(lb, cfg, M, n_cycles, obs, truth, n_obs):
+    """Analysis statistics of the mixed variant after n_cycles DA cycles, at the same
+    observation time as a shorter run (the plain variants' RMSE is compared there)."""
+    cfg = cfg.copy()
+    cfg.variant = lb.VARIANT_MIXED
+    nodes = cfg.nx * cfg.ny
+    with lb.Context(cfg, M) as ens:
+        for k in range(M):
+            ens.init_member(k, 1000 + k)
+        ens.forecast(cfg.da_interval)
+        for c in range(n_cycles):
+            ens.cycle_device(obs + 4 * n_obs * c)
+        ens.analysis_device(obs + 4 * n_obs * n_cycles)
+        tp = truth + 4 * 3 * nodes * n_cycles
+        return ens.ensemble_stats_device(tp, tp + 4 * nodes, tp + 8 * nodes)
 
 
 def measured_traffic(kernel, workgroups):
@@ -212,35 +239,37 @@ def kernel_table(r, M, nodes):
                            bytes_of_single_steps_GBs=LBM_BYTES_PER_NODE_UPDATE * M * nodes / avg / 1e9,
                            peak=HBM_PEAK_GBS, bound="hbm", share=sec)
     sec, n, sweeps = r["phases"]["solve"]
+    sweeps_gp = sweeps / (nodes * P) if P else 0.0
     if r["variant"] == "mixed":
-        # one-sided Jacobi: per sweep N-1 rounds x N columns x (N FMA for the pair
-        # inner product + N FMA for the rotation) ~ 4 N^3 flop (N = 64)
-        flops = 4.0 * M**3 * sweeps
-        out["letkf_solve"] = dict(avg_ms=sec / max(n, 1) * 1e3, achieved=flops / sec / 1e12 if sec else 0.0,
-                                  unit="TFLOP/s", peak=FP32_PEAK_TFLOPS, bound="mfma",
-                                  kernel="k_mixed_solve (gather + MFMA Gram + FP32 sweeps)",
-                                  engine="fp32 vector ALU (the published fp32 vector peak counts packed "
-                                         "FP32, which this build does not emit: attainable ceiling is half)",
-                                  sweeps_per_gridpoint=sweeps / (nodes * P), share=sec)
-        sec, n, _ = r["phases"]["finish"]
-        sweeps64 = r["phases"]["solve_fp64"][2]
-        flops = 4.0 * M**3 * sweeps64
-        out["letkf_finish"] = dict(avg_ms=sec / max(n, 1) * 1e3, achieved=flops / sec / 1e12 if sec else 0.0,
-                                   unit="TFLOP/s", peak=FP64_PEAK_TFLOPS, bound="mfma",
-                                   kernel="k_mixed_finish (FP64 sweeps + assembly + update)",
-                                   engine="fp64 vector ALU", fp64_sweeps_per_gridpoint=sweeps64 / (nodes * P),
-                                   share=sec)
+        # the fused analysis kernel is bound by its FP32 vector ALU (the one-sided
+        # sweeps).  Algorithmic flops of a one-sided sweep: N-1 rounds x N columns x
+        # (N FMA for the pair inner product + N FMA for the update) ~ 4 N^3 (N = M = 64).
+        # The two-sided count (M(M-1)/2 rotations x 18 M per sweep) is given beside it.
+        t = sec / max(n, 1)
+        flops_own = 4.0 * M**3 * sweeps_gp * nodes
+        flops_2s = M * (M - 1) / 2 * 18.0 * M * sweeps_gp * nodes
+        out["letkf_analysis"] = dict(
+            avg_ms=t * 1e3, achieved=flops_own / t / 1e12 if t else 0.0, unit="TFLOP/s",
+            peak=FP32_PEAK_TFLOPS, bound="valu",
+            kernel="k_mixed_analysis<64, 3> (gather + split-bf16 MFMA Gram + FP32 one-sided sweeps + "
+                   "FP64 coefficients + FP32 MFMA update)",
+            flop_count="one-sided Jacobi: 4 N^3 per sweep per gridpoint x measured sweeps",
+            peak_unpacked_fp32=FP32_UNPACKED_TFLOPS,
+            frac_of_unpacked_fp32=(flops_own / t / 1e12 / FP32_UNPACKED_TFLOPS) if t else 0.0,
+            achieved_two_sided_count=flops_2s / t / 1e12 if t else 0.0,
+            frac_two_sided_count=(flops_2s / t / 1e12 / FP32_PEAK_TFLOPS) if t else 0.0,
+            sweeps_per_gridpoint=sweeps_gp, fp64_sweeps_per_gridpoint=0.0, share=sec)
     else:
         flops = 9.0 * M**3 * sweeps  # two-sided rotation update (12M) + V update (6M) per rotation
         out["letkf_solve"] = dict(avg_ms=sec / max(n, 1) * 1e3, achieved=flops / sec / 1e12 if sec else 0.0,
-                                  unit="TFLOP/s", peak=FP64_PEAK_TFLOPS, bound="mfma",
-                                  engine="fp64 vector ALU (MI355X fp64 vector peak = fp64 matrix peak)",
-                                  sweeps_per_gridpoint=sweeps / (nodes * P), share=sec)
+                                  unit="TFLOP/s", peak=FP64_PEAK_TFLOPS, bound="valu",
+                                  engine="fp64 vector ALU + LDS (two-sided Jacobi)",
+                                  sweeps_per_gridpoint=sweeps_gp, share=sec)
     sec, n, pobs = r["phases"]["gram"]
     if n:
         gflops = 2.0 * M * M * pobs
         out["letkf_gram"] = dict(avg_ms=sec / max(n, 1) * 1e3, achieved=gflops / sec / 1e12 if sec else 0.0,
-                                 unit="TFLOP/s", peak=FP32_PEAK_TFLOPS, bound="mfma",
+                                 unit="TFLOP/s", peak=FP32_PEAK_TFLOPS, bound="valu",
                                  engine="fp32 vector ALU", local_obs_per_gridpoint=pobs / (nodes * P), share=sec)
     sec, n, _ = r["phases"]["obs"]
     out["obs_prepare"] = dict(avg_ms=sec / max(n, 1) * 1e3, share=sec)
@@ -250,34 +279,59 @@ def kernel_table(r, M, nodes):
     return out
 
 
+def cpu_info():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return os.cpu_count() or 1, model
+
+
 def cpu_baseline(cfg, f_state, yo, log):
-    """The FP64 oracle on the host cores over a bounded sample of one cycle,
-    starting from the GPU run's final ensemble and its last observations."""
+    """The FP64 oracle (OpenMP on every host core, LAPACK dsyev for the local
+    eigensolves) over one whole DA cycle: da_interval LBM steps of all members, then
+    the analysis at every gridpoint, starting from the GPU run's final ensemble and
+    its last observations."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_ref as orc
 
-    threads = min(16, os.cpu_count() or 1)
-    orc.set_threads(threads)
+    nproc, model = cpu_info()
+    orc.set_threads(nproc)
+    try:
+        from threadpoolctl import threadpool_limits
+        limits = threadpool_limits(1)  # one BLAS thread per OpenMP thread
+    except Exception:
+        limits = None
+    try:
+        solver = orc.set_dsyev(True)
+    except Exception as e:  # SciPy's LAPACK not reachable: the oracle's own Jacobi
+        solver = orc.set_dsyev(False) + f" (dsyev unavailable: {e})"
     n = cfg.nx
     M = f_state.shape[0]
     f = f_state.astype(np.float64)
-    steps = 2
     t0 = time.perf_counter()
-    for s in range(steps):
+    for s in range(cfg.da_interval):
         f = orc.lbm_step(cfg, f, s)
-    t_step = (time.perf_counter() - t0) / steps
-    stride = 64
+    t_lbm = time.perf_counter() - t0
     t0 = time.perf_counter()
-    orc.letkf_analysis(cfg, f, yo, g_stride=stride)
-    t_ana = (time.perf_counter() - t0) * stride
-    cycle = cfg.da_interval * t_step + t_ana
-    sample = (f"{steps} LBM steps of all {M} members at {n}^2 (x{cfg.da_interval} for one cycle) + "
-              f"LETKF analysis at every {stride}th gridpoint ({n * n // stride} local analyses, x{stride}); "
-              f"FP64, OpenMP")
-    log(f"[cpu] step {t_step * 1e3:.1f} ms, analysis {t_ana:.2f} s (extrapolated), cycle {cycle:.2f} s, "
-        f"{threads} threads")
-    return dict(value=1.0 / cycle, unit="DA cycles/s", cores=threads, kind="port", sample=sample,
-                lbm_step_s=t_step, analysis_s=t_ana)
+    orc.letkf_analysis(cfg, f, yo)
+    t_ana = time.perf_counter() - t0
+    orc.set_dsyev(False)
+    if limits is not None:
+        limits.unregister()
+    cycle = t_lbm + t_ana
+    sample = (f"one whole DA cycle: {cfg.da_interval} LBM steps of all {M} members at {n}^2, then the LETKF "
+              f"analysis at all {n * n} gridpoints; FP64; OpenMP; eigensolver {solver}")
+    log(f"[cpu] {nproc} threads ({model}): LBM {t_lbm:.2f} s, analysis {t_ana:.2f} s, cycle {cycle:.2f} s")
+    return dict(value=1.0 / cycle, unit="DA cycles/s", cores=nproc, kind="port", sample=sample,
+                nproc=nproc, cpu_model=model, eigensolver=solver, lbm_s=t_lbm, analysis_s=t_ana,
+                lbm_node_updates_per_s=cfg.da_interval * M * n * n / t_lbm,
+                local_analyses_per_s=n * n / t_ana)
 
 
 def bench_closure_step(lb, hip, M, log):
@@ -384,12 +438,19 @@ def bench_uneven(lb, hip, K, W, log, tune_cache=None):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20, help="timed DA cycles per run")
+    ap.add_argument("--steps", type=int, default=100, help="timed DA cycles per run (mixed variant)")
     ap.add_argument("--warmup", type=int, default=10, help="untimed DA cycles")
     ap.add_argument("--runs", type=int, default=3, help="timed runs of the mixed variant (median reported)")
+    ap.add_argument("--plain-steps", type=int, default=10,
+                    help="timed DA cycles of each plain variant (~0.5 s per cycle)")
     ap.add_argument("--members", type=int, default=64)
-    ap.add_argument("--variants", default="plain,mixed")
+    ap.add_argument("--variants", default="plain,plain_tol1e-6,mixed",
+                    help="plain (eigensolver tolerance 1e-10), plain_tol1e-6 (the reference's syevj "
+                         "tolerance), mixed")
     ap.add_argument("--prof-cycles", type=int, default=3, help="extra phase-timed cycles (eager launches)")
+    ap.add_argument("--tuning", default=None, metavar="LBM,CHAIN",
+                    help="fixed launch shapes of the mixed variant instead of the autotuner (for profiling "
+                         "runs: LBM shape code, analysis chain), e.g. -116,32")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-batch group measurement")
     ap.add_argument("--uneven-steps", type=int, default=3, help="timed group cycles of the uneven batch")
@@ -444,32 +505,44 @@ def main():
 
     res = {}
     for v in args.variants.split(","):
-        var = lb.VARIANT_PLAIN if v == "plain" else lb.VARIANT_MIXED
-        res[v] = bench_variant(lb, hip, cfg, var, M, K, W, P, obs, truth, n_obs, log,
-                               repeats=R if var == lb.VARIANT_MIXED else 1)
+        vcfg = cfg.copy()
+        if v.startswith("plain"):
+            var, k = lb.VARIANT_PLAIN, min(K, args.plain_steps)
+            if v == "plain_tol1e-6":
+                vcfg.jacobi_tol = 1e-6
+        else:
+            var, k = lb.VARIANT_MIXED, K
+        res[v] = bench_variant(lb, hip, vcfg, var, M, k, W if var == lb.VARIANT_MIXED else min(W, 2), P, obs,
+                               truth, n_obs, log, repeats=R if var == lb.VARIANT_MIXED else 1, label=v,
+                               fixed_tuning=[int(x) for x in args.tuning.split(",")] if args.tuning else None)
     head = res.get("mixed") or res[list(res)[0]]
     kern = kernel_table(head, M, nodes)
     dom_name = max(kern, key=lambda k: kern[k]["share"])
     dom = kern[dom_name]
     tune = head.get("tuning") or [1, 8]
     solve_wg = (nodes + tune[1] - 1) // tune[1]
-    traffic_solve = measured_traffic("k_mixed_solve<64, 3>", solve_wg) if head["variant"] == "mixed" else None
+    mixed = head["variant"] == "mixed"
+    traffic_ana = measured_traffic("k_mixed_analysis<64, 3>", solve_wg) if mixed else None
     if tune[0] <= -100:
         lbm_kernel, lbm_wg = "k_lbm_rows2<%d>" % (-tune[0] - 100), (cfg.ny // (-tune[0] - 100)) * M
     elif tune[0] < 0:
         lbm_kernel, lbm_wg = "k_lbm_rows<false, %d>" % -tune[0], (cfg.ny // -tune[0]) * M
     else:
         lbm_kernel, lbm_wg = "k_lbm_fused<false>", cfg.nx // 64 * (cfg.ny // 16) * M // tune[0]
-    traffic_lbm = measured_traffic(lbm_kernel, lbm_wg) if head["variant"] == "mixed" else None
-    roof = dict(kernel=dom_name, bound=dom.get("bound", "hbm"), achieved=dom.get("achieved"),
+    traffic_lbm = measured_traffic(lbm_kernel, lbm_wg) if mixed else None
+    roof = dict(kernel=dom.get("kernel", dom_name), bound=dom.get("bound", "hbm"), achieved=dom.get("achieved"),
                 peak=dom.get("peak"), unit=dom.get("unit"),
                 frac=(dom["achieved"] / dom["peak"]) if dom.get("peak") else None,
-                traffic=traffic_solve if dom_name == "letkf_solve" else None,
-                traffic_unit="bytes per launch, rocprofv3 FETCH_SIZE x2 + WRITE_SIZE (profiles/hbm_traffic_256x256_M64.md)",
+                traffic=traffic_ana if dom_name == "letkf_analysis" else None,
+                traffic_unit="bytes per launch, rocprofv3 FETCH_SIZE x2 + WRITE_SIZE "
+                             "(profiles/hbm_traffic_256x256_M64.md)",
                 avg_launch_ms=dom["avg_ms"], share_of_cycle=dom["share"])
-    if "engine" in dom:
-        roof["engine"] = dom["engine"]
+    for key in ("flop_count", "peak_unpacked_fp32", "frac_of_unpacked_fp32", "achieved_two_sided_count",
+                "frac_two_sided_count", "sweeps_per_gridpoint"):
+        if key in dom:
+            roof[key] = dom[key]
     lbm = kern["lbm_step"]
+    lbm_bytes = LBM_BYTES_PER_NODE_UPDATE * M * nodes
     out = {
         "metric": "DA cycles/s at 256²×M=64 (plain vs mixed+batched) at equal analysis RMSE",
         "value": K / head["wall"],
@@ -481,7 +554,9 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 (LBM, Gram) + fp64 (Jacobi, transform)",
+        "dtype": ("FP32 LBM; analysis: split-bf16x3 MFMA Gram (FP32 accumulation), FP32 one-sided Jacobi "
+                  "(0 FP64 sweeps), FP64 eigenvalue norms and coefficients, FP32 MFMA W.Xb update") if mixed
+                 else "FP32 LBM; FP32 Gram; FP64 two-sided Jacobi and transform",
         "data": "synthetic",
         "config": {"workload": "LETKF OSSE 256x256, M=64 members, 100 LBM steps per DA cycle, "
                                "(u,v) every 8th node, Gaussian L=12 cutoff 43.8, beta=1.1",
@@ -489,8 +564,9 @@ def main():
                    "n_obs": n_obs, "variant_timed": "mixed" if "mixed" in res else list(res)[0]},
         "runs": len(head["walls"]),
         "cycles_per_s_iqr": head["iqr_cycles_per_s"],
-        "variants": {k: {"cycles_per_s": K / r["wall"], "ms_per_cycle": r["wall"] / K * 1e3,
-                         "timed_runs": len(r["walls"]),
+        "variants": {k: {"cycles_per_s": r["K"] / r["wall"], "ms_per_cycle": r["wall"] / r["K"] * 1e3,
+                         "timed_cycles": r["K"], "timed_runs": len(r["walls"]),
+                         "jacobi_tol": r["jacobi_tol"],
                          "analysis_rmse": r["stats"][0], "analysis_spread": r["stats"][1],
                          "tuning_lbm_tiles_per_block_and_chain": r["tuning"],
                          "phase_ms_per_cycle": {p: r["phases"][p][0] / P * 1e3 for p in r["phases"]
@@ -499,15 +575,29 @@ def main():
         "roofline": roof,
         "roofline_lbm": dict(kernel=lbm_kernel, bound="hbm", achieved=lbm["achieved"], peak=HBM_PEAK_GBS,
                              unit="GB/s", frac=lbm["achieved"] / HBM_PEAK_GBS, traffic=traffic_lbm,
-                             algorithmic_bytes=LBM_BYTES_PER_NODE_UPDATE * M * nodes,
+                             algorithmic_bytes=lbm_bytes,
+                             bytes_note="algorithmic bytes per launch: every population read and written once "
+                                        "(72 B per node), whether the launch takes one step or several",
                              steps_per_launch=lbm["steps_per_launch"], avg_launch_ms=lbm["avg_launch_ms"],
-                             ms_per_step=lbm["avg_ms"]),
+                             ms_per_step=lbm["avg_ms"],
+                             per_step_72B_GBs=lbm_bytes / (lbm["avg_ms"] * 1e-3) / 1e9,
+                             per_step_72B_frac=lbm_bytes / (lbm["avg_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                             per_step_note="72 B per node update per step over the time per step (a launch "
+                                           "of several steps moves the bytes of one)"),
         "kernels": kern,
     }
-    if "plain" in res and "mixed" in res:
-        out["speedup_mixed_over_plain"] = res["plain"]["wall"] / res["mixed"]["wall"]
-        a, b = res["mixed"]["stats"][0], res["plain"]["stats"][0]
-        out["rmse_rel_diff_mixed_vs_plain"] = abs(a - b) / b
+    if "mixed" in res:
+        probes = {}
+        for pv in ("plain", "plain_tol1e-6"):
+            if pv in res:
+                out["speedup_mixed_over_" + pv] = (res[pv]["wall"] / res[pv]["K"]) / (res["mixed"]["wall"] / K)
+                ca = res[pv]["yo_index"]  # the plain run's analysis time
+                if ca not in probes:
+                    probes[ca] = rmse_probe(lb, cfg, M, ca, obs, truth, n_obs)
+                a, b = probes[ca][0], res[pv]["stats"][0]
+                out["rmse_rel_diff_mixed_vs_" + pv] = abs(a - b) / b
+                out["rmse_compared_at_cycle"] = ca
+                log(f"[rmse] cycle {ca}: mixed {a:.6e} {pv} {b:.6e} (rel {abs(a - b) / b:.2e})")
     if not args.no_uneven:
         out["uneven_batch"] = bench_uneven(lb, hip, args.uneven_steps, 1, log)
     out["closure_step"] = bench_closure_step(lb, hip, M, log)

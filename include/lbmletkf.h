@@ -333,13 +333,16 @@ int lbmletkf_autotune(lbmletkf_ctx* ctx, const char* cache_path, char* report, i
  * No counterpart in the reference (one OSSE per run).  A group of 1-8 contexts of
  * different grids and ensemble sizes (MIXED variant, grids multiples of 64 x 16,
  * one gram_passes / FP64-sweep setting) is analysed by ONE launch of each
- * analysis kernel over every gridpoint of every context, and stepped on the
- * group's stream by each context's own tuned LBM launches (a context with the
- * learned closure by its closure kernel, which needs lbmletkf_set_closure first;
- * the group calls check that before any launch); the group cycle (analysis,
- * then da_interval steps, equal in all contexts) is captured as one hipGraph.  The group works on its own stream after draining
- * the contexts' streams; do not use a member context concurrently with a group
- * call.  yo_device[k] is the observation vector (device) of the k-th context as
+ * analysis kernel over every gridpoint of every context, and stepped by each
+ * context's own tuned LBM launches (a context with the learned closure by its
+ * closure kernel, which needs lbmletkf_set_closure first; the group calls check
+ * that before any launch): the first context on the group's stream, the others
+ * on side streams of the group forked from it by an event and joined back to it
+ * (also when a launch fails), so that work ordered after a group call on the
+ * group's stream sees every context's steps; the group cycle (analysis, then
+ * da_interval steps, equal in all contexts) is captured as one hipGraph.  The
+ * group works on its own streams after draining the contexts' streams; do not use
+ * a member context concurrently with a group call.  yo_device[k] is the observation vector (device) of the k-th context as
  * passed to create.  Errors of the group calls read from lbmletkf_last_error(NULL). */
 typedef struct lbmletkf_group lbmletkf_group;
 int lbmletkf_group_create(lbmletkf_ctx* const* ctxs, int n, lbmletkf_group** out);
@@ -347,7 +350,7 @@ int lbmletkf_group_destroy(lbmletkf_group* g);
 int lbmletkf_group_forecast(lbmletkf_group* g, int n_steps);
 int lbmletkf_group_analysis(lbmletkf_group* g, const float* const* yo_device);
 int lbmletkf_group_cycle(lbmletkf_group* g, const float* const* yo_device);
-/* waits for the group's stream and reports a NaN flagged by any member analysis */
+/* waits for the group's streams and reports a NaN flagged by any member analysis */
 int lbmletkf_group_synchronize(lbmletkf_group* g);
 void* lbmletkf_group_get_stream(lbmletkf_group* g);
 

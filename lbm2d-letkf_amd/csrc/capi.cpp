@@ -355,12 +355,14 @@ int ensure_plain_buffers(Ctx* c) {
     return LBMLETKF_OK;
 }
 
-// the MIXED variant's workspace between its two kernels (allocated outside any capture)
+// the MIXED variant's workspace (allocated outside any capture): U and b between
+// the two kernels of the form with FP64 sweeps; the fused kernel needs none
 int ensure_mixed_buffers(Ctx* c) {
     const int N = mixed_width(c->M);
-    if (c->d_mix_u && c->mix_width == N) return LBMLETKF_OK;
-    HIP_TRY(c, dalloc(&c->d_mix_u, (size_t)c->nodes * N * N));
-    HIP_TRY(c, dalloc(&c->d_mix_b, (size_t)c->nodes * N));
+    if (!mixed_fused(*c) && !(c->d_mix_u && c->mix_width == N)) {
+        HIP_TRY(c, dalloc(&c->d_mix_u, (size_t)c->nodes * N * N));
+        HIP_TRY(c, dalloc(&c->d_mix_b, (size_t)c->nodes * N));
+    }
     c->mix_width = N;
     return LBMLETKF_OK;
 }
@@ -462,6 +464,16 @@ int analysis_launches(Ctx* c) {
     HIP_TRY(c, launch_obs_moments(*c, c->f[c->cur], c->d_hx, 0, c->M));
     HIP_TRY(c, launch_obs_prepare(*c));
     prof_end(c, PH_OBS, e0, 2, n_obs);
+    if (c->cfg.variant == LBMLETKF_VARIANT_MIXED && mixed_fused(*c)) {
+        // k_mixed_analysis: gather, MFMA Gram, FP32 sweeps, assembly and update in
+        // one kernel, without workspace
+        hipEvent_t e2 = prof_begin(c);
+        HIP_TRY(c, launch_letkf_mixed(*c, c->f[c->cur], c->prof ? c->d_sweeps : nullptr, nullptr,
+                                      c->cfg.gram_passes, nullptr, nullptr));
+        prof_end(c, PH_SOLVE, e2, 1, 0);
+        c->dump_kind = 1;
+        return LBMLETKF_OK;
+    }
     if (c->cfg.variant == LBMLETKF_VARIANT_MIXED) {
         // k_mixed_solve (gather + MFMA Gram + FP32 sweeps), then k_mixed_finish
         // (FP64 sweeps + assembly + update); the workspace is allocated by the caller
@@ -1225,6 +1237,7 @@ struct Group {
     ~Group() {
         drop();
         if (stream) (void)hipStreamSynchronize(stream);
+        for (hipStream_t s : side) (void)hipStreamSynchronize(s);
         for (hipStream_t s : side) (void)hipStreamDestroy(s);
         for (hipEvent_t e : ev) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
@@ -1322,24 +1335,32 @@ int group_context_steps(Group* g, Ctx* c, int n_steps) {
 int group_forecast_launches(Group* g, int n_steps) {
     const size_t n = g->cs.size();
     const bool fork = n > 1 && n_steps > 0;
-    if (fork) {
-        G_HIP(g, hipEventRecord(g->ev[0], g->stream));
-        for (size_t k = 1; k < n; ++k) {
-            G_HIP(g, hipStreamWaitEvent(g->side[k - 1], g->ev[0], 0));
-            g->cs[k]->stream = g->side[k - 1];  // StreamLend restores the context's own stream
-        }
-    }
+    size_t forked = 0;  // cs[1..forked] step on side streams that wait on ev[0]
     int st = LBMLETKF_OK;
-    for (Ctx* c : g->cs)
-        if ((st = group_context_steps(g, c, n_steps)) != LBMLETKF_OK) break;
     if (fork) {
-        for (size_t k = 1; k < n; ++k) {
-            g->cs[k]->stream = g->stream;
-            if (st == LBMLETKF_OK) {
-                G_HIP(g, hipEventRecord(g->ev[k], g->side[k - 1]));
-                G_HIP(g, hipStreamWaitEvent(g->stream, g->ev[k], 0));
+        hipError_t e = hipEventRecord(g->ev[0], g->stream);
+        for (size_t k = 1; e == hipSuccess && k < n; ++k) {
+            e = hipStreamWaitEvent(g->side[k - 1], g->ev[0], 0);
+            if (e == hipSuccess) {
+                g->cs[k]->stream = g->side[k - 1];  // StreamLend restores the context's own stream
+                forked = k;
             }
         }
+        if (e != hipSuccess)
+            st = gfail(g, LBMLETKF_ERR_HIP, std::string("forking the group's forecast: ") + hipGetErrorString(e));
+    }
+    if (st == LBMLETKF_OK)
+        for (Ctx* c : g->cs)
+            if ((st = group_context_steps(g, c, n_steps)) != LBMLETKF_OK) break;
+    // Every forked side stream is joined back, also after a failed launch: a capture
+    // ends only with all its streams joined, and eager work left on a side stream
+    // would not be covered by a wait on the group's stream.
+    for (size_t k = 1; k <= forked; ++k) {
+        g->cs[k]->stream = g->stream;
+        hipError_t e = hipEventRecord(g->ev[k], g->side[k - 1]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(g->stream, g->ev[k], 0);
+        if (e != hipSuccess && st == LBMLETKF_OK)
+            st = gfail(g, LBMLETKF_ERR_HIP, std::string("joining the group's forecast: ") + hipGetErrorString(e));
     }
     return st;
 }
@@ -1503,6 +1524,7 @@ int lbmletkf_group_synchronize(lbmletkf_group* grp) {
     Group* g = reinterpret_cast<Group*>(grp);
     if (!g) return fail(nullptr, LBMLETKF_ERR_ARG, "group is NULL");
     G_HIP(g, hipStreamSynchronize(g->stream));
+    for (hipStream_t s : g->side) G_HIP(g, hipStreamSynchronize(s));
     for (Ctx* c : g->cs)
         if (int st = sync_and_check(c)) return gfail(g, st, c->last_error);
     return LBMLETKF_OK;
@@ -1584,7 +1606,7 @@ std::string tune_key(const Ctx* c) {
     return dev + "/" + std::to_string(c->nx) + "x" + std::to_string(c->ny) + "/M" + std::to_string(c->M) + "/v" +
            std::to_string(g.variant) + "/p" + std::to_string(g.gram_passes) + "/f" +
            std::to_string(g.jacobi_fp64_sweeps > 0) + "/s" + std::to_string(c->obs.n_sites) + "/e" +
-           std::to_string(c->obs.n_entries) + "/t2";  // t2: the candidate set (two-step strips, chain 32)
+           std::to_string(c->obs.n_entries) + "/t3";  // t3: the candidate set and the fused analysis (column chains)
 }
 
 }  // namespace

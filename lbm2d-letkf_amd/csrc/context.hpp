@@ -108,7 +108,8 @@ struct Ctx {
     float* d_dyo = nullptr;       // [n_obs] innovation yo - ybar
     float* d_gram = nullptr;      // plain: [nodes][M][M] C matrices
     float* d_bvec = nullptr;      // plain: [nodes][M] dY^T R^-1 dyo
-    // MIXED workspace between its two kernels (N = mix_width, M padded to 16/32/64)
+    // MIXED workspace of the two-kernel form (FP64 sweeps requested; N = mix_width,
+    // M padded to 16/32/64); the fused form of the presets needs none
     float* d_mix_u = nullptr;     // [nodes][N][N] FP32 columns of U = C V after the FP32 sweeps
     float* d_mix_b = nullptr;     // [nodes][N] b = dY^T R^-1 dyo
     int mix_width = 0;
@@ -200,8 +201,12 @@ hipError_t launch_solve_fp64(Ctx& c, float* f, unsigned long long* sweeps);
 
 // ---- launchers (letkf_mixed.hip) ----------------------------------------------
 int mixed_width(int M);
-// ev_mid_end / ev_mid_begin (may be null) are recorded between the two kernels;
-// solve_only launches the first kernel alone (workspace only, the state untouched)
+// the presets' form: one fused kernel (no FP64 sweeps requested)
+bool mixed_fused(const Ctx& c);
+long long mixed_fused_blocks(int nx, int ny, int chain);
+// Fused form: one kernel; the events are not used; solve_only skips the state update.
+// Two-kernel form: ev_mid_end / ev_mid_begin (may be null) are recorded between the
+// two kernels; solve_only launches the first kernel alone (the state untouched).
 hipError_t launch_letkf_mixed(Ctx& c, float* f, unsigned long long* sweeps32,
                               unsigned long long* sweeps64, int passes, hipEvent_t ev_mid_end,
                               hipEvent_t ev_mid_begin, bool solve_only = false);

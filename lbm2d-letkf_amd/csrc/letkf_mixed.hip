@@ -42,6 +42,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 struct MixedParams {
     int M, nodes, n_vars;
+    int nx, ny;             // grid (k_mixed_analysis walks its chains along columns)
     long long member_stride;
     float diag;             // (M-1)/beta
     float coef[3];          // R^-1 per observed variable
@@ -51,6 +52,7 @@ struct MixedParams {
     int max32, max64;
     int dump_g0, dump_n;
     int chain;              // k_mixed_solve: consecutive gridpoints per wavefront
+    int write_state;        // k_mixed_analysis: 0 skips the state update (timing only)
     unsigned long long* sweeps32;
     unsigned long long* sweeps64;
 };
@@ -320,18 +322,31 @@ __device__ inline void hestenes_renorm(T (&u)[N], T& sc, T& al) {
 }
 
 // ---------------------------------------------------------------------------------
-// gather + MFMA Gram of gridpoint g: C (full, row stride N) into LDS, b into ws_b
+// gather + MFMA Gram of gridpoint g into accumulators: G^T G in the tiles (I, J) of C,
+// all NT x NT of them (FULL, tile I NT + J) or those with I <= J (the upper
+// triangle, in row order), then b = G^T sqrt(R^-1) dyo in the NT tiles after them.
+// Tile (I, J): lane (col, kq) holds rows 16 I + 4 kq + r, column 16 J + col.
 // ---------------------------------------------------------------------------------
-template <int N, int PASSES>
-__device__ inline void gram_to_lds(int g, int lane, const int* __restrict__ loc_ptr,
-                                   const int* __restrict__ loc_site, const float* __restrict__ loc_w,
-                                   const float* __restrict__ dy, const float* __restrict__ dyo,
-                                   float* __restrict__ ws_b, const MixedParams& mp, float* cfull) {
-    constexpr int NT = N / 16;              // member tiles of 16
-    constexpr int NTILE = NT * (NT + 1) / 2; // upper-triangular tiles of C
+template <int N, bool FULL>
+struct GramTiles {
+    static constexpr int NT = N / 16;
+    static constexpr int C = FULL ? NT * NT : NT * (NT + 1) / 2;
+    static constexpr int ALL = C + NT;
+};
+// accumulator count of the FULL Gram (a name without a comma, for array bounds in
+// parameter declarations)
+template <int N>
+constexpr int GRAM_FULL = GramTiles<N, true>::ALL;
+
+template <int N, int PASSES, bool FULL>
+__device__ __forceinline__ void gram_acc(int g, int lane, const int* __restrict__ loc_ptr,
+                                         const int* __restrict__ loc_site, const float* __restrict__ loc_w,
+                                         const float* __restrict__ dy, const float* __restrict__ dyo,
+                                         const MixedParams& mp, f32x4 (&acc)[GramTiles<N, FULL>::ALL]) {
+    constexpr int NT = N / 16;                       // member tiles of 16
+    constexpr int NTILE = GramTiles<N, FULL>::C;     // tiles of C
     const int M = mp.M;
     const int nv = mp.n_vars;
-    f32x4 acc[NTILE + NT];
 #pragma unroll
     for (int a = 0; a < NTILE + NT; ++a) acc[a] = f32x4{0.f, 0.f, 0.f, 0.f};
     const int e0 = loc_ptr[g], e1 = loc_ptr[g + 1];
@@ -394,7 +409,7 @@ __device__ inline void gram_to_lds(int g, int lane, const int* __restrict__ loc_
 #pragma unroll
         for (int I = 0; I < NT; ++I) {
 #pragma unroll
-            for (int J = I; J < NT; ++J, ++tix) {
+            for (int J = FULL ? 0 : I; J < NT; ++J, ++tix) {
                 acc[tix] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(hi[I], hi[J], acc[tix], 0, 0, 0);
                 if (PASSES == 3) {
                     acc[tix] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(hi[I], lo[J], acc[tix], 0, 0, 0);
@@ -411,6 +426,20 @@ __device__ inline void gram_to_lds(int g, int lane, const int* __restrict__ loc_
             }
         }
     }
+}
+
+// gather + MFMA Gram of gridpoint g: C (full, row stride N) into LDS, b into bg[0, N)
+template <int N, int PASSES>
+__device__ inline void gram_to_lds(int g, int lane, const int* __restrict__ loc_ptr,
+                                   const int* __restrict__ loc_site, const float* __restrict__ loc_w,
+                                   const float* __restrict__ dy, const float* __restrict__ dyo,
+                                   float* __restrict__ bg, const MixedParams& mp, float* cfull) {
+    constexpr int NT = N / 16;
+    constexpr int NTILE = GramTiles<N, false>::C;
+    const int M = mp.M;
+    const int col = lane & 15, kq = lane >> 4;
+    f32x4 acc[GramTiles<N, false>::ALL];
+    gram_acc<N, PASSES, false>(g, lane, loc_ptr, loc_site, loc_w, dy, dyo, mp, acc);
     // result tile (I,J), I <= J: lane holds rows 16I + 4*kq + r, column 16J + col;
     // the tiles below the diagonal are the mirrors of those above
     int tix = 0;
@@ -428,7 +457,6 @@ __device__ inline void gram_to_lds(int g, int lane, const int* __restrict__ loc_
             }
         }
     }
-    float* bg = ws_b + (long long)g * N;
 #pragma unroll
     for (int I = 0; I < NT; ++I)
         if (col == 0) {
@@ -446,8 +474,19 @@ __device__ inline void gram_to_lds(int g, int lane, const int* __restrict__ loc_
 // of columns 16 tj + lane % 16, so the column norms come from its squares summed
 // over the four lanes of a column (the same sum in all four).  U0 replaces C in
 // cfull (row-major) after every read of C is done.
+// V_prev row-major (up[k N + n] = V[k][n]); the loads of row k give the lane
+// columns 16 tj + col
 template <int N>
-__device__ inline void warm_start(const float* __restrict__ up, float* cfull, int lane) {
+struct RowMajorRows {
+    const float* __restrict__ up;
+    int col;
+    __device__ void operator()(int k, float (&b)[N / 16]) const {
+#pragma unroll
+        for (int tj = 0; tj < N / 16; ++tj) b[tj] = up[k * N + 16 * tj + col];
+    }
+};
+template <int N, class Rows>
+__device__ inline void warm_start(Rows rows, float* cfull, int lane) {
     constexpr int NT = N / 16;
     const int col = lane & 15, kq = lane >> 4;
     f32x4 acc[NT * NT];
@@ -462,11 +501,9 @@ __device__ inline void warm_start(const float* __restrict__ up, float* cfull, in
         float a[NT], b[NT];
 #pragma unroll
         for (int ti = 0; ti < NT; ++ti) a[ti] = cfull[k * N + 16 * ti + col];  // C symmetric: no bank conflicts
+        rows(k, b);
 #pragma unroll
-        for (int tj = 0; tj < NT; ++tj) {
-            b[tj] = up[k * N + 16 * tj + col];
-            nrm[tj] = fmaf(b[tj], b[tj], nrm[tj]);
-        }
+        for (int tj = 0; tj < NT; ++tj) nrm[tj] = fmaf(b[tj], b[tj], nrm[tj]);
 #pragma unroll
         for (int ti = 0; ti < NT; ++ti)
 #pragma unroll
@@ -490,6 +527,52 @@ __device__ inline void warm_start(const float* __restrict__ up, float* cfull, in
                 cfull[(16 * ti + 4 * kq + r) * N + 16 * tj + col] = acc[ti * NT + tj][r] * inv[tj];
 }
 
+// U0 (C, or the warm start) from LDS into the registers (lane j = column j; C is
+// symmetric, so rows and columns read alike), then the FP32 one-sided sweeps.  On
+// return the columns are sc * u[] and the matrix U = uscale * sc * u (uscale a power
+// of two that keeps the squared column norms in range).
+template <int N>
+__device__ __forceinline__ float jacobi_fp32(float (&u)[N], float& sc, const float* cfull, int lane,
+                                             const MixedParams& mp) {
+    const int l = lane & (N - 1);
+    float a0 = 0.f;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        const float x = cfull[i * N + l];
+        u[i] = (lane < N) ? x : 0.f;
+        a0 = fmaf(u[i], u[i], a0);
+    }
+    // keep the squared column norms below 1e18 (the range of the rotation
+    // formula): scale U by 2^-32 when needed, exactly, and undo it on output
+    float uscale = 1.f;
+    if (wave_max(a0) > 1e18f) {
+        uscale = 0x1p32f;
+#pragma unroll
+        for (int i = 0; i < N; ++i) u[i] *= 0x1p-32f;
+    }
+
+    // ---- 3. FP32 one-sided Jacobi ------------------------------------------------
+    // lanes >= N (N < 64) hold zero columns: their pair Gram is zero, they never rotate
+    // Stop at the tolerance, or when the largest cosine below 1e-4 no longer
+    // halves from one sweep to the next: that is the FP32 rounding floor of
+    // this matrix (convergence, even slowed by clustered eigenvalues, keeps
+    // dividing it by more than 2).
+    float al = 0.f, mc_prev = 1.f;
+    sc = 1.f;
+    int sw32 = 0;
+    while (sw32 < mp.max32) {
+        hestenes_renorm<N, float>(u, sc, al);
+        float isc = 1.f;
+        float mc = hestenes_sweep<N>(u, sc, isc, al, lane);
+        mc = wave_max(mc);
+        ++sw32;
+        if (mc <= mp.tol32 || (mc <= 1e-4f && mc > 0.5f * mc_prev)) break;
+        mc_prev = mc;
+    }
+    if (lane == 0 && mp.sweeps32) atomicAdd(mp.sweeps32, (unsigned long long)sw32);
+    return uscale;
+}
+
 // ---------------------------------------------------------------------------------
 // kernel 1: gather, MFMA Gram, FP32 sweeps for a chain of mp.chain consecutive
 // gridpoints.  The first starts from U = C; each later one from U = C V_prev, the
@@ -508,53 +591,19 @@ __device__ __forceinline__ void solve_body(int bx, int nbx, const int* __restric
                                            const MixedParams& mp, float* cfull) {
     // cfull: N*N floats of LDS, 16-byte aligned
     const int lane = threadIdx.x;
-    const int l = lane & (N - 1);
     const int g_begin = xcd_order(bx, nbx) * mp.chain;
     const int g_end = min(g_begin + mp.chain, mp.nodes);
     for (int g = g_begin; g < g_end; ++g) {
         __syncthreads();  // the previous gridpoint's reads of cfull are done
-        gram_to_lds<N, PASSES>(g, lane, loc_ptr, loc_site, loc_w, dy, dyo, ws_b, mp, cfull);
+        gram_to_lds<N, PASSES>(g, lane, loc_ptr, loc_site, loc_w, dy, dyo, ws_b + (long long)g * N, mp, cfull);
         __syncthreads();
         float u[N];
         if (g > g_begin) {
-            warm_start<N>(ws_u + (long long)(g - 1) * N * N, cfull, lane);
+            warm_start<N>(RowMajorRows<N>{ws_u + (long long)(g - 1) * N * N, lane & 15}, cfull, lane);
             __syncthreads();
         }
-        // column `lane` (C symmetric, so row and column read alike)
-        float a0 = 0.f;
-#pragma unroll
-        for (int i = 0; i < N; ++i) {
-            const float x = cfull[i * N + l];
-            u[i] = (lane < N) ? x : 0.f;
-            a0 = fmaf(u[i], u[i], a0);
-        }
-        // keep the squared column norms below 1e18 (the range of the rotation
-        // formula): scale U by 2^-32 when needed, exactly, and undo it on output
-        float uscale = 1.f;
-        if (wave_max(a0) > 1e18f) {
-            uscale = 0x1p32f;
-#pragma unroll
-            for (int i = 0; i < N; ++i) u[i] *= 0x1p-32f;
-        }
-
-        // ---- 3. FP32 one-sided Jacobi ------------------------------------------------
-        // lanes >= N (N < 64) hold zero columns: their pair Gram is zero, they never rotate
-        // Stop at the tolerance, or when the largest cosine below 1e-4 no longer
-        // halves from one sweep to the next: that is the FP32 rounding floor of
-        // this matrix (convergence, even slowed by clustered eigenvalues, keeps
-        // dividing it by more than 2).
-        float sc = 1.f, al = 0.f, mc_prev = 1.f;
-        int sw32 = 0;
-        while (sw32 < mp.max32) {
-            hestenes_renorm<N, float>(u, sc, al);
-            float isc = 1.f;
-            float mc = hestenes_sweep<N>(u, sc, isc, al, lane);
-            mc = wave_max(mc);
-            ++sw32;
-            if (mc <= mp.tol32 || (mc <= 1e-4f && mc > 0.5f * mc_prev)) break;
-            mc_prev = mc;
-        }
-        if (lane == 0 && mp.sweeps32) atomicAdd(mp.sweeps32, (unsigned long long)sw32);
+        float sc;
+        const float uscale = jacobi_fp32<N>(u, sc, cfull, lane, mp);
         if (lane < N) {
             float* ug = ws_u + (long long)g * N * N;
             const float so = sc * uscale;
@@ -571,6 +620,292 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     float* __restrict__ ws_b, MixedParams mp) {
     __shared__ __align__(16) float cfull[N * N];
     solve_body<N, PASSES>(blockIdx.x, gridDim.x, loc_ptr, loc_site, loc_w, dy, dyo, ws_u, ws_b, mp, cfull);
+}
+
+// ---------------------------------------------------------------------------------
+// The fused analysis (the presets: no FP64 sweeps).  The gather, Gram and FP32
+// sweeps of k_mixed_solve, then in the same wavefront the eigenpairs and the update
+// of the gridpoint's members:
+//   lambda_i = ||u_i||, v_i = u_i / lambda_i          (column norms in FP64)
+//   s_i = sqrt(M-1) / sqrt(lambda_i), h_i = (V^T b)_i / lambda_i   (FP64)
+//   Y = [dX; b^T] V                                   (FP32 matrix cores)
+//   Xa[q][l] = xbar[q] + sum_i Y[q][i] (s_i V[l][i] + h_i)   (FP32 matrix cores)
+// which is Xa = xbar 1^T + dX (W + w 1^T) with W = sqrt(M-1) V D^-1/2 V^T and
+// w = V D^-1 V^T b (letkf_solver.h:288-316), since dX w = dX V D^-1 V^T b = Y h.
+// The next gridpoint of the chain takes V from a per-workgroup slot in global
+// memory (its warm start); no other intermediate leaves the CU.
+// ---------------------------------------------------------------------------------
+constexpr int ASM_ROWS = Q + 1;  // dX rows and b^T: the A operand of the first product
+
+template <int N>
+struct FusedShared {
+    float cv[N * N];             // C, then U0, then V with swizzled rows (vswz)
+    float stage[N * ASM_ROWS];   // [k][ASM_ROWS]: dX^T and b (first product), then Y^T (second)
+    float bvec[N];               // b = G^T sqrt(R^-1) dyo from the Gram phase
+    float coef[2][N];            // s_i, h_i
+};
+
+// Element (k, i) of V sits at cv[k N + (i ^ vswz(k))].  The MFMA operand reads of
+// the warm start (lane (col, kq) reading rows k = 16 I + 4 kq + r, columns 16 tj +
+// col), of the first product (rows 4 s + grp, columns 16 T + c) and of the second
+// (rows 16 T + c, columns 4 s + grp), and the writes of a row by lane = column,
+// each meet 64 distinct banks: vswz puts bits 2-3 of k into bits 2-3 and
+// (bits 0-1 XOR bits 2-3) of k into bits 4-5.
+template <int N>
+__device__ inline int vswz(int k) {
+    return (4 * ((k >> 2) & 3) + 16 * ((k & 3) ^ ((k >> 2) & 3))) & (N - 1);
+}
+
+// C with its diagonal from the FULL Gram accumulators: (M-1)/beta on the members'
+// diagonal, 1 on the padding columns (a decoupled identity block)
+template <int N>
+__device__ inline void gram_add_diag(f32x4 (&acc)[GRAM_FULL<N>], int lane, int M, float diag) {
+    constexpr int NT = N / 16;
+    const int col = lane & 15, kq = lane >> 4;
+#pragma unroll
+    for (int I = 0; I < NT; ++I)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            if (4 * kq + r == col) acc[I * NT + I][r] += (16 * I + col < M) ? diag : 1.f;
+}
+
+// C from the FULL accumulators into cv, row-major (the first gridpoint of a chain)
+template <int N>
+__device__ inline void gram_to_cv(const f32x4 (&acc)[GRAM_FULL<N>], float* cv, int lane) {
+    constexpr int NT = N / 16;
+    const int col = lane & 15, kq = lane >> 4;
+#pragma unroll
+    for (int I = 0; I < NT; ++I)
+#pragma unroll
+        for (int J = 0; J < NT; ++J)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) cv[(16 * I + 4 * kq + r) * N + 16 * J + col] = acc[I * NT + J][r];
+}
+
+// Warm start of a chain's later gridpoints without leaving the CU: U0 = C V_prev on
+// the FP32 matrix cores (v_mfma_f32_16x16x4f32), with C read from the Gram's FULL
+// accumulators as the A operand and V_prev, the previous gridpoint's eigenvectors
+// (unit columns), from cv where its update left them.  The sum over k runs in the
+// order k = 16 (s >> 2) + 4 kq + (s & 3) for K-step s and lane group kq, so that
+// the A value C[16 ti + col][k] = C[k][16 ti + col] (C is symmetric) is register
+// s & 3 of the lane's own accumulator tile (s >> 2, ti): no data moves.  MFMA lane
+// layout: A[m][kk] from lane (m = lane % 16, kk = lane / 16), B[kk][n] from lane
+// (kk = lane / 16, n = lane % 16), D[4 (lane / 16) + r][lane % 16].  U0 then
+// replaces V_prev in cv (row-major).
+template <int N>
+__device__ inline void warm_start_acc(const f32x4 (&acc)[GRAM_FULL<N>], float* cv, int lane) {
+    constexpr int NT = N / 16;
+    // the swizzled read addresses are formed from an opaque copy of the lane index:
+    // hoisted out of the chain loop, they would stay live across the sweeps and spill
+    int ln = lane;
+    asm volatile("" : "+v"(ln));
+    const int col = ln & 15, kq = ln >> 4;
+    f32x4 d[NT * NT];
+#pragma unroll
+    for (int a = 0; a < NT * NT; ++a) d[a] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < N / 4; ++s) {
+        const int k = 16 * (s >> 2) + 4 * kq + (s & 3);
+        float b[NT];
+#pragma unroll
+        for (int tj = 0; tj < NT; ++tj) b[tj] = cv[k * N + ((16 * tj + col) ^ vswz<N>(k))];
+#pragma unroll
+        for (int ti = 0; ti < NT; ++ti)
+#pragma unroll
+            for (int tj = 0; tj < NT; ++tj)
+                d[ti * NT + tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(acc[(s >> 2) * NT + ti][s & 3], b[tj],
+                                                                       d[ti * NT + tj], 0, 0, 0);
+    }
+    __syncthreads();  // every read of V_prev is done
+#pragma unroll
+    for (int ti = 0; ti < NT; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < NT; ++tj)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) cv[(16 * ti + 4 * kq + r) * N + 16 * tj + col] = d[ti * NT + tj][r];
+}
+
+template <int N, int PASSES>
+__device__ __forceinline__ void analysis_body(int bx, int nbx, const int* __restrict__ loc_ptr,
+                                              const int* __restrict__ loc_site, const float* __restrict__ loc_w,
+                                              const float* __restrict__ dy, const float* __restrict__ dyo,
+                                              float* __restrict__ f, const MixedParams& mp,
+                                              double* __restrict__ dump, int* __restrict__ status,
+                                              FusedShared<N>& sh) {
+    constexpr int NT = N / 16;
+    const int lane = threadIdx.x;
+    const int M = mp.M;
+    // Chain blk covers rows [gy0, gy1) of column gx: the chains that run at the same
+    // time on one XCD (consecutive blk) are neighbouring columns of the same rows,
+    // so they share the cache lines of the member states (a line holds 32 columns
+    // of one member and population) instead of each holding its own
+    const int blk = xcd_order(bx, nbx);
+    const int gx = blk % mp.nx;
+    const int gy0 = (blk / mp.nx) * mp.chain;
+    const int gy1 = min(gy0 + mp.chain, mp.ny);
+    bool bad = false;
+    for (int gy = gy0; gy < gy1; ++gy) {
+        const int g = gy * mp.nx + gx;
+        f32x4 acc[GRAM_FULL<N>];
+        gram_acc<N, PASSES, true>(g, lane, loc_ptr, loc_site, loc_w, dy, dyo, mp, acc);
+        gram_add_diag<N>(acc, lane, M, mp.diag);
+        __syncthreads();  // the previous gridpoint's reads of stage and coef are done
+        if ((lane & 15) == 0) {
+#pragma unroll
+            for (int I = 0; I < NT; ++I)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) sh.bvec[16 * I + 4 * (lane >> 4) + r] = acc[NT * NT + I][r];
+        }
+        // U0: C itself for the first gridpoint of the chain, else C V_prev
+        if (gy == gy0)
+            gram_to_cv<N>(acc, sh.cv, lane);
+        else
+            warm_start_acc<N>(acc, sh.cv, lane);
+        __syncthreads();
+        float u[N];
+        float sc;
+        const float uscale = jacobi_fp32<N>(u, sc, sh.cv, lane, mp);
+
+        // ---- eigenpairs: lambda_i = ||u_i|| in FP64, v_i = u_i / ||u_i|| ------------------
+        double n4[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            u[i] *= sc;
+            n4[i & 3] = fma(double(u[i]), double(u[i]), n4[i & 3]);
+        }
+        const bool col = lane < N;
+        const double nrm = sqrt((n4[0] + n4[1]) + (n4[2] + n4[3]));
+        const double lam = (col && nrm > 0.0) ? nrm * double(uscale) : 1.0;
+        {
+            const float inv = (col && nrm > 0.0) ? float(1.0 / nrm) : 0.f;
+#pragma unroll
+            for (int i = 0; i < N; ++i) u[i] *= inv;
+        }
+        // lane-dependent LDS addresses of the assembly are formed from an opaque copy of
+        // the lane index, so that they are not hoisted out of the chain loop (held
+        // across the sweeps they would spill)
+        int ln = lane;
+        asm volatile("" : "+v"(ln));
+        const int c = ln & 15, grp = ln >> 4;  // MFMA layout
+        __syncthreads();  // every read of U0 in cv is done
+        if (col) {
+#pragma unroll
+            for (int k = 0; k < N; ++k) sh.cv[k * N + (ln ^ vswz<N>(k))] = u[k];
+        }
+        const bool dumping = (g >= mp.dump_g0) && (g < mp.dump_g0 + mp.dump_n) && mp.write_state;
+        if (dumping && lane < M) {
+            // V[k][lane]: the lane's own column read back from cv and normalised again in
+            // FP64, so that the dumped columns have unit norm to FP64 precision (rolled
+            // loops: the dump is rare and its addresses must not be hoisted)
+            double* dg = dump + (long long)(g - mp.dump_g0) * (M * M + 2 * M);
+            double vv = 0.0;
+#pragma unroll 1
+            for (int k = 0; k < N; ++k) {
+                const double x = double(sh.cv[k * N + (ln ^ vswz<N>(k))]);
+                vv = fma(x, x, vv);
+            }
+            const double iv = vv > 0.0 ? 1.0 / sqrt(vv) : 0.0;
+#pragma unroll 1
+            for (int k = 0; k < M; ++k) dg[k * M + lane] = double(sh.cv[k * N + (ln ^ vswz<N>(k))]) * iv;
+            dg[M * M + lane] = lam;
+            dg[M * M + M + lane] = double(sh.bvec[lane]);
+        }
+        // ---- member perturbations: lane = member --------------------------------------
+        const int member = lane < M ? lane : 0;
+        const long long base = (long long)member * mp.member_stride + g;
+        double xbar[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const float x = f[base + (long long)q * mp.nodes];
+            const double xd = lane < M ? double(x) : 0.0;
+            xbar[q] = wave_sum_d(xd) / double(M);
+            if (col) sh.stage[lane * ASM_ROWS + q] = lane < M ? float(xd - xbar[q]) : 0.f;
+        }
+        if (col) sh.stage[lane * ASM_ROWS + Q] = sh.bvec[lane];
+        __syncthreads();
+        // ---- first product: Y[r][i] = sum_k A[r][k] V[k][i], A = [dX; b^T] ------------------
+        // lane (c, grp) holds Y[4 grp + r][16 T + c]
+        f32x4 y[NT];
+#pragma unroll
+        for (int T = 0; T < NT; ++T) y[T] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < N / 4; ++s) {
+            const int k = 4 * s + grp;
+            const float a = c < ASM_ROWS ? sh.stage[k * ASM_ROWS + c] : 0.f;
+#pragma unroll
+            for (int T = 0; T < NT; ++T) {
+                const float b = sh.cv[k * N + ((16 * T + c) ^ vswz<N>(k))];
+                y[T] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, y[T], 0, 0, 0);
+            }
+        }
+        // ---- coefficients s_i, h_i (FP64), lane i -------------------------------------------
+        // (V^T b)_i = Y[Q][i] sits in lane (i % 16, grp = Q / 4), element Q % 4 of tile i / 16
+        {
+            float vb = 0.f;
+#pragma unroll
+            for (int T = 0; T < NT; ++T) {
+                const float t = __shfl(y[T][Q % 4], (Q / 4) * 16 + (lane & 15));
+                vb = (lane >> 4) == T ? t : vb;
+            }
+            if (col) {
+                sh.coef[0][lane] = float(mp.sqm1 / sqrt(lam));
+                sh.coef[1][lane] = float(double(vb) / lam);
+            }
+        }
+        __syncthreads();  // the first product's reads of stage are done; coef written
+        // Y^T into stage (rows q < Q only)
+#pragma unroll
+        for (int T = 0; T < NT; ++T)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int q = 4 * grp + r;
+                if (q < Q) sh.stage[(16 * T + c) * ASM_ROWS + q] = y[T][r];
+            }
+        __syncthreads();
+        // ---- second product: P[q][l] = sum_i Y[q][i] (s_i V[l][i] + h_i) ---------------------
+        f32x4 p[NT];
+#pragma unroll
+        for (int T = 0; T < NT; ++T) p[T] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < N / 4; ++s) {
+            const int i = 4 * s + grp;
+            const float a = c < Q ? sh.stage[i * ASM_ROWS + c] : 0.f;
+            const float si = sh.coef[0][i], hi = sh.coef[1][i];
+#pragma unroll
+            for (int T = 0; T < NT; ++T) {
+                const int l = 16 * T + c;
+                const float b = fmaf(si, sh.cv[l * N + (i ^ vswz<N>(l))], hi);
+                p[T] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, p[T], 0, 0, 0);
+            }
+        }
+        // ---- Xa = xbar + P: lane (c, grp) holds members 16 T + c, variables 4 grp + r --------
+        if (mp.write_state) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int q = 4 * grp + r;
+                const double xb = grp == 0 ? xbar[r] : (grp == 1 ? xbar[4 + r] : (r == 0 ? xbar[8] : 0.0));
+#pragma unroll
+                for (int T = 0; T < NT; ++T) {
+                    const int l = 16 * T + c;
+                    if (q < Q && l < M) {
+                        const float xa = float(xb + double(p[T][r]));
+                        bad |= !isfinite(xa);
+                        f[(long long)l * mp.member_stride + (long long)q * mp.nodes + g] = xa;
+                    }
+                }
+            }
+        }
+    }
+    if (bad) atomicOr(status, 1);
+}
+
+template <int N, int PASSES>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_mixed_analysis(
+    const int* __restrict__ loc_ptr, const int* __restrict__ loc_site, const float* __restrict__ loc_w,
+    const float* __restrict__ dy, const float* __restrict__ dyo, float* __restrict__ f, MixedParams mp,
+    double* __restrict__ dump, int* __restrict__ status) {
+    __shared__ __align__(16) FusedShared<N> sh;
+    analysis_body<N, PASSES>(blockIdx.x, gridDim.x, loc_ptr, loc_site, loc_w, dy, dyo, f, mp, dump, status, sh);
 }
 
 // ---------------------------------------------------------------------------------
@@ -852,6 +1187,35 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     }
 }
 
+template <int PASSES>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_mixed_analysis_group(MixedGroup G) {
+    int k = 0;
+    while (k + 1 < G.n && int(blockIdx.x) >= G.t[k + 1].b0_solve) ++k;
+    const MixedTask& T = G.t[k];
+    const int bx = int(blockIdx.x) - T.b0_solve;
+    // one buffer for every width (the layouts of smaller widths fit inside N = 64's)
+    __shared__ __align__(16) FusedShared<64> sh;
+    const int* lp = T.loc_ptr;
+    const int* ls = T.loc_site;
+    const float* lw = T.loc_w;
+    const float* ty = T.dy;
+    const float* to = T.dyo;
+    float* tf = T.f;
+    double* td = T.dump;
+    int* ts = T.status;
+    switch (T.N) {
+        case 16:
+            analysis_body<16, PASSES>(bx, T.nb_solve, lp, ls, lw, ty, to, tf, T.mp, td, ts,
+                                      *reinterpret_cast<FusedShared<16>*>(&sh));
+            break;
+        case 32:
+            analysis_body<32, PASSES>(bx, T.nb_solve, lp, ls, lw, ty, to, tf, T.mp, td, ts,
+                                      *reinterpret_cast<FusedShared<32>*>(&sh));
+            break;
+        default: analysis_body<64, PASSES>(bx, T.nb_solve, lp, ls, lw, ty, to, tf, T.mp, td, ts, sh); break;
+    }
+}
+
 template <bool SWEEP64>
 __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3))) void k_mixed_finish_group(MixedGroup G) {
     int k = 0;
@@ -873,6 +1237,8 @@ MixedParams mixed_params(const Ctx& c, unsigned long long* sweeps32, unsigned lo
     MixedParams mp{};
     mp.M = c.M;
     mp.nodes = int(c.nodes);
+    mp.nx = c.nx;
+    mp.ny = c.ny;
     mp.n_vars = c.obs.n_vars;
     mp.member_stride = c.lp.member_stride;
     mp.diag = float(double(c.M - 1) / c.cfg.inflation);
@@ -891,6 +1257,7 @@ MixedParams mixed_params(const Ctx& c, unsigned long long* sweeps32, unsigned lo
     mp.sweeps32 = sweeps32;
     mp.sweeps64 = sweeps64;
     mp.chain = c.mix_chain;
+    mp.write_state = 1;
     return mp;
 }
 
@@ -902,12 +1269,41 @@ int mixed_width(int M) {
     return N;
 }
 
+bool mixed_fused(const Ctx& c) { return c.cfg.jacobi_fp64_sweeps <= 0; }
+
+// workgroups of k_mixed_analysis: one per chain of `chain` rows of one column
+long long mixed_fused_blocks(int nx, int ny, int chain) { return (long long)nx * ((ny + chain - 1) / chain); }
+
 hipError_t launch_letkf_mixed(Ctx& c, float* f, unsigned long long* sweeps32, unsigned long long* sweeps64,
                               int passes, hipEvent_t ev_mid_end, hipEvent_t ev_mid_begin, bool solve_only) {
     const int N = mixed_width(c.M);
     if (!c.d_loc_ptr || !c.d_dy || !c.d_dyo || !f || (c.dump_n > 0 && !c.d_dump)) return hipErrorInvalidValue;
-    if (!c.d_mix_u || !c.d_mix_b || c.mix_width != N) return hipErrorInvalidValue;
+    if (c.mix_width != N || c.mix_chain < 1) return hipErrorInvalidValue;
     MixedParams mp = mixed_params(c, sweeps32, sweeps64);
+    if (mixed_fused(c)) {
+        // one kernel, no workspace
+        const long long nb = mixed_fused_blocks(c.nx, c.ny, mp.chain);
+        if (solve_only) {
+            mp.write_state = 0;
+            mp.dump_n = 0;
+        }
+        const dim3 grid_a{unsigned(nb), 1u, 1u};
+#define LAUNCH_FUSED(NN, PP)                                                                                   \
+    k_mixed_analysis<NN, PP><<<grid_a, dim3(64), 0, c.stream>>>(c.d_loc_ptr, c.d_loc_site, c.d_loc_w, c.d_dy,  \
+                                                                c.d_dyo, f, mp, c.d_dump, c.d_status)
+        switch (N * 10 + (passes == 3 ? 3 : 1)) {
+            case 163: LAUNCH_FUSED(16, 3); break;
+            case 161: LAUNCH_FUSED(16, 1); break;
+            case 323: LAUNCH_FUSED(32, 3); break;
+            case 321: LAUNCH_FUSED(32, 1); break;
+            case 643: LAUNCH_FUSED(64, 3); break;
+            case 641: LAUNCH_FUSED(64, 1); break;
+            default: return hipErrorInvalidValue;
+        }
+#undef LAUNCH_FUSED
+        return hipGetLastError();
+    }
+    if (!c.d_mix_u || !c.d_mix_b) return hipErrorInvalidValue;
     const dim3 grid(unsigned(c.nodes)), block(64), block2(128);
     const dim3 grid_solve(unsigned((c.nodes + mp.chain - 1) / mp.chain));
     const bool sweep64 = mp.max64 > 0;
@@ -956,9 +1352,9 @@ hipError_t launch_letkf_mixed_group(Ctx* const* cs, int n, hipStream_t stream) {
     for (int k = 0; k < n; ++k) {
         Ctx& c = *cs[k];
         const int N = mixed_width(c.M);
-        if (!c.d_loc_ptr || !c.d_dy || !c.d_dyo || !c.d_mix_u || !c.d_mix_b || c.mix_width != N ||
-            (c.dump_n > 0 && !c.d_dump))
+        if (!c.d_loc_ptr || !c.d_dy || !c.d_dyo || c.mix_width != N || (c.dump_n > 0 && !c.d_dump))
             return hipErrorInvalidValue;
+        if (sweep64 && (!c.d_mix_u || !c.d_mix_b)) return hipErrorInvalidValue;
         if (c.cfg.gram_passes != passes || (c.cfg.jacobi_fp64_sweeps > 0) != sweep64) return hipErrorInvalidValue;
         MixedTask& T = G.t[k];
         T.loc_ptr = c.d_loc_ptr;
@@ -973,7 +1369,8 @@ hipError_t launch_letkf_mixed_group(Ctx* const* cs, int n, hipStream_t stream) {
         T.status = c.d_status;
         T.mp = mixed_params(c, nullptr, nullptr);
         T.N = N;
-        const long long chains = (c.nodes + T.mp.chain - 1) / T.mp.chain;
+        const long long chains = sweep64 ? (c.nodes + T.mp.chain - 1) / T.mp.chain
+                                         : mixed_fused_blocks(c.nx, c.ny, T.mp.chain);
         T.nb_solve = int((chains + 7) / 8 * 8);
         T.nb_finish = int((c.nodes + 7) / 8 * 8);
         T.b0_solve = int(bs);
@@ -982,6 +1379,13 @@ hipError_t launch_letkf_mixed_group(Ctx* const* cs, int n, hipStream_t stream) {
         bf += T.nb_finish;
     }
     if (bs > (1LL << 31) - 1 || bf > (1LL << 31) - 1) return hipErrorInvalidValue;
+    if (!sweep64) {
+        if (passes == 3)
+            k_mixed_analysis_group<3><<<unsigned(bs), 64, 0, stream>>>(G);
+        else
+            k_mixed_analysis_group<1><<<unsigned(bs), 64, 0, stream>>>(G);
+        return hipGetLastError();
+    }
     if (passes == 3)
         k_mixed_solve_group<3><<<unsigned(bs), 64, 0, stream>>>(G);
     else

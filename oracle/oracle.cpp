@@ -23,6 +23,8 @@ inline double wq_of(int q) {                    // src/config.h:260-263
     return ee == 0 ? 4.0 / 9.0 : (ee == 1 ? 1.0 / 9.0 : 1.0 / 36.0);
 }
 inline int pmod(int a, int n) { int r = a % n; return r < 0 ? r + n : r; }
+// pmod for -n <= a < 2n (neighbour indices), without a division
+inline int wrap(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
 
 // src/lbm.h:31-39 (non-ICLBM branch)
 inline double feq_q(const orc_config& c, double rho, double u, double v, int q) {
@@ -71,6 +73,12 @@ void normal4(uint64_t key, const uint32_t ctr[4], double out[4]) {
         out[2 * h + 1] = rad * std::sin(2 * PI * u2);
     }
 }
+
+// LAPACK dsyev (Fortran calling convention, 32-bit integers), when the caller
+// provides one through orc_set_dsyev; else the eigensolve is orc_jacobi_eig
+typedef void (*dsyev_fn)(const char*, const char*, const int*, double*, const int*, double*, double*,
+                         const int*, int*);
+dsyev_fn g_dsyev = nullptr;
 
 constexpr uint32_t STREAM_FORCE = 0x464f5243u, STREAM_IC = 0x49434b34u, STREAM_OBS = 0x4f425345u;
 
@@ -122,12 +130,27 @@ void moments(const orc_config& c, const double* f, long long N, long long node, 
     v = jy / rr;
 }
 
+// Smagorinsky eddy viscosity of node (i, j) from the central differences of the
+// streamed velocities, src/lbm.cu:36-60
+double les_node(const orc_config& c, const double* u, const double* v, int i, int j) {
+    const int nx = c.nx, ny = c.ny;
+    const double dx = c.dx, cs = c.cs_smag;
+    const long long e = wrap(i + 1, nx) + (long long)nx * j, w = wrap(i - 1, nx) + (long long)nx * j;
+    const long long n = i + (long long)nx * wrap(j + 1, ny), s = i + (long long)nx * wrap(j - 1, ny);
+    const double ux = (u[e] - u[w]) / (2 * dx), uy = (u[n] - u[s]) / (2 * dx);
+    const double vx = (v[e] - v[w]) / (2 * dx), vy = (v[n] - v[s]) / (2 * dx);
+    const double s00 = ux, s01 = 0.5 * (uy + vx), s10 = s01, s11 = vy;
+    const double ss = std::sqrt(2 * (s00 * s00 + s01 * s10 + s10 * s01 + s11 * s11));
+    return cs * cs * dx * dx * ss;
+}
+
 }  // namespace
 
 extern "C" {
 
 int orc_sizeof_config(void) { return int(sizeof(orc_config)); }
 void orc_set_threads(int n) { if (n > 0) omp_set_num_threads(n); }
+void orc_set_dsyev(void* fn) { g_dsyev = reinterpret_cast<dsyev_fn>(fn); }
 
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
     philox_block(ctr, key[0], key[1], out);
@@ -229,45 +252,53 @@ void orc_stream(const orc_config* cfg, const double* f, double* fn) {
 // src/lbm.cu:24-65 (Smagorinsky :36-60, LES_CSM not defined)
 void orc_les(const orc_config* cfg, const double* u, const double* v, double* nus) {
     const int nx = cfg->nx, ny = cfg->ny;
-    const double dx = cfg->dx, cs = cfg->cs_smag;
 #pragma omp parallel for
     for (int j = 0; j < ny; ++j)
-        for (int i = 0; i < nx; ++i) {
-            const long long e = pmod(i + 1, nx) + (long long)nx * j, w = pmod(i - 1, nx) + (long long)nx * j;
-            const long long n = i + (long long)nx * pmod(j + 1, ny), s = i + (long long)nx * pmod(j - 1, ny);
-            const double ux = (u[e] - u[w]) / (2 * dx), uy = (u[n] - u[s]) / (2 * dx);
-            const double vx = (v[e] - v[w]) / (2 * dx), vy = (v[n] - v[s]) / (2 * dx);
-            const double s00 = ux, s01 = 0.5 * (uy + vx), s10 = s01, s11 = vy;
-            const double ss = std::sqrt(2 * (s00 * s00 + s01 * s10 + s10 * s01 + s11 * s11));
-            nus[i + (long long)nx * j] = cs * cs * dx * dx * ss;
-        }
+        for (int i = 0; i < nx; ++i) nus[i + (long long)nx * j] = les_node(*cfg, u, v, i, j);
 }
 
-// src/lbm.h:16-29: stream + moments, LES, collide + force, for every member
+// src/lbm.h:16-29: stream + moments, LES, collide + force, for every member.
+// Two passes over (member, row): the pull streaming and the moments of every
+// node, then the Smagorinsky viscosity (it needs the streamed velocities of the
+// four neighbours) and the collision.
 void orc_lbm_step_forced(const orc_config* cfg, int M, const double* fin, double* fout, const double* Fx,
                          const double* Fy) {
     const orc_config& c = *cfg;
-    const long long N = (long long)c.nx * c.ny;
-    std::vector<double> r(N), u(N), v(N), nus(N);
-    for (int k = 0; k < M; ++k) {
-        const double* f = fin + (long long)k * Q * N;
-        double* fn = fout + (long long)k * Q * N;
-        orc_stream(cfg, f, fn);                               // src/lbm.cu:67-103
-        orc_macro(cfg, fn, r.data(), u.data(), v.data());
-        orc_les(cfg, u.data(), v.data(), nus.data());         // src/lbm.cu:24-65
-#pragma omp parallel for
-        for (long long n = 0; n < N; ++n) {                    // src/lbm.cu:105-160
-            const double nu = c.nu + nus[n];
-            const double tau = 0.5 + 3.0 * nu / c.c / c.c / c.dt;   // src/lbm.cu:134-138
-            const double omega = 1.0 / tau;
-            for (int q = 0; q < Q; ++q) {
-                double fc = fn[q * N + n];
-                fc -= omega * (fc - feq_q(c, r[n], u[n], v[n], q));   // lbm_collision_srt.h:8-16
-                fc += force_q(c, Fx[n], Fy[n], q, r[n], u[n], v[n]);  // src/lbm.cu:147-152
-                fn[q * N + n] = fc;
+    const int nx = c.nx, ny = c.ny;
+    const long long N = (long long)nx * ny;
+    std::vector<double> r(size_t(M) * N), u(size_t(M) * N), v(size_t(M) * N);
+#pragma omp parallel for collapse(2)
+    for (int k = 0; k < M; ++k)
+        for (int j = 0; j < ny; ++j) {
+            const double* f = fin + (long long)k * Q * N;
+            double* fn = fout + (long long)k * Q * N;
+            for (int i = 0; i < nx; ++i) {
+                const long long n = i + (long long)nx * j;
+                for (int q = 0; q < Q; ++q) {                       // src/lbm.cu:81-89
+                    const int iu = wrap(i - ex_of(q), nx), ju = wrap(j - ey_of(q), ny);
+                    fn[q * N + n] = f[q * N + iu + (long long)nx * ju];
+                }
+                moments(c, fn, N, n, r[k * N + n], u[k * N + n], v[k * N + n]);   // src/lbm.cu:92-100
             }
         }
-    }
+#pragma omp parallel for collapse(2)
+    for (int k = 0; k < M; ++k)
+        for (int j = 0; j < ny; ++j) {
+            double* fn = fout + (long long)k * Q * N;
+            const double *rk = r.data() + k * N, *uk = u.data() + k * N, *vk = v.data() + k * N;
+            for (int i = 0; i < nx; ++i) {                           // src/lbm.cu:105-160
+                const long long n = i + (long long)nx * j;
+                const double nu = c.nu + les_node(c, uk, vk, i, j);         // src/lbm.cu:24-65
+                const double tau = 0.5 + 3.0 * nu / c.c / c.c / c.dt;   // src/lbm.cu:134-138
+                const double omega = 1.0 / tau;
+                for (int q = 0; q < Q; ++q) {
+                    double fc = fn[q * N + n];
+                    fc -= omega * (fc - feq_q(c, rk[n], uk[n], vk[n], q));   // lbm_collision_srt.h:8-16
+                    fc += force_q(c, Fx[n], Fy[n], q, rk[n], uk[n], vk[n]);  // src/lbm.cu:147-152
+                    fn[q * N + n] = fc;
+                }
+            }
+        }
 }
 
 void orc_lbm_step(const orc_config* cfg, int M, const double* fin, double* fout, long long step) {
@@ -524,7 +555,20 @@ int orc_letkf_transform(int M, int p, const double* dY, const double* rinv, cons
             for (int bb = 0; bb < M; ++bb) C[a * M + bb] += ra * y[bb];
         }
     }
-    const int sweeps = orc_jacobi_eig(M, C.data(), V.data(), d.data(), tol, max_sweeps);
+    int sweeps = 0;
+    if (g_dsyev) {
+        // LAPACK dsyev on the lower triangle (column-major; C is symmetric).  The
+        // eigenvectors come back as the columns of A: V[k][i] = A[k + M i].
+        std::vector<double> A(C), work(size_t(66) * M);
+        const int lwork = int(work.size());
+        int info = 0;
+        g_dsyev("V", "L", &M, A.data(), &M, d.data(), work.data(), &lwork, &info);
+        if (info != 0) return -1;
+        for (int k = 0; k < M; ++k)
+            for (int i = 0; i < M; ++i) V[k * M + i] = A[k + (size_t)M * i];
+    } else {
+        sweeps = orc_jacobi_eig(M, C.data(), V.data(), d.data(), tol, max_sweeps);
+    }
     // letkf_solver.h:288-306: Pa = V D^-1 V^T, w = Pa dY^T rR dyo, W = sqrt(M-1) V D^-1/2 V^T
     std::vector<double> g(M, 0.0);
     for (int i = 0; i < M; ++i) {
@@ -537,10 +581,12 @@ int orc_letkf_transform(int M, int p, const double* dY, const double* rinv, cons
         w[k] = s;
     }
     const double sq = std::sqrt(double(M - 1));
+    std::vector<double> dis(M);
+    for (int i = 0; i < M; ++i) dis[i] = 1.0 / std::sqrt(d[i]);
     for (int k = 0; k < M; ++k)
         for (int l = 0; l < M; ++l) {
             double s = 0;
-            for (int i = 0; i < M; ++i) s += V[k * M + i] * V[l * M + i] / std::sqrt(d[i]);
+            for (int i = 0; i < M; ++i) s += V[k * M + i] * V[l * M + i] * dis[i];
             W[k * M + l] = sq * s;
         }
     if (evals) {

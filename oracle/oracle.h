@@ -44,6 +44,9 @@ typedef struct orc_config {
 
 int  orc_sizeof_config(void);
 void orc_set_threads(int n);
+/* Use LAPACK dsyev (a function pointer with the Fortran LP64 signature) for the
+   local eigensolves of orc_letkf_transform; NULL: the cyclic Jacobi below. */
+void orc_set_dsyev(void* dsyev);
 
 /* Philox4x32-10 block (Salmon et al. SC'11) and the Box-Muller normals built on it */
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);

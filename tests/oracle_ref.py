@@ -22,6 +22,7 @@ P, I, D = C.c_void_p, C.c_int, C.c_double
 _sig = {
     "orc_sizeof_config": ([], I),
     "orc_set_threads": ([I], None),
+    "orc_set_dsyev": ([P], None),
     "orc_philox4x32_10": ([P, P, P], None),
     "orc_philox_normal4": ([C.c_uint64, P, P], None),
     "orc_force_modes": ([P, P, P, P, I], I),
@@ -80,6 +81,26 @@ def sizeof_config() -> int:
 
 def set_threads(n: int):
     _o.orc_set_threads(n)
+
+
+def set_dsyev(on: bool) -> str:
+    """Route the oracle's local eigensolves through LAPACK dsyev (SciPy's LAPACK,
+    reached through scipy.linalg.cython_lapack) or back to its own cyclic Jacobi.
+    Returns the name of the eigensolver in use."""
+    if not on:
+        _o.orc_set_dsyev(None)
+        return "oracle cyclic Jacobi (FP64)"
+    from scipy.linalg import cython_lapack
+
+    cap = cython_lapack.__pyx_capi__["dsyev"]
+    get = C.pythonapi.PyCapsule_GetPointer
+    get.restype = C.c_void_p
+    get.argtypes = [C.py_object, C.c_char_p]
+    name = C.pythonapi.PyCapsule_GetName
+    name.restype = C.c_char_p
+    name.argtypes = [C.py_object]
+    _o.orc_set_dsyev(C.c_void_p(get(cap, name(cap))))
+    return "LAPACK dsyev (SciPy " + __import__("scipy").__version__ + ")"
 
 
 def philox(ctr, key):

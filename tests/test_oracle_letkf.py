@@ -246,3 +246,26 @@ def test_ensemble_stats_definition(lb, orc):
     np.testing.assert_allclose(out[:2], [rmse, spread], rtol=1e-10)
     np.testing.assert_allclose(out[2:], [np.sqrt(((R.mean(0) - tr) ** 2).mean()), np.sqrt(R.var(0).mean())],
                                rtol=1e-8)
+
+
+def test_dsyev_route_matches_the_jacobi_route(orc):
+    """The CPU baseline routes the local eigensolves through LAPACK dsyev; on the same
+    local problem it gives the W, w and eigenvalues of the oracle's own Jacobi."""
+    from threadpoolctl import threadpool_limits
+
+    rng = np.random.default_rng(11)
+    for M, p in ((8, 3), (16, 40), (64, 188)):
+        dY = rng.standard_normal((p, M))
+        dY -= dY.mean(1, keepdims=True)
+        rinv = rng.uniform(0.5, 2, p) * 1e2
+        dyo = rng.standard_normal(p)
+        W0, w0, e0 = orc.letkf_transform(dY, rinv, dyo, 1.1)
+        try:
+            with threadpool_limits(1):
+                assert "dsyev" in orc.set_dsyev(True)
+                W1, w1, e1 = orc.letkf_transform(dY, rinv, dyo, 1.1)
+        finally:
+            orc.set_dsyev(False)
+        assert np.linalg.norm(W1 - W0) <= 1e-12 * np.linalg.norm(W0)
+        assert np.linalg.norm(w1 - w0) <= 1e-11 * np.linalg.norm(w0)
+        np.testing.assert_allclose(e1, e0, rtol=1e-12)
