@@ -205,6 +205,15 @@ int lbmletkf_cycle(lbmletkf_ctx* ctx, const float* yo_device);
 int lbmletkf_ensemble_stats(lbmletkf_ctx* ctx, const float* truth_rho, const float* truth_u,
                             const float* truth_v, int truth_on_device, double out[4]);
 
+/* lbmletkf_ensemble_stats with the truth on the device and no allocation: no
+ * counterpart in the reference; for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops).  The per-block partial sums
+ * go to a static device array of the library (2 MiB per device, grids of up to
+ * 2^24 nodes; larger grids return LBMLETKF_ERR_ARG).  Calls that overlap on one
+ * device share that array and race.  Same result as lbmletkf_ensemble_stats. */
+int lbmletkf_ensemble_stats_noalloc(lbmletkf_ctx* ctx, const float* truth_rho_device,
+                                    const float* truth_u_device, const float* truth_v_device, double out[4]);
+
 /* Local observation set of one gridpoint (parity diagnostics): site indices and
  * localization weights in gather order.  *n receives the count; at most cap are written. */
 int lbmletkf_local_obs(lbmletkf_ctx* ctx, int gridpoint, int* n, int* sites, float* weights, int cap);

@@ -706,6 +706,7 @@ int lbmletkf_create(const lbmletkf_config* cfg, int n_members, lbmletkf_ctx** ou
         }                                                                                     \
     } while (0)
     CREATE_TRY(hipGetDevice(&c->device));
+    CREATE_TRY(stats_scratch_address(&c->stats_scratch));
     CREATE_TRY(hipStreamCreate(&c->stream));
     const size_t fsz = size_t(c->M) * Q * c->nodes;
     CREATE_TRY(dalloc(&c->f[0], fsz));
@@ -1048,6 +1049,38 @@ int lbmletkf_ensemble_stats(lbmletkf_ctx* ctx, const float* truth_rho, const flo
     long double s[4] = {0, 0, 0, 0};
     for (int b = 0; b < nb; ++b)
         for (int a = 0; a < 4; ++a) s[a] += part[size_t(b) * 4 + a];
+    out[0] = double(std::sqrt(s[0] / (2.0L * n)));
+    out[1] = double(std::sqrt(s[1] / (2.0L * n)));
+    out[2] = double(std::sqrt(s[2] / (long double)n));
+    out[3] = double(std::sqrt(s[3] / (long double)n));
+    return LBMLETKF_OK;
+}
+
+int lbmletkf_ensemble_stats_noalloc(lbmletkf_ctx* ctx, const float* truth_rho, const float* truth_u,
+                                    const float* truth_v, double out[4]) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    CHECK_CTX(c);
+    if (!truth_rho || !truth_u || !truth_v || !out) return fail(c, LBMLETKF_ERR_ARG, "NULL argument");
+    const int nb = ensemble_stats_blocks(*c);
+    if (nb > STATS_SCRATCH_BLOCKS)
+        return fail(c, LBMLETKF_ERR_ARG, "grid too large for lbmletkf_ensemble_stats_noalloc: at most " +
+                                              std::to_string((long long)STATS_SCRATCH_BLOCKS * 256) + " nodes");
+    if (!c->stats_scratch) return fail(c, LBMLETKF_ERR_STATE, "static scratch address not resolved");
+    // every block writes its four partial sums: nothing to clear
+    HIP_TRY(c, launch_ensemble_stats(*c, truth_rho, truth_u, truth_v, c->stats_scratch, nb));
+    if (int st = sync_and_check(c)) return st;
+    // the partials summed in block order (as lbmletkf_ensemble_stats does), read back
+    // through a fixed host buffer
+    long double s[4] = {0, 0, 0, 0};
+    double part[4 * 512];
+    for (int b0 = 0; b0 < nb; b0 += 512) {
+        const int k = std::min(512, nb - b0);
+        HIP_TRY(c, hipMemcpy(part, c->stats_scratch + 4 * size_t(b0), sizeof(double) * 4 * size_t(k),
+                             hipMemcpyDeviceToHost));
+        for (int b = 0; b < k; ++b)
+            for (int a = 0; a < 4; ++a) s[a] += part[size_t(b) * 4 + a];
+    }
+    const size_t n = size_t(c->nodes);
     out[0] = double(std::sqrt(s[0] / (2.0L * n)));
     out[1] = double(std::sqrt(s[1] / (2.0L * n)));
     out[2] = double(std::sqrt(s[2] / (long double)n));

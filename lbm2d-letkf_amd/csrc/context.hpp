@@ -129,6 +129,7 @@ struct Ctx {
 
     // macro scratch for stats
     float* d_truth = nullptr;     // [3][nodes]
+    double* stats_scratch = nullptr;  // device address of the static partial sums (noalloc stats)
 
     // phase timers
     bool prof = false;
@@ -182,6 +183,10 @@ hipError_t launch_macro(Ctx& c, const float* f_member, float* rho, float* u, flo
 hipError_t launch_ensemble_stats(Ctx& c, const float* truth_rho, const float* truth_u,
                                  const float* truth_v, double* partials, int n_blocks);
 int ensemble_stats_blocks(const Ctx& c);
+// the static partial-sum array of lbmletkf_ensemble_stats_noalloc: blocks it holds
+// (grids of up to 2^24 nodes) and its device address on the current device
+constexpr int STATS_SCRATCH_BLOCKS = 65536;
+hipError_t stats_scratch_address(double** out);
 hipError_t launch_step_advance(Ctx& c, int n);
 bool lbm_fused_supported(int nx, int ny);
 // full-row strips of h rows (the alternative shape of the fused step)

@@ -15,6 +15,9 @@
 //     256^2 x 64: the tiles of one workgroup do not overlap their loads).
 //   * the plain form keeps the reference's three passes (stream+macro, les,
 //     collide) and is launched member by member, as one reference rank would.
+#include <map>
+#include <mutex>
+
 #include "context.hpp"
 #include "lbm_node.hpp"
 #include "philox.hpp"
@@ -696,6 +699,33 @@ hipError_t launch_macro(Ctx& c, const float* f_member, float* rho, float* u, flo
 }
 
 int ensemble_stats_blocks(const Ctx& c) { return node_blocks(c.nodes, STATS_NT); }
+
+// The partial sums of lbmletkf_ensemble_stats_noalloc live in this file-scope array
+// instead of a buffer the library allocates: 4 doubles per block of k_ensemble_stats,
+// for grids of up to STATS_SCRATCH_BLOCKS * STATS_NT = 2^24 nodes (2 MiB per device,
+// held until the process ends).  Every call on a device shares that device's copy
+// and overwrites the slots it uses, so calls that overlap on one device race.
+static __device__ __align__(256) double scratch[STATS_SCRATCH_BLOCKS * 4];
+
+hipError_t stats_scratch_address(double** out) {
+    // the device address of `scratch` on the current device, looked up once per device
+    // (the host-side name is not a device address)
+    static std::mutex mu;
+    static std::map<int, double*> addr;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = addr.find(dev);
+    if (it == addr.end()) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        it = addr.emplace(dev, static_cast<double*>(p)).first;
+    }
+    *out = it->second;
+    return hipSuccess;
+}
 
 hipError_t launch_ensemble_stats(Ctx& c, const float* truth_rho, const float* truth_u,
                                  const float* truth_v, double* partials, int n_blocks) {

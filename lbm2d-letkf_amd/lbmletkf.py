@@ -98,6 +98,7 @@ def _load():
         "lbmletkf_analysis": ([P, P, I], I),
         "lbmletkf_cycle": ([P, P], I),
         "lbmletkf_ensemble_stats": ([P, P, P, P, I, P], I),
+        "lbmletkf_ensemble_stats_noalloc": ([P, P, P, P, P], I),
         "lbmletkf_local_obs": ([P, I, C.POINTER(I), P, P, I], I),
         "lbmletkf_set_dump": ([P, I, I], I),
         "lbmletkf_get_dump": ([P, P, P, P], I),
