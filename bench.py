@@ -450,7 +450,7 @@ def main():
     ap.add_argument("--prof-cycles", type=int, default=3, help="extra phase-timed cycles (eager launches)")
     ap.add_argument("--tuning", default=None, metavar="LBM,CHAIN",
                     help="fixed launch shapes of the mixed variant instead of the autotuner (for profiling "
-                         "runs: LBM shape code, analysis chain), e.g. -116,32")
+                         "runs: LBM shape code, analysis chain), e.g. --tuning=-116,32")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven-batch group measurement")
     ap.add_argument("--uneven-steps", type=int, default=3, help="timed group cycles of the uneven batch")
