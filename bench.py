@@ -217,8 +217,11 @@ def measured_traffic(kernel, workgroups):
 
 
 def lbm_steps_per_launch(tune):
-    """Steps one launch of the tuned LBM shape takes: two for the two-step row
-    strips (shape code -(100 + h); the benchmark flow has no body force)."""
+    """Steps one launch of the tuned LBM shape takes: three for the three-step row
+    strips (shape code -(200 + h)), two for the two-step strips (-(100 + h)); the
+    benchmark flow has no body force."""
+    if tune and tune[0] <= -200:
+        return 3
     return 2 if tune and tune[0] <= -100 else 1
 
 
@@ -523,7 +526,9 @@ def main():
     solve_wg = (nodes + tune[1] - 1) // tune[1]
     mixed = head["variant"] == "mixed"
     traffic_ana = measured_traffic("k_mixed_analysis<64, 3>", solve_wg) if mixed else None
-    if tune[0] <= -100:
+    if tune[0] <= -200:
+        lbm_kernel, lbm_wg = "k_lbm_rows3<%d>" % (-tune[0] - 200), (cfg.ny // (-tune[0] - 200)) * M
+    elif tune[0] <= -100:
         lbm_kernel, lbm_wg = "k_lbm_rows2<%d>" % (-tune[0] - 100), (cfg.ny // (-tune[0] - 100)) * M
     elif tune[0] < 0:
         lbm_kernel, lbm_wg = "k_lbm_rows<false, %d>" % -tune[0], (cfg.ny // -tune[0]) * M

@@ -322,11 +322,12 @@ int lbmletkf_trainer_train(lbmletkf_trainer* tr, int batch, int steps, double lr
  * lbm_tiles_per_block = k > 0: k tiles of 64 x 16 per workgroup; = -h: full-row
  * strips of h rows (h = 8, 16, 32; nx a multiple of 64 up to 1024, one thread per
  * column); = -(100 + h): full-row strips of h rows advanced two steps per launch
- * (h = 16, 32; nx a multiple of 64 up to 1024; flows without body force or friction,
- * others take one step per launch; bit-identical to single steps) -- and the chain
- * length of the FP32 solve kernel (consecutive gridpoints per wavefront, each
- * warm-started from the previous one's eigenvectors; the result changes by FP32
- * rounding).
+ * (h = 16, 32; nx a multiple of 64 up to 1024); = -(200 + h): three steps per
+ * launch (h = 16, 32; nx a multiple of 64 up to 512).  The multi-step strips serve
+ * flows without body force or friction (others take one step per launch) and are
+ * bit-identical to single steps -- and the chain length of the analysis (gridpoints
+ * of one column per wavefront, each warm-started from the previous one's
+ * eigenvectors; the result changes by FP32 rounding).
  * No counterpart in the reference. */
 int lbmletkf_set_tuning(lbmletkf_ctx* ctx, int lbm_tiles_per_block, int mix_chain);
 int lbmletkf_get_tuning(lbmletkf_ctx* ctx, int* lbm_tiles_per_block, int* mix_chain);

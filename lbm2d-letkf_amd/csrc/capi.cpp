@@ -424,11 +424,12 @@ int forecast_launches(Ctx* c, int n_steps) {
     if (mlp && !c->closure_ready) return fail(c, LBMLETKF_ERR_STATE, "closure_kind is MLP: call lbmletkf_set_closure first");
     const bool fused = mlp || (c->cfg.variant == LBMLETKF_VARIANT_MIXED && lbm_fused_supported(c->nx, c->ny));
     hipEvent_t ev = prof_begin(c);
-    // the two-step strips take two steps per launch (no forcing: no per-step field)
-    const int two = (fused && !mlp) ? lbm_launch_steps(*c) : 1;
+    // the two- and three-step strips take several steps per launch (no forcing: no
+    // per-step field); the last steps of a forecast take what is left
+    const int per_launch = (fused && !mlp) ? lbm_launch_steps(*c) : 1;
     long long launches = 0;
     for (int s = 0; s < n_steps;) {
-        const int k = (two == 2 && s + 2 <= n_steps) ? 2 : 1;
+        const int k = std::min(per_launch, n_steps - s);
         if (c->lp.has_force) HIP_TRY(c, launch_force_field(*c, s));
         const float* fin = c->f[c->cur];
         float* fout = c->f[1 - c->cur];
@@ -1338,15 +1339,15 @@ int group_analysis_launches(Group* g) {
     return LBMLETKF_OK;
 }
 
-// Each context steps in its own tuned shape: the tuned shapes (one- or two-step row
+// Each context steps in its own tuned shape: the tuned shapes (one- to three-step row
 // strips) beat one tile launch over all contexts, which measured the same as the
 // sum of the contexts' one-step launches.  A context with the learned closure steps
 // with the closure kernel, one step per launch, as it does on its own.
 int group_context_steps(Group* g, Ctx* c, int n_steps) {
     const bool mlp = c->cfg.closure_kind == LBMLETKF_CLOSURE_MLP;
-    const int two = mlp ? 1 : lbm_launch_steps(*c);
+    const int per_launch = mlp ? 1 : lbm_launch_steps(*c);
     for (int s = 0; s < n_steps;) {
-        const int k = (two == 2 && s + 2 <= n_steps) ? 2 : 1;
+        const int k = std::min(per_launch, n_steps - s);
         if (c->lp.has_force) G_HIP(g, launch_force_field(*c, s));
         if (mlp)
             G_HIP(g, launch_lbm_mlp(*c, c->f[c->cur], c->f[1 - c->cur], 0, (c->nx / 64) * (c->ny / 16) * c->M,
@@ -1573,19 +1574,24 @@ void* lbmletkf_group_get_stream(lbmletkf_group* grp) {
 int lbmletkf_set_tuning(lbmletkf_ctx* ctx, int lbm_tiles_per_block, int mix_chain) {
     Ctx* c = reinterpret_cast<Ctx*>(ctx);
     CHECK_CTX(c);
-    const bool two = lbm_tiles_per_block <= -LBM_TWO_STEP;
+    const bool three = lbm_tiles_per_block <= -LBM_THREE_STEP;
+    const bool two = lbm_tiles_per_block <= -LBM_TWO_STEP && !three;
     if (lbm_tiles_per_block == 0 || lbm_tiles_per_block > 16 || mix_chain < 1 || mix_chain > 64 ||
-        (lbm_tiles_per_block < -32 && !two))
+        (lbm_tiles_per_block < -32 && !two && !three))
         return fail(c, LBMLETKF_ERR_ARG,
-                    "the LBM shape must be 1..16 tiles, -8/-16/-32 rows or -116/-132 two-step rows, the chain in [1, 64]");
+                    "the LBM shape must be 1..16 tiles, -8/-16/-32 rows, -116/-132 two-step or -216/-232 three-step "
+                    "rows, the chain in [1, 64]");
     if (lbm_tiles_per_block > 0 && lbm_fused_supported(c->nx, c->ny) &&
         ((long long)(c->nx / 64) * (c->ny / 16) * c->M) % lbm_tiles_per_block)
         return fail(c, LBMLETKF_ERR_ARG, "tiles per block must divide the number of tiles");
-    if (lbm_tiles_per_block < 0 && !two && !lbm_rows_supported(c->nx, c->ny, -lbm_tiles_per_block))
+    if (lbm_tiles_per_block < 0 && !two && !three && !lbm_rows_supported(c->nx, c->ny, -lbm_tiles_per_block))
         return fail(c, LBMLETKF_ERR_ARG, "row strips need nx a multiple of 64 up to 1024 and ny a multiple of the height");
     if (two && !lbm_rows2_supported(c->nx, c->ny, -lbm_tiles_per_block - LBM_TWO_STEP))
         return fail(c, LBMLETKF_ERR_ARG,
                     "two-step row strips need nx a multiple of 64 up to 1024 and ny a multiple of the height (16 or 32)");
+    if (three && !lbm_rows3_supported(c->nx, c->ny, -lbm_tiles_per_block - LBM_THREE_STEP))
+        return fail(c, LBMLETKF_ERR_ARG,
+                    "three-step row strips need nx a multiple of 64 up to 512 and ny a multiple of the height (16 or 32)");
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     c->drop_graphs();
     c->lbm_tiles_per_block = lbm_tiles_per_block;
@@ -1639,7 +1645,7 @@ std::string tune_key(const Ctx* c) {
     return dev + "/" + std::to_string(c->nx) + "x" + std::to_string(c->ny) + "/M" + std::to_string(c->M) + "/v" +
            std::to_string(g.variant) + "/p" + std::to_string(g.gram_passes) + "/f" +
            std::to_string(g.jacobi_fp64_sweeps > 0) + "/s" + std::to_string(c->obs.n_sites) + "/e" +
-           std::to_string(c->obs.n_entries) + "/t3";  // t3: the candidate set and the fused analysis (column chains)
+           std::to_string(c->obs.n_entries) + "/t4";  // t4: three-step strips, fused analysis with column chains
 }
 
 }  // namespace
@@ -1693,12 +1699,15 @@ int lbmletkf_autotune(lbmletkf_ctx* ctx, const char* cache_path, char* report, i
         const long long tiles = (long long)(c->nx / 64) * (c->ny / 16) * c->M;
         double best = 1e300;
         rep += "lbm us/step:";
-        for (int tpb : {1, 2, 4, -8, -16, -LBM_TWO_STEP - 16, -LBM_TWO_STEP - 32}) {
+        for (int tpb : {1, 2, 4, -8, -16, -LBM_TWO_STEP - 16, -LBM_TWO_STEP - 32, -LBM_THREE_STEP - 16,
+                        -LBM_THREE_STEP - 32}) {
             if (tpb > 0 && tiles % tpb) continue;
             if (tpb < 0 && tpb > -LBM_TWO_STEP && !lbm_rows_supported(c->nx, c->ny, -tpb)) continue;
-            if (tpb <= -LBM_TWO_STEP && !lbm_rows2_supported(c->nx, c->ny, -tpb - LBM_TWO_STEP)) continue;
+            if (tpb <= -LBM_TWO_STEP && tpb > -LBM_THREE_STEP && !lbm_rows2_supported(c->nx, c->ny, -tpb - LBM_TWO_STEP))
+                continue;
+            if (tpb <= -LBM_THREE_STEP && !lbm_rows3_supported(c->nx, c->ny, -tpb - LBM_THREE_STEP)) continue;
             c->lbm_tiles_per_block = tpb;
-            const int k = lbm_launch_steps(*c);  // time per step (a launch may take two)
+            const int k = lbm_launch_steps(*c);  // time per step (a launch may take several)
             double us = 0;
             if (int st = time_launches(c, 2, 10, [&] { return launch_lbm_fused(*c, c->f[c->cur], c->f[1 - c->cur], k); }, us))
                 return st;

@@ -15,6 +15,7 @@ constexpr int MAX_MEMBERS = 64;      // one wavefront of lanes per ensemble matr
 constexpr int MAX_FORCE_MODES = 100; // src/data.cu:191-212 prunes to 100 modes
 constexpr int MIXED_GROUP_MAX = 8;   // contexts in one group launch
 constexpr int LBM_TWO_STEP = 100;    // shape code offset of the two-step strips
+constexpr int LBM_THREE_STEP = 200;  // shape code offset of the three-step strips
 
 // Per-step LBM constants, in the arithmetic type of the kernels (FP32).
 struct LbmParams {
@@ -116,7 +117,8 @@ struct Ctx {
     int mix_chain = 8;            // consecutive gridpoints per wavefront in k_mixed_solve
     // shape of the fused LBM step: k > 0 tiles of 64x16 per workgroup (k_lbm_fused),
     // -h full-row strips of h rows (k_lbm_rows), -(LBM_TWO_STEP + h) full-row strips
-    // of h rows advanced two steps per launch (k_lbm_rows2)
+    // of h rows advanced two steps per launch (k_lbm_rows2), -(LBM_THREE_STEP + h)
+    // three steps per launch (k_lbm_rows3)
     int lbm_tiles_per_block = 1;
 
     // transform dump
@@ -161,11 +163,12 @@ int report_error(Ctx* c, int code, const std::string& msg);
 
 // ---- launchers (lbm.hip) ------------------------------------------------------
 hipError_t launch_force_field(Ctx& c, int step_offset);
-// one launch of the fused step in the tuned shape, advancing `steps` steps (2 only
-// where lbm_launch_steps(c) == 2)
+// one launch of the fused step in the tuned shape, advancing `steps` steps (up to
+// lbm_launch_steps(c))
 hipError_t launch_lbm_fused(Ctx& c, const float* fin, float* fout, int steps = 1);
-// steps one launch of the tuned shape can take: 2 for the two-step strips
-// (lbm_tiles_per_block <= -LBM_TWO_STEP) without forcing, else 1
+// steps one launch of the tuned shape can take without forcing: 3 for the
+// three-step strips (lbm_tiles_per_block <= -LBM_THREE_STEP), 2 for the two-step
+// strips (<= -LBM_TWO_STEP), else 1; 1 for any forced flow
 int lbm_launch_steps(const Ctx& c);
 // the MIXED analysis of several contexts (obs already prepared) in one launch of
 // each of its two kernels, on `stream`
@@ -193,6 +196,7 @@ bool lbm_fused_supported(int nx, int ny);
 bool lbm_rows_supported(int nx, int ny, int h);
 // full-row strips of h rows advanced two steps per launch (k_lbm_rows2)
 bool lbm_rows2_supported(int nx, int ny, int h);
+bool lbm_rows3_supported(int nx, int ny, int h);
 
 // ---- launchers (letkf.hip) ----------------------------------------------------
 hipError_t launch_local_count(Ctx& c, int* d_count);

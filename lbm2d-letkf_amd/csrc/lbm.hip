@@ -321,6 +321,195 @@ __global__ __launch_bounds__(1024) void k_lbm_rows2(const float* __restrict__ fi
     }
 }
 
+// Three steps per launch (no body force, no friction), the two-step pipeline
+// extended by two stages: iteration k runs six stages a row apart:
+//   A (row k):   pull from global (t), moments -> velocity ring 1
+//   B (row k-1): Smagorinsky, collide -> populations at t+1 -> population ring 1
+//   C (row k-2): pull from population ring 1, moments -> velocity ring 2
+//   D (row k-3): Smagorinsky, collide -> populations at t+2 -> population ring 2
+//   E (row k-4): pull from population ring 2, moments -> velocity ring 3
+//   F (row k-5): Smagorinsky, collide -> populations at t+3 -> global
+// A strip of H rows reads rows -6 .. H+5 and runs step one on rows -4 .. H+3,
+// step two on -2 .. H+1 (each step has a halo of two rows: one for the pull, one
+// for the central differences of the Smagorinsky viscosity).
+// The rings keep each value only as long as other columns read it, so that four
+// workgroups of nx = 256 fit one CU (144 B per column, 36 KB):
+// * velocity rings: row r in slot r & 1; the collide of row r reads the east and
+//   west neighbours of row r there, while its own column's velocities of rows
+//   r - 1 and r + 1 stay in the thread's registers;
+// * population rings hold the six populations that stream across columns, by
+//   when they are read: the pull of row r takes q = 0, 2 from row r + 1 (one slot:
+//   written and read in one iteration), q = 3, 5 from row r (two slots) and
+//   q = 6, 8 from row r - 1 (three slots); q = 1, 4, 7 (e_x = 0) stream within
+//   the column and stay in registers.
+// Two barriers per iteration suffice, after B and after D: the only values a
+// stage reads from the LDS in the iteration that wrote them are the q = 0, 2
+// populations of the row just collided; every other ring slot is read one or two
+// iterations after its write and rewritten after a barrier that follows its last
+// read.
+// Every node runs the per-node functions of the single-step kernels, so three of
+// these steps equal three single steps bit for bit.  HBM traffic: 36 B read + 36 B
+// written per node per launch, i.e. per three steps.
+struct PopRing {
+    float* q02;  // [2][nx]     q = 0, 2 of the last collided row
+    float* q35;  // [2][2][nx]  q = 3, 5 of rows of parity 0 / 1
+    float* q68;  // [3][2][nx]  q = 6, 8 of rows r mod 3
+};
+template <int H>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k_lbm_rows3(const float* __restrict__ fin, float* __restrict__ fout,
+                                                  LbmParams p, int strips_per_member) {
+    extern __shared__ float lds3[];  // velocity rings [3 stages][u, v][2][nx]; population rings [2][12][nx]
+    const int nx = p.nx, ny = p.ny;
+    const int nodes = int(p.nodes);  // offsets within one member fit in 32 bits
+    float* vel = lds3;
+    PopRing r1{vel + 12 * nx, vel + 14 * nx, vel + 18 * nx};
+    PopRing r2{vel + 24 * nx, vel + 26 * nx, vel + 30 * nx};
+    const int nb = gridDim.x, b = blockIdx.x;
+    const int bb = (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
+    const int member = bb / strips_per_member;
+    const int y0 = (bb - member * strips_per_member) * H;
+    const float* f = fin + (long long)member * p.member_stride;
+    float* fo = fout + (long long)member * p.member_stride;
+    const int i = threadIdx.x;
+    const int im = i == 0 ? nx - 1 : i - 1, ip = i == nx - 1 ? 0 : i + 1;
+    const unsigned un = unsigned(nodes), ui = unsigned(i), uim = unsigned(im), uip = unsigned(ip);
+    auto pull_g = [&](int r, float (&fc)[Q]) {
+        const int j = wrap(y0 + r, ny);
+        const unsigned rm = unsigned(wrap(j - 1, ny) * nx), r0 = unsigned(j * nx), rp = unsigned(wrap(j + 1, ny) * nx);
+        fc[0] = f[0 * un + rp + uip];
+        fc[1] = f[1 * un + rp + ui];
+        fc[2] = f[2 * un + rp + uim];
+        fc[3] = f[3 * un + r0 + uip];
+        fc[4] = f[4 * un + r0 + ui];
+        fc[5] = f[5 * un + r0 + uim];
+        fc[6] = f[6 * un + rm + uip];
+        fc[7] = f[7 * un + rm + ui];
+        fc[8] = f[8 * un + rm + uim];
+    };
+    // LDS column offsets of this thread and its west and east neighbours, renewed
+    // from an opaque copy at every iteration: formed once outside the loop, the
+    // ring addresses built on them would stay live across it and spill
+    int ci = i, cm = im, cp = ip;
+    auto q35_slot = [&](const PopRing& R, int r) { return R.q35 + (r & 1) * 2 * nx; };
+    auto q68_slot = [&](const PopRing& R, int r) { return R.q68 + ((r + 12) % 3) * 2 * nx; };
+    // pull of row r from a population ring; q = 1, 4, 7 from the thread's own rows
+    auto pull_ring = [&](const PopRing& R, int r, float q1, float q4, float q7, float (&fc)[Q]) {
+        const float* a = R.q02;               // row r + 1
+        const float* c = q35_slot(R, r);      // row r
+        const float* d = q68_slot(R, r - 1);  // row r - 1
+        fc[0] = a[cp];
+        fc[1] = q1;
+        fc[2] = a[nx + cm];
+        fc[3] = c[cp];
+        fc[4] = q4;
+        fc[5] = c[nx + cm];
+        fc[6] = d[cp];
+        fc[7] = q7;
+        fc[8] = d[nx + cm];
+    };
+    auto to_ring = [&](const PopRing& R, int r, const float (&fd)[Q]) {
+        R.q02[ci] = fd[0];
+        R.q02[nx + ci] = fd[2];
+        float* c = q35_slot(R, r);
+        c[ci] = fd[3];
+        c[nx + ci] = fd[5];
+        float* d = q68_slot(R, r);
+        d[ci] = fd[6];
+        d[nx + ci] = fd[8];
+    };
+    // velocity ring of stage s (0, 1, 2): u of row r at vel[(s * 4 + (r & 1)) * nx],
+    // v two slots further
+    auto vel_put = [&](int s, int r, float u, float v) {
+        vel[(s * 4 + (r & 1)) * nx + ci] = u;
+        vel[(s * 4 + 2 + (r & 1)) * nx + ci] = v;
+    };
+    // collide of row r: own velocities of rows r+1 (uN, vN) and r-1 (uS, vS) from
+    // registers, those of the east and west neighbours of row r from the ring
+    auto collide_row = [&](int s, int r, float uN, float vN, float uS, float vS, float (&fd)[Q],
+                           const float (&md)[3]) {
+        const float* ur = vel + (s * 4 + (r & 1)) * nx;
+        const float* vr = vel + (s * 4 + 2 + (r & 1)) * nx;
+        const float nut = smagorinsky(ur[cp], ur[cm], uN, uS, vr[cp], vr[cm], vN, vS, p);
+        collide9<false>(fd, md[0], md[1], md[2], nut, 0.f, 0.f, p);
+    };
+    // e_x = 0 populations of the collided rows: q4 of the last row, q7 of the last two
+    float a4m1 = 0.f, a7m1 = 0.f, a7m2 = 0.f;  // step one
+    float b4m1 = 0.f, b7m1 = 0.f, b7m2 = 0.f;  // step two
+    float fP[Q];
+    // One iteration; the stage pairs (A, B), (C, D), (E, F) alternate register sets
+    // between consecutive iterations, so no row is copied from one set to the other.
+    // A moments stage's set holds, on entry, the moments of its row two iterations
+    // back: the south velocities of its collide stage.
+    auto iter = [&](int k, float (&fa)[Q], float (&ma)[3], float (&fb)[Q], float (&mb)[3], float (&fc)[Q],
+                    float (&mc)[3], float (&fd)[Q], float (&md)[3], float (&fe)[Q], float (&me)[3],
+                    float (&ff)[Q], float (&mf)[3]) {
+        ci = i;
+        asm volatile("" : "+v"(ci));
+        cm = ci == 0 ? nx - 1 : ci - 1;
+        cp = ci == nx - 1 ? 0 : ci + 1;
+        // A: row k (the pull was issued one iteration earlier)
+        const float uSa = ma[1], vSa = ma[2];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) fa[q] = fP[q];
+        if (k < H + 4) pull_g(k + 1, fP);
+        moments(fa, p.c, ma[0], ma[1], ma[2]);
+        vel_put(0, k, ma[1], ma[2]);
+        __builtin_amdgcn_sched_barrier(0);  // stages stay in order (register pressure)
+        // B: row k-1 -> populations at t+1
+        if (k >= -3) {
+            collide_row(0, k - 1, ma[1], ma[2], uSa, vSa, fb, mb);
+            to_ring(r1, k - 1, fb);
+        }
+        __syncthreads();
+        // C: row k-2 at t+1
+        const float uSc = mc[1], vSc = mc[2];
+        if (k >= -1) {
+            pull_ring(r1, k - 2, fb[1], a4m1, a7m2, fc);
+            moments(fc, p.c, mc[0], mc[1], mc[2]);
+            vel_put(1, k - 2, mc[1], mc[2]);
+        }
+        a7m2 = a7m1;
+        a7m1 = fb[7];
+        a4m1 = fb[4];
+        __builtin_amdgcn_sched_barrier(0);
+        // D: row k-3 -> populations at t+2
+        if (k >= 1) {
+            collide_row(1, k - 3, mc[1], mc[2], uSc, vSc, fd, md);
+            to_ring(r2, k - 3, fd);
+        }
+        __syncthreads();
+        // E: row k-4 at t+2
+        const float uSe = me[1], vSe = me[2];
+        if (k >= 3) {
+            pull_ring(r2, k - 4, fd[1], b4m1, b7m2, fe);
+            moments(fe, p.c, me[0], me[1], me[2]);
+            vel_put(2, k - 4, me[1], me[2]);
+        }
+        b7m2 = b7m1;
+        b7m1 = fd[7];
+        b4m1 = fd[4];
+        __builtin_amdgcn_sched_barrier(0);
+        // F: row k-5 -> populations at t+3, written out
+        if (k >= 5) {
+            collide_row(2, k - 5, me[1], me[2], uSe, vSe, ff, mf);
+            const unsigned node = unsigned((y0 + k - 5) * nx) + ui;
+#pragma unroll
+            for (int q = 0; q < Q; ++q) fo[unsigned(q) * un + node] = ff[q];
+        }
+    };
+    float fX[Q], fY[Q], fZ[Q], fW[Q], fS[Q], fT[Q];
+    float mX[3] = {0.f, 0.f, 0.f}, mY[3] = {0.f, 0.f, 0.f}, mZ[3] = {0.f, 0.f, 0.f}, mW[3] = {0.f, 0.f, 0.f},
+          mS[3] = {0.f, 0.f, 0.f}, mT[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int q = 0; q < Q; ++q) fY[q] = fW[q] = fT[q] = 0.f;
+    pull_g(-5, fP);
+    static_assert((H + 10) % 2 == 0, "iterations come in pairs");
+    for (int k = -5; k <= H + 4; k += 2) {
+        iter(k, fX, mX, fY, mY, fZ, mZ, fW, mW, fS, mS, fT, mT);
+        iter(k + 1, fY, mY, fX, mX, fW, mW, fZ, mZ, fT, mT, fS, mS);
+    }
+}
+
 // ----------------------------------------------------------------------------
 // plain form: the reference's three passes for one member
 // ----------------------------------------------------------------------------
@@ -614,35 +803,64 @@ bool lbm_rows2_supported(int nx, int ny, int h) {
     return nx % 64 == 0 && nx <= 1024 && (h == 16 || h == 32) && ny % h == 0 && ny >= 2;
 }
 
+// LDS of the three-step strips: 144 B per column (12 velocity rows + 2 x 12
+// population rows of 4 B), 72 KB at nx = 512
+bool lbm_rows3_supported(int nx, int ny, int h) {
+    return nx % 64 == 0 && nx <= 512 && (h == 16 || h == 32) && ny % h == 0 && ny >= 2;
+}
+
 int lbm_launch_steps(const Ctx& c) {
     const bool force = c.lp.has_force || c.lp.fric_over_dt != 0.f;
-    return (c.lbm_tiles_per_block <= -LBM_TWO_STEP && !force) ? 2 : 1;
+    if (force) return 1;
+    if (c.lbm_tiles_per_block <= -LBM_THREE_STEP) return 3;
+    return c.lbm_tiles_per_block <= -LBM_TWO_STEP ? 2 : 1;
+}
+
+static hipError_t launch_rows2(Ctx& c, const float* fin, float* fout, int h) {
+    if (!lbm_rows2_supported(c.nx, c.ny, h)) return hipErrorInvalidValue;
+    const int strips = c.ny / h;
+    const unsigned nb = unsigned(strips * c.M);
+    const size_t lds = sizeof(float) * (12 + 3 * P1Q) * size_t(c.nx);
+    const void* fn = h == 16 ? reinterpret_cast<const void*>(&k_lbm_rows2<16>)
+                             : reinterpret_cast<const void*>(&k_lbm_rows2<32>);
+    if (lds > 65536) {  // beyond the default dynamic LDS of a workgroup
+        const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+        if (e != hipSuccess) return e;
+    }
+    if (h == 16)
+        k_lbm_rows2<16><<<nb, c.nx, lds, c.stream>>>(fin, fout, c.lp, strips);
+    else
+        k_lbm_rows2<32><<<nb, c.nx, lds, c.stream>>>(fin, fout, c.lp, strips);
+    return hipGetLastError();
+}
+
+static hipError_t launch_rows3(Ctx& c, const float* fin, float* fout, int h) {
+    if (!lbm_rows3_supported(c.nx, c.ny, h)) return hipErrorInvalidValue;
+    const int strips = c.ny / h;
+    const unsigned nb = unsigned(strips * c.M);
+    const size_t lds = sizeof(float) * 36 * size_t(c.nx);
+    const void* fn = h == 16 ? reinterpret_cast<const void*>(&k_lbm_rows3<16>)
+                             : reinterpret_cast<const void*>(&k_lbm_rows3<32>);
+    if (lds > 65536) {
+        const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+        if (e != hipSuccess) return e;
+    }
+    if (h == 16)
+        k_lbm_rows3<16><<<nb, c.nx, lds, c.stream>>>(fin, fout, c.lp, strips);
+    else
+        k_lbm_rows3<32><<<nb, c.nx, lds, c.stream>>>(fin, fout, c.lp, strips);
+    return hipGetLastError();
 }
 
 hipError_t launch_lbm_fused(Ctx& c, const float* fin, float* fout, int steps) {
     const bool force = c.lp.has_force || c.lp.fric_over_dt != 0.f;
-    if (steps != 1 && !(steps == 2 && lbm_launch_steps(c) == 2)) return hipErrorInvalidValue;
+    if (steps < 1 || steps > lbm_launch_steps(c)) return hipErrorInvalidValue;
     if (c.lbm_tiles_per_block <= -LBM_TWO_STEP) {
-        // full-row strips of h rows, two steps per launch (or one, for a remaining
-        // odd step or a forced flow)
-        const int h = -c.lbm_tiles_per_block - LBM_TWO_STEP;
-        if (steps == 2) {
-            if (!lbm_rows2_supported(c.nx, c.ny, h)) return hipErrorInvalidValue;
-            const int strips = c.ny / h;
-            const unsigned nb = unsigned(strips * c.M);
-            const size_t lds = sizeof(float) * (12 + 3 * P1Q) * size_t(c.nx);
-            const void* fn = h == 16 ? reinterpret_cast<const void*>(&k_lbm_rows2<16>)
-                                     : reinterpret_cast<const void*>(&k_lbm_rows2<32>);
-            if (lds > 65536) {  // beyond the default dynamic LDS of a workgroup
-                const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
-                if (e != hipSuccess) return e;
-            }
-            if (h == 16)
-                k_lbm_rows2<16><<<nb, c.nx, lds, c.stream>>>(fin, fout, c.lp, strips);
-            else
-                k_lbm_rows2<32><<<nb, c.nx, lds, c.stream>>>(fin, fout, c.lp, strips);
-            return hipGetLastError();
-        }
+        // full-row strips of h rows taking three (or two) steps per launch, or one,
+        // for the last steps of a forecast or a forced flow
+        const int h = -c.lbm_tiles_per_block - (c.lbm_tiles_per_block <= -LBM_THREE_STEP ? LBM_THREE_STEP : LBM_TWO_STEP);
+        if (steps == 3) return launch_rows3(c, fin, fout, h);
+        if (steps == 2) return launch_rows2(c, fin, fout, h);
         return launch_rows1(c, fin, fout, h, force);
     }
     if (c.lbm_tiles_per_block < 0) return launch_rows1(c, fin, fout, -c.lbm_tiles_per_block, force);

@@ -25,6 +25,8 @@ def test_more_than_one_gpu_is_refused():
 
 def test_steps_per_launch_of_the_shape_codes():
     b = _bench()
+    assert b.lbm_steps_per_launch([-216, 32]) == 3
+    assert b.lbm_steps_per_launch([-232, 32]) == 3
     assert b.lbm_steps_per_launch([-116, 32]) == 2
     assert b.lbm_steps_per_launch([-132, 16]) == 2
     assert b.lbm_steps_per_launch([-16, 32]) == 1

@@ -131,15 +131,16 @@ def test_full_size_batched_step(lb, orc):
     assert _rel(f3, f0) < TOL_F
 
 
-@pytest.mark.parametrize("shape", [-116, -132])
-def test_two_step_strips_are_bit_identical(lb, orc, shape):
-    """Two steps per launch (no body force, no friction): the same per-node
-    operations as single steps, so 7 steps (three two-step launches and one single
-    step) give the single steps' state bit for bit, and the oracle's to TOL_F."""
+@pytest.mark.parametrize("shape", [-116, -132, -216, -232])
+def test_multi_step_strips_are_bit_identical(lb, orc, shape):
+    """Two or three steps per launch (no body force, no friction): the same
+    per-node operations as single steps, so 7 steps (three two-step launches and a
+    single step, or two three-step launches and a single step) give the single
+    steps' state bit for bit, and the oracle's to TOL_F."""
     cfg = small_cfg(lb, "baseline", n=128)
     cfg.variant = lb.VARIANT_MIXED
     M = 3
-    print(f"case: two-step strips {shape} 128x128 M={M}", flush=True)
+    print(f"case: multi-step strips {shape} 128x128 M={M}", flush=True)
     out = []
     for sh in (1, -16, shape):
         with _ensemble(lb, cfg, M) as ctx:
@@ -173,18 +174,53 @@ def test_wide_two_step_strips(lb, nx):
     np.testing.assert_array_equal(out[2], out[0])
 
 
+@pytest.mark.parametrize("steps", [1, 2, 3, 5, 100])
+def test_three_step_strips_take_any_step_count(lb, steps):
+    """A forecast of any length in the three-step shape (three-step launches, then
+    a two-step or one-step launch for what is left) equals the tile shape's."""
+    cfg = small_cfg(lb, "baseline", n=64)
+    cfg.nx, cfg.ny = 256, 64
+    cfg.variant = lb.VARIANT_MIXED
+    print(f"case: three-step strips, {steps} steps, 256x64 M=2", flush=True)
+    out = []
+    for sh in (1, -216):
+        with _ensemble(lb, cfg, 2) as ctx:
+            ctx.set_tuning(sh, 8)
+            ctx.forecast(steps)
+            out.append(ctx.get_all_states())
+    np.testing.assert_array_equal(out[1], out[0])
+
+
+def test_wide_three_step_strips(lb):
+    """Rows of 512 nodes in the three-step strips (110 KB of LDS per workgroup):
+    the same bits as the tile shape over 7 steps."""
+    cfg = small_cfg(lb, "baseline", n=64)
+    cfg.nx, cfg.ny = 512, 64
+    cfg.variant = lb.VARIANT_MIXED
+    print("case: three-step strips 512x64 M=2", flush=True)
+    out = []
+    for sh in (1, -216, -232):
+        with _ensemble(lb, cfg, 2) as ctx:
+            ctx.set_tuning(sh, 8)
+            ctx.forecast(7)
+            out.append(ctx.get_all_states())
+    np.testing.assert_array_equal(out[1], out[0])
+    np.testing.assert_array_equal(out[2], out[0])
+
+
 def test_two_step_strips_with_forcing_fall_back(lb):
     """A forced flow (ref preset) steps one step per launch in the two-step shape:
     the state equals the one-step strips' bit for bit."""
     cfg = small_cfg(lb, "ref", n=128)
     cfg.variant = lb.VARIANT_MIXED
     out = []
-    for sh in (-16, -116):
+    for sh in (-16, -116, -216):
         with _ensemble(lb, cfg, 2) as ctx:
             ctx.set_tuning(sh, 8)
             ctx.forecast(4)
             out.append(ctx.get_all_states())
     np.testing.assert_array_equal(out[1], out[0])
+    np.testing.assert_array_equal(out[2], out[0])
 
 
 @pytest.mark.parametrize("shape", [1, 2, -8, -16, -32])

@@ -63,3 +63,8 @@ def test_set_tuning_rejects(lb):
         for bad in (-108, -100, -50):  # two-step strips are 16 or 32 rows
             with pytest.raises(lb.LbmLetkfError):
                 ctx.set_tuning(bad, 2)
+        ctx.set_tuning(-216, 2)  # three-step strips of 16 rows
+        assert ctx.get_tuning() == (-216, 2)
+        for bad in (-208, -200, -264):  # three-step strips are 16 or 32 rows
+            with pytest.raises(lb.LbmLetkfError):
+                ctx.set_tuning(bad, 2)
