@@ -224,6 +224,18 @@ int lbmletkf_local_obs(lbmletkf_ctx* ctx, int gridpoint, int* n, int* sites, flo
  * Diagnostics counterpart of the reference's HDF5 dumps (src/util/letkf_solver.h:244-345). */
 int lbmletkf_set_dump(lbmletkf_ctx* ctx, int g0, int count);
 int lbmletkf_get_dump(lbmletkf_ctx* ctx, double* W, double* w, double* evals);
+/* The eigensolve side of the same dump (src/util/letkf_solver.h:235-272, the _Q and
+ * _V matrices of solve_evd): per gridpoint C = (M-1)/beta I + dY^T R~^-1 dY as the
+ * kernels formed it (M x M row-major), the eigenvectors V (row-major, column j the
+ * eigenvector of evals[j]) and the eigenvalues in the solver's order (unsorted, as
+ * syevjBatched returns them with sorting off), in FP64. */
+int lbmletkf_get_dump_evd(lbmletkf_ctx* ctx, double* C, double* V, double* evals);
+
+/* The observation-space ensemble of the last analysis (src/util/letkf_solver.h:
+ * 238-243: dY = Y - ybar, dyo = yo - ybar): dy [n_obs][M] and dyo [n_obs] in the
+ * observation order of lbmletkf_obs_count (site-major, variables u, v, rho as
+ * configured).  With lbmletkf_local_obs they give each gridpoint's _dYf and _dyo. */
+int lbmletkf_get_obs_ensemble(lbmletkf_ctx* ctx, float* dy, float* dyo);
 
 /* Phase timers on the context's stream (HIP events around each phase), the
  * counterpart of the reference's util::timer tags (src/util/timer.hpp:17-85,

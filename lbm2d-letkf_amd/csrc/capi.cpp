@@ -1118,7 +1118,7 @@ int lbmletkf_set_dump(lbmletkf_ctx* ctx, int g0, int count) {
     c->dump_g0 = g0;
     c->dump_n = count;
     if (count > 0) {
-        const size_t per = size_t(c->M) * c->M + 2 * size_t(c->M);
+        const size_t per = size_t(dump_record(c->M));
         HIP_TRY(c, dalloc(&c->d_dump, per * count));
         HIP_TRY(c, hipMemset(c->d_dump, 0, sizeof(double) * per * count));
     }
@@ -1131,7 +1131,7 @@ int lbmletkf_get_dump(lbmletkf_ctx* ctx, double* W, double* w, double* evals) {
     if (c->dump_n == 0) return fail(c, LBMLETKF_ERR_STATE, "no dump range set");
     if (int st = sync_and_check(c)) return st;
     const int M = c->M;
-    const size_t per = size_t(M) * M + 2 * size_t(M);
+    const size_t per = size_t(dump_record(M));
     std::vector<double> h(per * c->dump_n);
     HIP_TRY(c, hipMemcpy(h.data(), c->d_dump, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
     for (int g = 0; g < c->dump_n; ++g) {
@@ -1177,6 +1177,38 @@ int lbmletkf_get_dump(lbmletkf_ctx* ctx, double* W, double* w, double* evals) {
             std::copy(d.begin(), d.end(), evals + size_t(M) * g);
         }
     }
+    return LBMLETKF_OK;
+}
+
+int lbmletkf_get_dump_evd(lbmletkf_ctx* ctx, double* C, double* V, double* evals) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    CHECK_CTX(c);
+    if (c->dump_n == 0) return fail(c, LBMLETKF_ERR_STATE, "no dump range set");
+    if (int st = sync_and_check(c)) return st;
+    const int M = c->M;
+    const size_t per = size_t(dump_record(M)), MM = size_t(M) * M;
+    std::vector<double> h(per * c->dump_n);
+    HIP_TRY(c, hipMemcpy(h.data(), c->d_dump, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+    for (int g = 0; g < c->dump_n; ++g) {
+        const double* src = h.data() + per * g;
+        // MIXED records V and d first; PLAIN records W, w, d, and V after C
+        const double* Vs = c->dump_kind == 1 ? src : src + MM + 2 * M + MM;
+        const double* ds = c->dump_kind == 1 ? src + MM : src + MM + M;
+        if (C) std::copy(src + MM + 2 * M, src + MM + 2 * M + MM, C + MM * g);
+        if (V) std::copy(Vs, Vs + MM, V + MM * g);
+        if (evals) std::copy(ds, ds + M, evals + size_t(M) * g);
+    }
+    return LBMLETKF_OK;
+}
+
+int lbmletkf_get_obs_ensemble(lbmletkf_ctx* ctx, float* dy, float* dyo) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    CHECK_CTX(c);
+    if (!c->obs_ready || !c->d_dy) return fail(c, LBMLETKF_ERR_STATE, "no analysis has run");
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    const size_t n_obs = size_t(c->obs.n_sites) * c->obs.n_vars;
+    if (dy) HIP_TRY(c, hipMemcpy(dy, c->d_dy, sizeof(float) * n_obs * c->M, hipMemcpyDeviceToHost));
+    if (dyo) HIP_TRY(c, hipMemcpy(dyo, c->d_dyo, sizeof(float) * n_obs, hipMemcpyDeviceToHost));
     return LBMLETKF_OK;
 }
 

@@ -14,6 +14,8 @@ constexpr int Q = 9;
 constexpr int MAX_MEMBERS = 64;      // one wavefront of lanes per ensemble matrix
 constexpr int MAX_FORCE_MODES = 100; // src/data.cu:191-212 prunes to 100 modes
 constexpr int MIXED_GROUP_MAX = 8;   // contexts in one group launch
+// doubles per gridpoint of the transform dump (layout at Ctx::d_dump)
+__host__ __device__ constexpr long long dump_record(int M) { return 3LL * M * M + 2LL * M; }
 constexpr int LBM_TWO_STEP = 100;    // shape code offset of the two-step strips
 constexpr int LBM_THREE_STEP = 200;  // shape code offset of the three-step strips
 
@@ -121,9 +123,14 @@ struct Ctx {
     // three steps per launch (k_lbm_rows3)
     int lbm_tiles_per_block = 1;
 
-    // transform dump
+    // transform dump: per gridpoint a record of dump_record(M) doubles,
+    //   [0, M^2)          W (PLAIN) or V (MIXED), row-major
+    //   [M^2, M^2+M)      w (PLAIN) or the eigenvalues d (MIXED)
+    //   [M^2+M, M^2+2M)   d (PLAIN) or b = G^T sqrt(R^-1) dyo (MIXED)
+    //   [M^2+2M, 2M^2+2M) C, the matrix the eigensolve took, row-major
+    //   [2M^2+2M, 3M^2+2M) V (PLAIN), row-major
     int dump_g0 = 0, dump_n = 0;
-    double* d_dump = nullptr;     // [dump_n][M*M + 2M]
+    double* d_dump = nullptr;     // [dump_n][dump_record(M)]
 
     // scratch for reductions
     double* d_red = nullptr;      // [red_cap]

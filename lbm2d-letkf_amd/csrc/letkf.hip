@@ -331,6 +331,10 @@ __global__ __launch_bounds__(64) void k_solve_fp64(const float* __restrict__ gra
         Vt[i * LD + j] = (i == j) ? 1.0 : 0.0;
     }
     for (int i = lane; i < N; i += 64) bv[i] = (i < M) ? double(bvec[(long long)g * M + i]) : 0.0;
+    const bool dumping = (g >= sp.dump_g0) && (g < sp.dump_g0 + sp.dump_n);
+    double* dg = dumping ? dump + (long long)(g - sp.dump_g0) * dump_record(M) : nullptr;
+    if (dumping)  // C as the eigensolve takes it
+        for (int idx = lane; idx < M * M; idx += 64) dg[M * M + 2 * M + idx] = double(Cg[idx]);
     __syncthreads();
 
     int sweep = 0;
@@ -423,8 +427,8 @@ __global__ __launch_bounds__(64) void k_solve_fp64(const float* __restrict__ gra
     }
     __syncthreads();
     // column `lane` of W into A (A is no longer needed): T = W + w 1^T
-    const bool dumping = (g >= sp.dump_g0) && (g < sp.dump_g0 + sp.dump_n);
-    double* dg = dumping ? dump + (long long)(g - sp.dump_g0) * (M * M + 2 * M) : nullptr;
+    if (dumping && lane < M)  // V[k][lane] = Vt[lane][k], the eigenvectors in the solver's order
+        for (int k = 0; k < M; ++k) dg[2 * M * M + 2 * M + k * M + lane] = Vt[lane * LD + k];
     if (lane < N) {
         double ucol[N];
 #pragma unroll

@@ -588,7 +588,7 @@ __device__ __forceinline__ void solve_body(int bx, int nbx, const int* __restric
                                            const int* __restrict__ loc_site, const float* __restrict__ loc_w,
                                            const float* __restrict__ dy, const float* __restrict__ dyo,
                                            float* __restrict__ ws_u, float* __restrict__ ws_b,
-                                           const MixedParams& mp, float* cfull) {
+                                           const MixedParams& mp, float* cfull, double* __restrict__ dump) {
     // cfull: N*N floats of LDS, 16-byte aligned
     const int lane = threadIdx.x;
     const int g_begin = xcd_order(bx, nbx) * mp.chain;
@@ -597,6 +597,11 @@ __device__ __forceinline__ void solve_body(int bx, int nbx, const int* __restric
         __syncthreads();  // the previous gridpoint's reads of cfull are done
         gram_to_lds<N, PASSES>(g, lane, loc_ptr, loc_site, loc_w, dy, dyo, ws_b + (long long)g * N, mp, cfull);
         __syncthreads();
+        if (dump && g >= mp.dump_g0 && g < mp.dump_g0 + mp.dump_n) {
+            double* dc = dump + (long long)(g - mp.dump_g0) * dump_record(mp.M) + mp.M * mp.M + 2 * mp.M;
+#pragma unroll 1
+            for (int idx = lane; idx < mp.M * mp.M; idx += 64) dc[idx] = double(cfull[(idx / mp.M) * N + idx % mp.M]);
+        }
         float u[N];
         if (g > g_begin) {
             warm_start<N>(RowMajorRows<N>{ws_u + (long long)(g - 1) * N * N, lane & 15}, cfull, lane);
@@ -617,9 +622,9 @@ template <int N, int PASSES>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_mixed_solve(
     const int* __restrict__ loc_ptr, const int* __restrict__ loc_site, const float* __restrict__ loc_w,
     const float* __restrict__ dy, const float* __restrict__ dyo, float* __restrict__ ws_u,
-    float* __restrict__ ws_b, MixedParams mp) {
+    float* __restrict__ ws_b, MixedParams mp, double* __restrict__ dump) {
     __shared__ __align__(16) float cfull[N * N];
-    solve_body<N, PASSES>(blockIdx.x, gridDim.x, loc_ptr, loc_site, loc_w, dy, dyo, ws_u, ws_b, mp, cfull);
+    solve_body<N, PASSES>(blockIdx.x, gridDim.x, loc_ptr, loc_site, loc_w, dy, dyo, ws_u, ws_b, mp, cfull, dump);
 }
 
 // ---------------------------------------------------------------------------------
@@ -667,6 +672,26 @@ __device__ inline void gram_add_diag(f32x4 (&acc)[GRAM_FULL<N>], int lane, int M
 #pragma unroll
         for (int r = 0; r < 4; ++r)
             if (4 * kq + r == col) acc[I * NT + I][r] += (16 * I + col < M) ? diag : 1.f;
+}
+
+// C from the FULL accumulators into a dump record (row-major M x M, FP64); the
+// element offsets come from an opaque copy of the lane index (the dump is rare and
+// its addresses must not be hoisted out of the chain loop)
+template <int N>
+__device__ inline void dump_c_acc(const f32x4 (&acc)[GRAM_FULL<N>], double* dc, int M, int lane) {
+    constexpr int NT = N / 16;
+    int ln = lane;
+    asm volatile("" : "+v"(ln));
+    const int col = ln & 15, kq = ln >> 4;
+#pragma unroll
+    for (int I = 0; I < NT; ++I)
+#pragma unroll
+        for (int J = 0; J < NT; ++J)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = 16 * I + 4 * kq + r, cc = 16 * J + col;
+                if (row < M && cc < M) dc[row * M + cc] = double(acc[I * NT + J][r]);
+            }
 }
 
 // C from the FULL accumulators into cv, row-major (the first gridpoint of a chain)
@@ -749,6 +774,8 @@ __device__ __forceinline__ void analysis_body(int bx, int nbx, const int* __rest
         f32x4 acc[GRAM_FULL<N>];
         gram_acc<N, PASSES, true>(g, lane, loc_ptr, loc_site, loc_w, dy, dyo, mp, acc);
         gram_add_diag<N>(acc, lane, M, mp.diag);
+        if (mp.write_state && g >= mp.dump_g0 && g < mp.dump_g0 + mp.dump_n)
+            dump_c_acc<N>(acc, dump + (long long)(g - mp.dump_g0) * dump_record(M) + M * M + 2 * M, M, lane);
         __syncthreads();  // the previous gridpoint's reads of stage and coef are done
         if ((lane & 15) == 0) {
 #pragma unroll
@@ -797,7 +824,7 @@ __device__ __forceinline__ void analysis_body(int bx, int nbx, const int* __rest
             // V[k][lane]: the lane's own column read back from cv and normalised again in
             // FP64, so that the dumped columns have unit norm to FP64 precision (rolled
             // loops: the dump is rare and its addresses must not be hoisted)
-            double* dg = dump + (long long)(g - mp.dump_g0) * (M * M + 2 * M);
+            double* dg = dump + (long long)(g - mp.dump_g0) * dump_record(M);
             double vv = 0.0;
 #pragma unroll 1
             for (int k = 0; k < N; ++k) {
@@ -1055,7 +1082,7 @@ __device__ __forceinline__ void finish_body(int bx, int nbx, const float* __rest
     const double s_i = (lane < N) ? mp.sqm1 / sqrt(d_i) : 0.0;
     const bool dumping = (g >= mp.dump_g0) && (g < mp.dump_g0 + mp.dump_n);
     if (dumping && lane < M) {
-        double* dg = dump + (long long)(g - mp.dump_g0) * (M * M + 2 * M);
+        double* dg = dump + (long long)(g - mp.dump_g0) * dump_record(M);
 #pragma unroll
         for (int k = 0; k < H; ++k)  // static indices keep ud[] in registers
             if (r0 + k < M) dg[(r0 + k) * M + lane] = ud[k];  // V[r0+k][lane]
@@ -1181,9 +1208,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     const int bx = int(blockIdx.x) - T.b0_solve;
     __shared__ __align__(16) float cfull[64 * 64];  // one buffer for every width
     switch (T.N) {
-        case 16: solve_body<16, PASSES>(bx, T.nb_solve, T.loc_ptr, T.loc_site, T.loc_w, T.dy, T.dyo, T.ws_u, T.ws_b, T.mp, cfull); break;
-        case 32: solve_body<32, PASSES>(bx, T.nb_solve, T.loc_ptr, T.loc_site, T.loc_w, T.dy, T.dyo, T.ws_u, T.ws_b, T.mp, cfull); break;
-        default: solve_body<64, PASSES>(bx, T.nb_solve, T.loc_ptr, T.loc_site, T.loc_w, T.dy, T.dyo, T.ws_u, T.ws_b, T.mp, cfull); break;
+        case 16: solve_body<16, PASSES>(bx, T.nb_solve, T.loc_ptr, T.loc_site, T.loc_w, T.dy, T.dyo, T.ws_u, T.ws_b, T.mp, cfull, T.dump); break;
+        case 32: solve_body<32, PASSES>(bx, T.nb_solve, T.loc_ptr, T.loc_site, T.loc_w, T.dy, T.dyo, T.ws_u, T.ws_b, T.mp, cfull, T.dump); break;
+        default: solve_body<64, PASSES>(bx, T.nb_solve, T.loc_ptr, T.loc_site, T.loc_w, T.dy, T.dyo, T.ws_u, T.ws_b, T.mp, cfull, T.dump); break;
     }
 }
 
@@ -1309,7 +1336,8 @@ hipError_t launch_letkf_mixed(Ctx& c, float* f, unsigned long long* sweeps32, un
     const bool sweep64 = mp.max64 > 0;
 #define LAUNCH_SOLVE(NN, PP)                                                                           \
     k_mixed_solve<NN, PP><<<grid_solve, block, 0, c.stream>>>(c.d_loc_ptr, c.d_loc_site, c.d_loc_w, c.d_dy, \
-                                                         c.d_dyo, c.d_mix_u, c.d_mix_b, mp)
+                                                         c.d_dyo, c.d_mix_u, c.d_mix_b, mp, \
+                                                         solve_only ? nullptr : c.d_dump)
 #define LAUNCH_FINISH(NN)                                                                                     \
     (sweep64 ? k_mixed_finish<NN, true><<<grid, block2, 0, c.stream>>>(c.d_mix_u, c.d_mix_b, f, mp, c.d_dump,  \
                                                                         c.d_status)                             \

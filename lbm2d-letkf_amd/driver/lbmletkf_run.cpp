@@ -260,7 +260,7 @@ std::vector<float> state_matrices(const Ctx& c, int M, size_t nodes, long long g
 // before and after the analysis: _Xf, _dXf, _Pa = V D^-1 V^T = W W / (M-1),
 // _dWa = W, _Xa, _Wa = W + w 1^T, _AI = Xa - Xf, _Z = Wa - I, _Lambda = diag(d),
 // _Wma = w 1^T.  W, w and d are the FP64 results of the analysis kernels; the
-// eigenvalues are in ascending order.
+// eigenvalues are in the solver's order (that of the columns of _V).
 void save_matrices(const std::string& dir, long long t, int M, long long n, const std::vector<float>& Xf,
                    const std::vector<float>& Xa, const std::vector<double>& W, const std::vector<double>& w,
                    const std::vector<double>& d) {
@@ -304,6 +304,53 @@ void save_matrices(const std::string& dir, long long t, int M, long long n, cons
     save_matrix(dir, "_Z", M, M, N, t, Z);
     save_matrix(dir, "_Lambda", M, M, N, t, Lam);
     save_matrix(dir, "_Wma", M, M, N, t, Wma);
+}
+
+// The reference's solve_evd diagnostics (src/util/letkf_solver.h:235-272): per
+// gridpoint _dYf (local observations x members: the rows of dY = Y - ybar the
+// gridpoint's analysis gathers, in its gather order), _dyo (local observations x 1),
+// _Q (C = (M-1)/beta I + dY^T R~^-1 dY as the kernels formed it) and _V (its
+// eigenvectors, column j for eigenvalue j of _Lambda).  The reference's windows have
+// a fixed row count; here the local sets vary per gridpoint (Gaussian localization)
+// and the row count is the largest of the range, shorter sets padded with zeros.
+void save_evd_matrices(Ctx& c, const std::string& dir, long long t, int M, long long g0, long long n,
+                       int n_sites, int n_obs) {
+    using lbmletkf_io::save_matrix;
+    const size_t MM = size_t(M) * M, N = size_t(n);
+    const int nv = n_sites > 0 ? n_obs / n_sites : 1;
+    std::vector<double> C(MM * N), V(MM * N), ev(size_t(M) * N);
+    c.ck(lbmletkf_get_dump_evd(c.h, C.data(), V.data(), ev.data()), "get_dump_evd");
+    std::vector<float> dy(size_t(std::max(n_obs, 1)) * M), dyo(size_t(std::max(n_obs, 1)));
+    c.ck(lbmletkf_get_obs_ensemble(c.h, dy.data(), dyo.data()), "get_obs_ensemble");
+    std::vector<std::vector<int>> sets(N);
+    size_t rows = 1;
+    for (size_t b = 0; b < N; ++b) {
+        int k = 0;
+        std::vector<int> sites(size_t(std::max(n_sites, 1)));
+        c.ck(lbmletkf_local_obs(c.h, int(g0 + (long long)b), &k, sites.data(), nullptr, int(sites.size())), "local_obs");
+        sites.resize(size_t(std::min(k, int(sites.size()))));
+        sets[b] = sites;
+        rows = std::max(rows, sites.size() * size_t(nv));
+    }
+    std::vector<float> dYf(rows * M * N, 0.f), dyol(rows * N, 0.f);
+    for (size_t b = 0; b < N; ++b)
+        for (size_t e = 0; e < sets[b].size(); ++e)
+            for (int a = 0; a < nv; ++a) {
+                const size_t r = e * nv + a, o = size_t(sets[b][e]) * nv + a;
+                for (int m = 0; m < M; ++m) dYf[r + rows * (m + size_t(M) * b)] = dy[o * M + m];
+                dyol[r + rows * b] = dyo[o];
+            }
+    std::vector<double> Q(MM * N), Vc(MM * N);
+    for (size_t b = 0; b < N; ++b)
+        for (int col = 0; col < M; ++col)
+            for (int r = 0; r < M; ++r) {
+                Q[r + size_t(M) * col + MM * b] = C[MM * b + size_t(r) * M + col];
+                Vc[r + size_t(M) * col + MM * b] = V[MM * b + size_t(r) * M + col];
+            }
+    save_matrix(dir, "_dYf", rows, M, N, t, dYf);
+    save_matrix(dir, "_dyo", rows, 1, N, t, dyol);
+    save_matrix(dir, "_Q", M, M, N, t, Q);
+    save_matrix(dir, "_V", M, M, N, t, Vc);
 }
 
 int run(const Options& o) {
@@ -380,9 +427,12 @@ int run(const Options& o) {
             std::printf("t=%lld: analysis with %d observations\n", t, n_obs);
             if (o.savemat) {
                 const size_t nm = size_t(mat_n);
-                std::vector<double> W(nm * M * M), w(nm * M), d(nm * M);
+                std::vector<double> W(nm * M * M), w(nm * M), d(nm * M), ev(nm * M);
                 c.ck(lbmletkf_get_dump(c.h, W.data(), w.data(), d.data()), "get_dump");
-                save_matrices(mat_dir, t, M, mat_n, Xf, state_matrices(c, M, n, mat_g0, mat_n), W, w, d);
+                // _Lambda in the solver's order, the order of the columns of _V
+                c.ck(lbmletkf_get_dump_evd(c.h, nullptr, nullptr, ev.data()), "get_dump_evd");
+                save_matrices(mat_dir, t, M, mat_n, Xf, state_matrices(c, M, n, mat_g0, mat_n), W, w, ev);
+                save_evd_matrices(c, mat_dir, t, M, mat_g0, mat_n, n_sites, n_obs);
                 std::printf("t=%lld: LETKF matrices of %lld gridpoints -> %s\n", t, mat_n, mat_dir.c_str());
             }
         }

@@ -102,6 +102,8 @@ def _load():
         "lbmletkf_local_obs": ([P, I, C.POINTER(I), P, P, I], I),
         "lbmletkf_set_dump": ([P, I, I], I),
         "lbmletkf_get_dump": ([P, P, P, P], I),
+        "lbmletkf_get_dump_evd": ([P, P, P, P], I),
+        "lbmletkf_get_obs_ensemble": ([P, P, P], I),
         "lbmletkf_profile_enable": ([P, I], I),
         "lbmletkf_profile_read": ([P, C.c_char_p, C.POINTER(D), C.POINTER(C.c_longlong),
                                    C.POINTER(C.c_longlong)], I),
@@ -295,6 +297,25 @@ class Context:
         self._ck(lib.lbmletkf_get_dump(self.h, _ptr(W), _ptr(w), _ptr(ev)))
         return W, w, ev
 
+    def get_dump_evd(self):
+        """(C, V, evals) of the dumped gridpoints: C and V [count][M][M] (row-major, column
+        j of V the eigenvector of evals[:, j]), eigenvalues in the solver's order."""
+        g0, count = self._dump
+        M = self.M
+        Cm = np.empty((count, M, M))
+        V = np.empty((count, M, M))
+        ev = np.empty((count, M))
+        self._ck(lib.lbmletkf_get_dump_evd(self.h, _ptr(Cm), _ptr(V), _ptr(ev)))
+        return Cm, V, ev
+
+    def get_obs_ensemble(self):
+        """(dy [n_obs][M], dyo [n_obs]) of the last analysis."""
+        _, n_obs = self.obs_count()
+        dy = np.empty((n_obs, self.M), np.float32)
+        dyo = np.empty(n_obs, np.float32)
+        self._ck(lib.lbmletkf_get_obs_ensemble(self.h, _ptr(dy), _ptr(dyo)))
+        return dy, dyo
+
     def ensemble_stats(self, truth_rho, truth_u, truth_v):
         arrs = [np.ascontiguousarray(x, np.float32) for x in (truth_rho, truth_u, truth_v)]
         out = np.empty(4)
@@ -322,7 +343,8 @@ class Context:
 
     def set_tuning(self, lbm_tiles_per_block: int, mix_chain: int):
         """LBM shape (k > 0 tiles per workgroup, -h row strips, -(100 + h) two-step
-        row strips) and FP32 solve chain length; see lbmletkf_set_tuning."""
+        and -(200 + h) three-step row strips) and analysis chain length; see
+        lbmletkf_set_tuning."""
         self._ck(lib.lbmletkf_set_tuning(self.h, lbm_tiles_per_block, mix_chain))
 
     def get_tuning(self):

@@ -116,7 +116,7 @@ def _read_matrix(path, rows, cols, batch):
     return a.reshape(batch, cols, rows)
 
 
-def test_savemat_letkf_matrices(tmp_path):
+def test_savemat_letkf_matrices(tmp_path, lb, orc):
     io = str(tmp_path / "io")
     short = [a for a in COMMON]
     short[short.index("--outputs") + 1] = "5"
@@ -127,7 +127,7 @@ def test_savemat_letkf_matrices(tmp_path):
     names = sorted(os.listdir(d))
     print(names)
     ext = names[0].rsplit(".", 1)[1]
-    assert ext in ("h5", "bin") and len(names) == 10
+    assert ext in ("h5", "bin") and len(names) == 14
 
     def rd(label, rows, typ):
         return _read_matrix(os.path.join(d, f"{label}.{typ}_{rows}_{M}_{n}.{t}.{ext}"), rows, M, n)
@@ -150,9 +150,39 @@ def test_savemat_letkf_matrices(tmp_path):
     # (dY 1 = 0 up to the FP32 rounding of the perturbations and the split-bf16 Gram)
     print("max |W 1 - sqrt(beta)|:", np.abs(dWa.sum(axis=2) - np.sqrt(beta)).max())
     np.testing.assert_allclose(dWa.sum(axis=2), np.sqrt(beta), rtol=1e-4)
-    ev = np.diagonal(Lam, axis1=1, axis2=2)
-    assert (np.diff(ev, axis=1) >= 0).all()
-    np.testing.assert_allclose(np.sort(1 / np.linalg.eigvalsh(Pa), axis=1), ev, rtol=1e-8)
+    ev = np.diagonal(Lam, axis1=1, axis2=2)  # the solver's order (unsorted), that of the columns of _V
+    np.testing.assert_allclose(np.sort(1 / np.linalg.eigvalsh(Pa), axis=1), np.sort(ev, axis=1), rtol=1e-8)
+
+    # solve_evd side: _Q = C, _V its eigenvectors in the order of _Lambda, _dYf and _dyo
+    # the local rows of dY and of the innovation in the gridpoint's gather order
+    rows = int(next(f for f in names if f.startswith("_dYf.")).split("_")[2])
+    Q, V = (T(rd(k, M, "double")) for k in ("_Q", "_V"))
+    dYf = T(rd("_dYf", rows, "float")).astype(np.float64)
+    dyo = _read_matrix(os.path.join(d, f"_dyo.float_{rows}_1_{n}.{t}.{ext}"), rows, 1, n)[:, 0, :]
+    np.testing.assert_allclose(np.swapaxes(V, 1, 2) @ V, np.broadcast_to(np.eye(M), V.shape), atol=1e-5)
+    np.testing.assert_allclose(V @ (ev[:, :, None] * np.swapaxes(V, 1, 2)), Q, rtol=0, atol=1e-5 * np.abs(Q).max())
+    np.testing.assert_allclose(dYf.sum(axis=2), 0, atol=1e-6)
+    # against the FP64 oracle: C and the transform (W, w) of each gridpoint formed from
+    # the dumped dY and innovation with the oracle's local weights
+    cfg = lb.preset("baseline")
+    cfg.nx = cfg.ny = N
+    cfg.obs_stride, cfg.loc_scale, cfg.loc_cutoff = 4, 6.0, 14.0
+    nv = 2  # (u, v)
+    rinv_u = 1.0 / cfg.obs_sigma_u ** 2
+    worst = dict(C=0.0, W=0.0, w=0.0)
+    for b in range(n):
+        sites, wl = orc.local_obs(cfg, g0 + b)
+        p = len(sites) * nv
+        assert p <= rows and not dYf[b, p:].any()
+        dY = dYf[b, :p]
+        rinv = np.repeat(wl, nv) * rinv_u
+        C_o = (M - 1) / beta * np.eye(M) + dY.T @ (rinv[:, None] * dY)
+        W_o, w_o, _ = orc.letkf_transform(dY, rinv, dyo[b, :p].astype(np.float64), beta)
+        worst["C"] = max(worst["C"], np.linalg.norm(Q[b] - C_o) / np.linalg.norm(C_o))
+        worst["W"] = max(worst["W"], np.linalg.norm(dWa[b] - W_o) / np.linalg.norm(W_o))
+        worst["w"] = max(worst["w"], np.linalg.norm(Wma[b][:, 0] - w_o) / max(np.linalg.norm(w_o), 1e-30))
+    print("worst rel. difference from the oracle over the dumped gridpoints:", worst)
+    assert worst["C"] < 1e-4 and worst["W"] < 1e-4 and worst["w"] < 1e-3
     # the states of the matrices are those of the run's own output at that step
     u_calc = _read(io, "calc", 1, "u", 5)
     assert np.isfinite(u_calc).all()

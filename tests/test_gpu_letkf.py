@@ -111,8 +111,36 @@ def test_analysis_matches_oracle(lb, orc, case, form):
         ens.analysis(yo)
         fa = ens.get_all_states().astype(np.float64)
         W, w, ev = ens.get_dump()
+        Cd, Vd, evd = ens.get_dump_evd()
     fo = fb.copy()
     Wo, wo, evo = orc.letkf_analysis(cfg, fo, yo.astype(np.float64), g0, nd)
+    # the eigensolve side of the dump: C against the oracle's C of the same local
+    # problem, and V diag(evals) V^T = C with V orthonormal (solver order, unsorted)
+    hx = orc.obs_moments(cfg, fb)
+    dy_all = hx - hx.mean(axis=1, keepdims=True)
+    nv = len(dy_all) // orc.obs_count(cfg)[0]
+    rinv_var = {0: 1 / cfg.obs_sigma_u ** 2, 1: 1 / cfg.obs_sigma_u ** 2, 2: 1 / cfg.obs_sigma_rho ** 2}
+    var_ids = [v for v, bit in ((0, 1), (1, 2), (2, 4)) if cfg.obs_vars & bit]
+    # V's orthogonality: FP64 rotations (plain) keep it to rounding; the FP32 phase
+    # of the mixed solver accumulates ~sweeps * M rotations of FP32 rounding each
+    # (measured 1.3e-5 at M = 32 with the near-singular rloc problem).
+    tol_orth = 1e-12 if form == "plain" else 4e-5
+    errC = errEVD = errOrth = 0.0
+    for i in range(nd):
+        sites, wl = orc.local_obs(cfg, g0 + i)
+        rows = [s_ * nv + a for s_ in sites for a in range(nv)]
+        rinv = np.array([wl[e] * rinv_var[var_ids[a]] for e in range(len(sites)) for a in range(nv)])
+        dY = dy_all[rows]
+        C_o = (M - 1) / cfg.inflation * np.eye(M) + dY.T @ (rinv[:, None] * dY)
+        errC = max(errC, _rel(Cd[i], C_o))
+        errEVD = max(errEVD, _rel(Vd[i] @ np.diag(evd[i]) @ Vd[i].T, Cd[i]))
+        errOrth = max(errOrth, float(np.abs(Vd[i].T @ Vd[i] - np.eye(M)).max()))
+        np.testing.assert_allclose(np.sort(evd[i]), ev[i], rtol=1e-12)
+    print(f"  C rel {errC:.2e}  V diag(evals) V^T vs C rel {errEVD:.2e}  max|V^T V - I| {errOrth:.2e}",
+          flush=True)
+    assert errC < tol_w
+    assert errOrth < tol_orth
+    assert errEVD < 1e-5
     errW = max(_rel(W[i], Wo[i]) for i in range(nd))
     errw = max(np.abs(w[i] - wo[i]).max() / max(np.abs(wo[i]).max(), 1e-30) for i in range(nd))
     errev = max(np.abs(ev[i] - evo[i]).max() / np.abs(evo[i]).max() for i in range(nd))
