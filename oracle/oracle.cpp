@@ -597,8 +597,10 @@ int orc_letkf_transform(int M, int p, const double* dY, const double* rinv, cons
     return sweeps;
 }
 
-int orc_letkf_analysis(const orc_config* cfg, int M, double* f, const double* yo, int dump_g0,
-                       int dump_n, double* W, double* w, double* evals, int g_stride) {
+// the analysis of gridpoints g_begin, g_begin + g_stride, ... < g_end
+static int letkf_analysis_range(const orc_config* cfg, int M, double* f, const double* yo, long long g_begin,
+                                long long g_end, int g_stride, int dump_g0, int dump_n, double* W, double* w,
+                                double* evals) {
     const orc_config& c = *cfg;
     const ObsInfo o = obs_info(c);
     const long long N = (long long)c.nx * c.ny;
@@ -618,7 +620,7 @@ int orc_letkf_analysis(const orc_config* cfg, int M, double* f, const double* yo
         std::vector<int> sites(cap);
         std::vector<double> wl(cap), dY, rinv, dyo, Wl(size_t(M) * M), wv(M), ev(M);
 #pragma omp for schedule(dynamic, 64)
-        for (long long g = 0; g < N; g += g_stride) {
+        for (long long g = g_begin; g < g_end; g += g_stride) {
             const int n = std::min(orc_local_obs(cfg, int(g), sites.data(), wl.data(), cap), cap);
             dY.assign(size_t(n) * o.n_vars * M, 0.0);
             rinv.assign(size_t(n) * o.n_vars, 0.0);
@@ -658,6 +660,19 @@ int orc_letkf_analysis(const orc_config* cfg, int M, double* f, const double* yo
         }
     }
     return bad ? -3 : 0;
+}
+
+int orc_letkf_analysis(const orc_config* cfg, int M, double* f, const double* yo, int dump_g0,
+                       int dump_n, double* W, double* w, double* evals, int g_stride) {
+    return letkf_analysis_range(cfg, M, f, yo, 0, (long long)cfg->nx * cfg->ny, g_stride, dump_g0, dump_n, W, w,
+                                evals);
+}
+
+int orc_letkf_analysis_strip(const orc_config* cfg, int M, double* f, const double* yo, int g0, int n,
+                             double* W, double* w, double* evals) {
+    const long long N = (long long)cfg->nx * cfg->ny;
+    if (g0 < 0 || n < 0 || g0 + (long long)n > N) return -1;
+    return letkf_analysis_range(cfg, M, f, yo, g0, (long long)g0 + n, 1, g0, n, W, w, evals);
 }
 
 void orc_ensemble_stats(const orc_config* cfg, int M, const double* f, const double* trho,

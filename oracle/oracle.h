@@ -93,6 +93,12 @@ int  orc_letkf_transform(int M, int p, const double* dY, const double* rinv, con
 int  orc_letkf_analysis(const orc_config* cfg, int M, double* f, const double* yo,
                         int dump_g0, int dump_n, double* W, double* w, double* evals, int g_stride);
 
+/* the analysis of gridpoints [g0, g0+n) only (f updated there), with W, w, evals
+ * of each; the observation moments still come from the whole ensemble.  Returns
+ * 0, -1 on a range outside the grid, -3 on NaN. */
+int  orc_letkf_analysis_strip(const orc_config* cfg, int M, double* f, const double* yo, int g0, int n,
+                              double* W, double* w, double* evals);
+
 /* ensemble_stat/main.cpp:108-177 with N = nx*ny: out = {rmse_uv, spread_uv, rmse_rho, spread_rho} */
 void orc_ensemble_stats(const orc_config* cfg, int M, const double* f, const double* trho,
                         const double* tu, const double* tv, double out[4]);

@@ -42,6 +42,7 @@ _sig = {
     "orc_jacobi_eig": ([I, P, P, P, D, I], I),
     "orc_letkf_transform": ([I, I, P, P, P, D, D, I, P, P, P], I),
     "orc_letkf_analysis": ([P, I, P, P, I, I, P, P, P, I], I),
+    "orc_letkf_analysis_strip": ([P, I, P, P, I, I, P, P, P], I),
     "orc_ensemble_stats": ([P, I, P, P, P, P, P], None),
     "orc_closure_features": ([I, I, D, D, P, P, P, P], None),
     "orc_closure_nut": ([P, P, C.c_longlong, D, I, P], None),
@@ -245,6 +246,21 @@ def letkf_analysis(cfg, f, yo, dump_g0=0, dump_n=0, g_stride=1):
     if st != 0:
         raise FloatingPointError("oracle analysis produced a non-finite state")
     return W[:dump_n], w[:dump_n], ev[:dump_n]
+
+
+def letkf_analysis_strip(cfg, f, yo, g0, n):
+    """Analyse gridpoints [g0, g0+n) of f (float64 [M][9][ny][nx]) in place; returns
+    (W, w, evals) of each."""
+    assert f.dtype == np.float64 and f.flags.c_contiguous
+    M = f.shape[0]
+    W = np.empty((max(n, 1), M, M))
+    w = np.empty((max(n, 1), M))
+    ev = np.empty((max(n, 1), M))
+    st = _o.orc_letkf_analysis_strip(_c(cfg), M, _p(f), _p(_f64(yo)), g0, n, _p(W), _p(w), _p(ev))
+    if st != 0:
+        raise (ValueError("strip outside the grid") if st == -1 else
+               FloatingPointError("oracle analysis produced a non-finite state"))
+    return W[:n], w[:n], ev[:n]
 
 
 def ensemble_stats(cfg, f, trho, tu, tv):

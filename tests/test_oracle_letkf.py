@@ -269,3 +269,23 @@ def test_dsyev_route_matches_the_jacobi_route(orc):
         assert np.linalg.norm(W1 - W0) <= 1e-12 * np.linalg.norm(W0)
         assert np.linalg.norm(w1 - w0) <= 1e-11 * np.linalg.norm(w0)
         np.testing.assert_allclose(e1, e0, rtol=1e-12)
+
+
+def test_strip_analysis_is_the_full_analysis_there(lb, orc):
+    # the strip form analyses [g0, g0+n) only: equal to the full analysis on the
+    # strip (and its dumped W, w, evals), the rest of f untouched
+    cfg, f, _, yo = _small_osse(lb, orc)
+    n, g0, nd = cfg.nx, 37, 150
+    fa = f.copy()
+    Wf, wf, evf = orc.letkf_analysis(cfg, fa, yo, g0, nd)
+    fs = f.copy()
+    Ws, ws, evs = orc.letkf_analysis_strip(cfg, fs, yo, g0, nd)
+    np.testing.assert_array_equal(Ws, Wf)
+    np.testing.assert_array_equal(ws, wf)
+    np.testing.assert_array_equal(evs, evf)
+    flat_a, flat_s, flat_b = (x.reshape(x.shape[0], 9, n * n) for x in (fa, fs, f))
+    np.testing.assert_array_equal(flat_s[:, :, g0:g0 + nd], flat_a[:, :, g0:g0 + nd])
+    np.testing.assert_array_equal(flat_s[:, :, :g0], flat_b[:, :, :g0])
+    np.testing.assert_array_equal(flat_s[:, :, g0 + nd:], flat_b[:, :, g0 + nd:])
+    with pytest.raises(ValueError):
+        orc.letkf_analysis_strip(cfg, f.copy(), yo, n * n - 10, 11)
