@@ -20,8 +20,10 @@ cycles; a few further cycles with phase timing on give the per-kernel breakdown.
 The mixed variant's launch shapes come from the autotuner (lbmletkf_autotune)
 before timing.  The JSON line also carries the uneven batch (four OSSEs of
 128^2..512^2 and M = 8..64 with random site densities, stepped and analysed by
-shared launches; --no-uneven skips it) and one LBM step with the learned closure
-against the Smagorinsky step.
+shared launches; --no-uneven skips it), one LBM step with the learned closure
+against the Smagorinsky step, and the mixed cycle with the learned closure in
+every forecast step (variant mixed_mlp: Glorot-initialised weights, whose values
+do not change the work of a step).
 
 Prints ONE JSON line.  --gpus above 1 is not supported (one GPU per box).
 """
@@ -106,15 +108,20 @@ class Hip:
 
 
 def bench_variant(lb, hip, cfg, variant, M, K, W, P, obs, truth, n_obs, log, repeats=1, label=None,
-                  fixed_tuning=None):
+                  fixed_tuning=None, closure=None):
     """K timed DA cycles (after W untimed ones) through lbmletkf_cycle -- for the
     mixed variant the captured graph is replayed -- then the analysis RMSE at the
     next observation time; then repeats-1 further timed runs of K cycles (their
     median and spread are reported), then P more cycles with phase timing on (eager
-    launches) for the per-kernel breakdown."""
+    launches) for the per-kernel breakdown.  With `closure` (network parameters) the
+    members step with the learned closure instead of Smagorinsky."""
     cfg = cfg.copy()
     cfg.variant = variant
+    if closure is not None:
+        cfg.closure_kind = lb.CLOSURE_MLP
     ens = lb.Context(cfg, M)
+    if closure is not None:
+        ens.set_closure(closure)
     for k in range(M):
         ens.init_member(k, 1000 + k)
     steps = cfg.da_interval
@@ -125,12 +132,13 @@ def bench_variant(lb, hip, cfg, variant, M, K, W, P, obs, truth, n_obs, log, rep
 
     ens.forecast(steps)  # to the time of the first observation
     tuning = None
+    name = label or ("plain" if variant == lb.VARIANT_PLAIN else "mixed")
     if variant == lb.VARIANT_MIXED:
         if fixed_tuning:
             ens.set_tuning(*fixed_tuning)
-            log(f"[mixed] tuning fixed at {fixed_tuning}")
+            log(f"[{name}] tuning fixed at {fixed_tuning}")
         else:
-            log("[mixed] autotune: " + ens.autotune())
+            log(f"[{name}] autotune: " + ens.autotune())
         tuning = list(ens.get_tuning())
     for c in range(W):
         ens.cycle_device(yo(c))
@@ -172,7 +180,6 @@ def bench_variant(lb, hip, cfg, variant, M, K, W, P, obs, truth, n_obs, log, rep
     ens.synchronize()
     phases = {ph: ens.profile_read(ph) for ph in ("forecast", "obs", "gram", "solve", "finish", "solve_fp64")}
     ens.close()
-    name = label or ("plain" if variant == lb.VARIANT_PLAIN else "mixed")
     ws = sorted(walls)
     wall = ws[len(ws) // 2]
     q1, q3 = (np.percentile(ws, 25), np.percentile(ws, 75)) if len(ws) > 1 else (ws[0], ws[0])
@@ -183,7 +190,7 @@ def bench_variant(lb, hip, cfg, variant, M, K, W, P, obs, truth, n_obs, log, rep
     return dict(wall=wall, walls=walls, iqr_cycles_per_s=[K / q3, K / q1], event_s=float(np.median(evs)) / 1e3,
                 phases=phases, stats=stats, steps=steps, state=final_state, yo_index=ca, prof_cycles=P,
                 variant="plain" if variant == lb.VARIANT_PLAIN else "mixed", tuning=tuning, K=K,
-                jacobi_tol=cfg.jacobi_tol)
+                jacobi_tol=cfg.jacobi_tol, closure="smagorinsky" if closure is None else "mlp")
 
 
 def rmse_probe(lb, cfg, M, n_cycles, obs, truth, n_obs):
@@ -447,9 +454,10 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=10,
                     help="timed DA cycles of each plain variant (~0.5 s per cycle)")
     ap.add_argument("--members", type=int, default=64)
-    ap.add_argument("--variants", default="plain,plain_tol1e-6,mixed",
+    ap.add_argument("--variants", default="plain,plain_tol1e-6,mixed,mixed_mlp",
                     help="plain (eigensolver tolerance 1e-10), plain_tol1e-6 (the reference's syevj "
-                         "tolerance), mixed")
+                         "tolerance), mixed, mixed_mlp (mixed with the learned closure in the forecast)")
+    ap.add_argument("--mlp-steps", type=int, default=20, help="timed DA cycles of mixed_mlp")
     ap.add_argument("--prof-cycles", type=int, default=3, help="extra phase-timed cycles (eager launches)")
     ap.add_argument("--tuning", default=None, metavar="LBM,CHAIN",
                     help="fixed launch shapes of the mixed variant instead of the autotuner (for profiling "
@@ -513,11 +521,15 @@ def main():
             var, k = lb.VARIANT_PLAIN, min(K, args.plain_steps)
             if v == "plain_tol1e-6":
                 vcfg.jacobi_tol = 1e-6
+        elif v == "mixed_mlp":
+            var, k = lb.VARIANT_MIXED, min(K, args.mlp_steps)
         else:
             var, k = lb.VARIANT_MIXED, K
-        res[v] = bench_variant(lb, hip, vcfg, var, M, k, W if var == lb.VARIANT_MIXED else min(W, 2), P, obs,
-                               truth, n_obs, log, repeats=R if var == lb.VARIANT_MIXED else 1, label=v,
-                               fixed_tuning=[int(x) for x in args.tuning.split(",")] if args.tuning else None)
+        single = var == lb.VARIANT_PLAIN or v == "mixed_mlp"
+        res[v] = bench_variant(lb, hip, vcfg, var, M, k, min(W, 2) if single else W, P, obs,
+                               truth, n_obs, log, repeats=1 if single else R, label=v,
+                               fixed_tuning=[int(x) for x in args.tuning.split(",")] if args.tuning else None,
+                               closure=lb.closure_init_params(1) if v == "mixed_mlp" else None)
     head = res.get("mixed") or res[list(res)[0]]
     kern = kernel_table(head, M, nodes)
     dom_name = max(kern, key=lambda k: kern[k]["share"])
@@ -571,7 +583,7 @@ def main():
         "cycles_per_s_iqr": head["iqr_cycles_per_s"],
         "variants": {k: {"cycles_per_s": r["K"] / r["wall"], "ms_per_cycle": r["wall"] / r["K"] * 1e3,
                          "timed_cycles": r["K"], "timed_runs": len(r["walls"]),
-                         "jacobi_tol": r["jacobi_tol"],
+                         "jacobi_tol": r["jacobi_tol"], "closure": r["closure"],
                          "analysis_rmse": r["stats"][0], "analysis_spread": r["stats"][1],
                          "tuning_lbm_tiles_per_block_and_chain": r["tuning"],
                          "phase_ms_per_cycle": {p: r["phases"][p][0] / P * 1e3 for p in r["phases"]

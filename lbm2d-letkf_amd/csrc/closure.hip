@@ -284,6 +284,17 @@ __device__ __forceinline__ float gelu_mlp(float x) {
     return x * __builtin_amdgcn_rcpf(1.f + e);
 }
 
+// softplus(z) = max(z, 0) + log1p(e), e = exp(-|z|), on the hardware exp2 and log2:
+// log1p(e) = log(u) e / (u - 1) with u = 1 + e rounded (the rounding of u cancels),
+// and e (1 - e/2) below e = 2^-10 (relative error under e^2/3).  A NaN z stays NaN.
+__device__ __forceinline__ float softplus_hw(float z) {
+    const float e = __builtin_amdgcn_exp2f(-fabsf(z) * 1.4426950408889634f);
+    const float u = 1.f + e;
+    const float big = __builtin_amdgcn_logf(u) * 0.6931471805599453f * (e * __builtin_amdgcn_rcpf(u - 1.f));
+    const float lp = e < 0x1p-10f ? e * fmaf(-0.5f, e, 1.f) : big;
+    return fmaxf(z, 0.f) + lp;
+}
+
 // nu_t of 16 nodes (lane % 16) from their bf16 features (this lane's k-block
 // lane / 16), returned on every lane of the node
 __device__ __forceinline__ float mlp_nu(const bf16x8 (&A1)[4], const bf16x8 (&A2)[4][2], const f32x4 (&B1)[4],
@@ -306,8 +317,7 @@ __device__ __forceinline__ float mlp_nu(const bf16x8 (&A1)[4], const bf16x8 (&A2
     }
     zp += __shfl_xor(zp, 16);
     zp += __shfl_xor(zp, 32);
-    const float z = zp + b3;
-    return nu_scale * (fmaxf(z, 0.f) + log1pf(expf(-fabsf(z))));
+    return nu_scale * softplus_hw(zp + b3);
 }
 
 // One LBM step of tiles [tile0, tile0 + gridDim.x) with nu_t from the network:

@@ -289,10 +289,26 @@ def test_trained_closure_in_the_letkf_osse(lb):
 
     coarse = lb.preset("baseline")
     ct = _trainer_module()
-    print("case: closure OSSE 256^2 M=16, 6 cycles against the filtered 1024^2 truth", flush=True)
+    print("case: closure OSSE 256^2 M=64, 10 cycles against the filtered 1024^2 truth", flush=True)
     p, _ = ct.train_closure(coarse, factor=4, seeds=tuple(range(1, 21)), n_snap=3, every=25, spinup=150,
                             batch=4096, steps=1500, lr=2e-3)
-    r = closure_osse.closure_osse(coarse, p, members=16, cycles=6)
+    # the eddy viscosity the trained network gives on a baseline state, next to
+    # Smagorinsky's on the same state (a network with a NaN output falls back to it)
+    cfg = coarse.copy()
+    cfg.closure_kind = lb.CLOSURE_MLP
+    with lb.Context(cfg, 1) as ctx:
+        ctx.init_member(0, 1000)
+        ctx.set_closure(p)
+        ctx.forecast(100)
+        nut_mlp = ctx.closure_nut(0)
+        p_nan = np.zeros(NP, np.float32)
+        p_nan[B3] = float("nan")
+        ctx.set_closure(p_nan)
+        nut_smag = ctx.closure_nut(0)
+    print(f"  nu_t after 100 steps: MLP mean {nut_mlp.mean():.3e} max {nut_mlp.max():.3e}; Smagorinsky "
+          f"(Cs={coarse.cs_smag}) mean {nut_smag.mean():.3e} max {nut_smag.max():.3e}; nu {coarse.nu:.3e}",
+          flush=True)
+    r = closure_osse.closure_osse(coarse, p, members=64, cycles=10)
     for k in ("free", "smagorinsky", "mlp"):
         print(f"  {k:12s} analysis rmse " + " ".join(f"{x:.3e}" for x in r[k]), flush=True)
     assert np.isfinite(r["mlp"]).all() and np.isfinite(r["smagorinsky"]).all()
