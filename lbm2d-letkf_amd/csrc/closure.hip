@@ -296,24 +296,26 @@ __device__ __forceinline__ float softplus_hw(float z) {
 }
 
 // nu_t of 16 nodes (lane % 16) from their bf16 features (this lane's k-block
-// lane / 16), returned on every lane of the node
-__device__ __forceinline__ float mlp_nu(const bf16x8 (&A1)[4], const bf16x8 (&A2)[4][2], const f32x4 (&B1)[4],
-                                        const f32x4 (&B2)[4], const f32x4 (&W3)[4], bf16x8 x, float b3,
-                                        float nu_scale) {
+// lane / 16), returned on every lane of the node; sb: this lane's bias and
+// output-weight vectors in LDS, b1v at sb[64 I], b2v at sb[64 (4 + I)], w3v at
+// sb[64 (8 + I)]
+__device__ __forceinline__ float mlp_nu(const bf16x8 (&A1)[4], const bf16x8 (&A2)[4][2], const f32x4* sb, bf16x8 x,
+                                        float b3, float nu_scale) {
     bf16x8 h[2];
 #pragma unroll
     for (int I = 0; I < 4; ++I) {
-        const f32x4 a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A1[I], x, B1[I], 0, 0, 0);
+        const f32x4 a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A1[I], x, sb[64 * I], 0, 0, 0);
 #pragma unroll
         for (int r = 0; r < 4; ++r) h[I / 2][(I % 2) * 4 + r] = (__bf16)gelu_mlp(a[r]);
     }
     float zp = 0.f;
 #pragma unroll
     for (int I = 0; I < 4; ++I) {
-        f32x4 a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A2[I][0], h[0], B2[I], 0, 0, 0);
+        f32x4 a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A2[I][0], h[0], sb[64 * (4 + I)], 0, 0, 0);
         a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A2[I][1], h[1], a, 0, 0, 0);
+        const f32x4 w3 = sb[64 * (8 + I)];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) zp = fmaf(W3[I][r], gelu_mlp(a[r]), zp);
+        for (int r = 0; r < 4; ++r) zp = fmaf(w3[r], gelu_mlp(a[r]), zp);
     }
     zp += __shfl_xor(zp, 16);
     zp += __shfl_xor(zp, 32);
@@ -327,7 +329,7 @@ __device__ __forceinline__ float mlp_nu(const bf16x8 (&A1)[4], const bf16x8 (&A2
 // that nu_t.  Where the network gives a non-finite value or one outside
 // [0, nu_max] the node falls back to the Smagorinsky nu_t of src/lbm.cu:24-65.
 template <bool FORCE>
-__global__ __launch_bounds__(MNT) void k_lbm_mlp(const float* __restrict__ fin, float* __restrict__ fout,
+__global__ __launch_bounds__(MNT) __attribute__((amdgpu_waves_per_eu(4))) void k_lbm_mlp(const float* __restrict__ fin, float* __restrict__ fout,
                                                   const float* __restrict__ force, LbmParams p,
                                                   const ClosureWeights* __restrict__ cw, int tiles_x,
                                                   int tiles_per_member, int tile0, float* __restrict__ nut_out) {
@@ -348,14 +350,15 @@ __global__ __launch_bounds__(MNT) void k_lbm_mlp(const float* __restrict__ fin, 
     __shared__ float sv[VY][VX];
     __shared__ float2 sab[SY][SX];
     __shared__ float snu[MY][MX];
+    __shared__ f32x4 sbw[12 * 64];
 
     const int lx = threadIdx.x % MX;
     const int ly0 = threadIdx.x / MX;
     const int i = x0 + lx;
     const int im = wrap(i - 1, nx), ip = wrap(i + 1, nx);
 
-    float fc[MNODES][Q];
-    float rr[MNODES], uu[MNODES], vv[MNODES];
+    // the streamed velocity of the tile; the populations are pulled again for the
+    // collision (from L2), so that none is held in registers through the network
 #pragma unroll
     for (int r = 0; r < MNODES; ++r) {
         const int ly = ly0 + r * MROWS;
@@ -363,10 +366,11 @@ __global__ __launch_bounds__(MNT) void k_lbm_mlp(const float* __restrict__ fin, 
         const long long r0 = (long long)j * nx;
         const long long rm = (long long)wrap(j - 1, ny) * nx;
         const long long rp = (long long)wrap(j + 1, ny) * nx;
-        pull9(f, nodes, im, i, ip, rm, r0, rp, fc[r]);
-        moments(fc[r], p.c, rr[r], uu[r], vv[r]);
-        su[ly + 2][lx + 2] = uu[r];
-        sv[ly + 2][lx + 2] = vv[r];
+        float fc[Q], rr, uu, vv;
+        pull9(f, nodes, im, i, ip, rm, r0, rp, fc);
+        moments(fc, p.c, rr, uu, vv);
+        su[ly + 2][lx + 2] = uu;
+        sv[ly + 2][lx + 2] = vv;
     }
     // halo ring: rows -2, -1 and MY, MY+1 over VX columns, then columns -2, -1, MX,
     // MX+1 over the MY tile rows (frame coordinates hx, hy from the tile corner)
@@ -403,19 +407,17 @@ __global__ __launch_bounds__(MNT) void k_lbm_mlp(const float* __restrict__ fin, 
         sab[sy][sx] = make_float2(0.5f * (ux - vy) * s_inv, 0.5f * (uy + vx) * s_inv);
     }
 
-    // network weights of this lane (layout in closure.hpp)
+    // network weights of this lane (layout in closure.hpp): the bf16 matrices in
+    // registers, the bias and output-weight vectors (b1v, b2v, w3v, contiguous) in LDS
     const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
     bf16x8 A1[4], A2[4][2];
-    f32x4 B1[4], B2[4], W3[4];
 #pragma unroll
     for (int I = 0; I < 4; ++I) {
         A1[I] = *reinterpret_cast<const bf16x8*>(cw->a1[I][lane]);
         A2[I][0] = *reinterpret_cast<const bf16x8*>(cw->a2[I][0][lane]);
         A2[I][1] = *reinterpret_cast<const bf16x8*>(cw->a2[I][1][lane]);
-        B1[I] = *reinterpret_cast<const f32x4*>(cw->b1v[I][lane]);
-        B2[I] = *reinterpret_cast<const f32x4*>(cw->b2v[I][lane]);
-        W3[I] = *reinterpret_cast<const f32x4*>(cw->w3v[I][lane]);
     }
+    for (int k = threadIdx.x; k < 12 * 64; k += MNT) sbw[k] = reinterpret_cast<const f32x4*>(cw->b1v)[k];
     const float b3 = cw->b3, nu_scale = cw->nu_scale;
     __syncthreads();
 
@@ -433,17 +435,32 @@ __global__ __launch_bounds__(MNT) void k_lbm_mlp(const float* __restrict__ fin, 
             x[2 * tq] = (__bf16)ab.x;
             x[2 * tq + 1] = (__bf16)ab.y;
         }
-        const float nu = mlp_nu(A1, A2, B1, B2, W3, x, b3, nu_scale);
+        // an opaque lane index keeps the LDS vectors from being hoisted into registers
+        int ln = lane;
+        asm volatile("" : "+v"(ln));
+        const float nu = mlp_nu(A1, A2, sbw + ln, x, b3, nu_scale);
         if (kq == 0) snu[ly][gx] = nu;
     }
     __syncthreads();
 
     const float nu_max = cw->nu_max;
+    // a pointer the compiler cannot see through: the second pull is not merged
+    // with the first (whose values would otherwise stay live through the network)
+    const float* f2 = f;
+    asm volatile("" : "+s"(f2));
 #pragma unroll
     for (int r = 0; r < MNODES; ++r) {
         const int ly = ly0 + r * MROWS;
         const int j = y0 + ly;
         const int cy = ly + 2, cx = lx + 2;
+        float fc[Q], rr, uu, vv;
+        {
+            const long long r0 = (long long)j * nx;
+            const long long rm = (long long)wrap(j - 1, ny) * nx;
+            const long long rp = (long long)wrap(j + 1, ny) * nx;
+            pull9(f2, nodes, im, i, ip, rm, r0, rp, fc);
+            moments(fc, p.c, rr, uu, vv);
+        }
         float nut = snu[ly][lx];
         if (!(nut >= 0.f && nut <= nu_max))  // also catches NaN
             nut = smagorinsky(su[cy][cx + 1], su[cy][cx - 1], su[cy + 1][cx], su[cy - 1][cx], sv[cy][cx + 1],
@@ -455,9 +472,9 @@ __global__ __launch_bounds__(MNT) void k_lbm_mlp(const float* __restrict__ fin, 
             Fx = force[node];
             Fy = force[nodes + node];
         }
-        collide9<FORCE>(fc[r], rr[r], uu[r], vv[r], nut, Fx, Fy, p);
+        collide9<FORCE>(fc, rr, uu, vv, nut, Fx, Fy, p);
 #pragma unroll
-        for (int q = 0; q < Q; ++q) fo[q * nodes + node] = fc[r][q];
+        for (int q = 0; q < Q; ++q) fo[q * nodes + node] = fc[q];
     }
 }
 

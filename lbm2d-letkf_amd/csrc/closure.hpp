@@ -10,6 +10,7 @@
 // x[2s] = a, x[2s+1] = b.
 // Flat FP32 parameter vector: W1[64][18], b1[64], W2[64][64], b2[64], w3[64], b3.
 #pragma once
+#include <cstddef>
 
 namespace lbmletkf {
 
@@ -36,5 +37,9 @@ struct ClosureWeights {
     float b3;
     float inv_strain_scale, nu_scale, nu_max;
 };
+// the collide kernel copies b1v, b2v, w3v to LDS as one block of 16-byte vectors
+static_assert(offsetof(ClosureWeights, b1v) % 16 == 0, "b1v alignment");
+static_assert(offsetof(ClosureWeights, b2v) == offsetof(ClosureWeights, b1v) + sizeof(float) * 4 * 64 * 4, "b2v");
+static_assert(offsetof(ClosureWeights, w3v) == offsetof(ClosureWeights, b2v) + sizeof(float) * 4 * 64 * 4, "w3v");
 
 }  // namespace lbmletkf
