@@ -637,8 +637,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
 //   Xa[q][l] = xbar[q] + sum_i Y[q][i] (s_i V[l][i] + h_i)   (FP32 matrix cores)
 // which is Xa = xbar 1^T + dX (W + w 1^T) with W = sqrt(M-1) V D^-1/2 V^T and
 // w = V D^-1 V^T b (letkf_solver.h:288-316), since dX w = dX V D^-1 V^T b = Y h.
-// The next gridpoint of the chain takes V from a per-workgroup slot in global
-// memory (its warm start); no other intermediate leaves the CU.
+// The next gridpoint of the chain (the next row of the same column) starts from
+// C V with this V, which stays in LDS; C stays in the Gram's accumulators.  No
+// intermediate leaves the CU: the kernel reads the local observation lists, dY and
+// the state, and writes the analysed state.
 // ---------------------------------------------------------------------------------
 constexpr int ASM_ROWS = Q + 1;  // dX rows and b^T: the A operand of the first product
 
