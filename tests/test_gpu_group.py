@@ -3,7 +3,8 @@
 The group kernels run the same per-tile and per-gridpoint code as a single
 context's, so a group step, analysis and graph-replayed cycle must reproduce the
 contexts' own results bit for bit; explicit uneven networks (some gridpoints with
-no observation) take part like lattices."""
+no observation) take part like lattices, and so do a context with the learned
+closure and a forced one (the ref preset's Watanabe-97 forcing and friction)."""
 import numpy as np
 import pytest
 
@@ -15,17 +16,22 @@ pytestmark = pytest.mark.gpu
 SHAPES = [(64, 64, 8), (128, 64, 16), (64, 128, 32), (128, 128, 64)]
 
 
-def _cfg(lb, nx, ny, uneven):
-    cfg = small_cfg(lb, "baseline", n=64, loc_scale=4.0, loc_cutoff=10.0, obs_stride=8)
+def _cfg(lb, nx, ny, uneven, forced=False):
+    if forced:
+        # the ref preset: Watanabe-97 forcing and friction (one-step launches with a
+        # force field per step on the context's side stream), GC-window localization
+        cfg = small_cfg(lb, "ref", n=64, obs_stride=4, variant=lb.VARIANT_MIXED)
+    else:
+        cfg = small_cfg(lb, "baseline", n=64, loc_scale=4.0, loc_cutoff=10.0, obs_stride=8)
     cfg.nx, cfg.ny = nx, ny
     cfg.da_interval = 7
     return cfg
 
 
-def _make(lb, uneven, mlp=False):
+def _make(lb, uneven, mlp=False, forced=False):
     out = []
     for k, (nx, ny, M) in enumerate(SHAPES):
-        cfg = _cfg(lb, nx, ny, uneven)
+        cfg = _cfg(lb, nx, ny, uneven, forced=forced and k == 2)
         if mlp and k == 1:  # one context with the learned closure
             cfg.closure_kind = lb.CLOSURE_MLP
         ens = lb.Context(cfg, M)
@@ -47,11 +53,12 @@ def _obs(pairs, counter):
     return [nat.observe(0, seed=3, counter=counter, noise=True) for _, nat in pairs]
 
 
-@pytest.mark.parametrize("mode", ["lattice", "uneven", "mlp"])
+@pytest.mark.parametrize("mode", ["lattice", "uneven", "mlp", "forced"])
 def test_group_matches_contexts(lb, hipbuf, mode):
-    uneven, mlp = mode == "uneven", mode == "mlp"
+    uneven, mlp, forced = mode == "uneven", mode == "mlp", mode == "forced"
     print(f"case: group of {len(SHAPES)} contexts, {mode}", flush=True)
-    a, b = _make(lb, uneven, mlp), _make(lb, uneven, mlp)  # a: one context at a time; b: the group
+    # a: one context at a time; b: the group
+    a, b = _make(lb, uneven, mlp, forced), _make(lb, uneven, mlp, forced)
     try:
         # forecast
         for ens, _ in a:
