@@ -378,6 +378,13 @@ void* lbmletkf_group_get_stream(lbmletkf_group* g);
 
 /* Stream access for callers that time or order work around the context. */
 void* lbmletkf_get_stream(lbmletkf_ctx* ctx);
+/* Run the context's work on the caller's hipStream_t from now on (NULL: back to a
+ * stream the context owns).  Work queued before the call completes first; the
+ * caller's stream must outlive its use by the context (destroy the context or set
+ * another stream first).  Replaces the reference's implicit per-thread default
+ * stream (config/makefile.compiler.in:43, --default-stream per-thread), on which
+ * update() and assimilate() run. */
+int lbmletkf_set_stream(lbmletkf_ctx* ctx, void* stream);
 int lbmletkf_synchronize(lbmletkf_ctx* ctx);
 
 #ifdef __cplusplus

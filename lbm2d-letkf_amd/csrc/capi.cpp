@@ -566,7 +566,7 @@ Ctx::~Ctx() {
         (void)hipEventDestroy(r.b);
     }
     for (hipEvent_t e : event_pool) (void)hipEventDestroy(e);
-    if (stream) (void)hipStreamDestroy(stream);
+    if (stream && own_stream) (void)hipStreamDestroy(stream);
 }
 }  // namespace lbmletkf
 
@@ -1266,6 +1266,26 @@ int lbmletkf_profile_read(lbmletkf_ctx* ctx, const char* phase, double* seconds,
 void* lbmletkf_get_stream(lbmletkf_ctx* ctx) {
     Ctx* c = reinterpret_cast<Ctx*>(ctx);
     return c ? reinterpret_cast<void*>(c->stream) : nullptr;
+}
+
+int lbmletkf_set_stream(lbmletkf_ctx* ctx, void* stream) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    CHECK_CTX(c);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (s == c->stream) return LBMLETKF_OK;
+    // the work queued so far completes on the old stream; captured cycles are
+    // dropped and re-captured on the new one at the next cycle call
+    c->drop_graphs();
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (s == nullptr) {
+        hipStream_t own = nullptr;
+        HIP_TRY(c, hipStreamCreate(&own));
+        s = own;
+    }
+    if (c->own_stream) (void)hipStreamDestroy(c->stream);
+    c->own_stream = (stream == nullptr);
+    c->stream = s;
+    return LBMLETKF_OK;
 }
 
 int lbmletkf_synchronize(lbmletkf_ctx* ctx) {

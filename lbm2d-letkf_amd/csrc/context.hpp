@@ -74,6 +74,7 @@ struct Ctx {
     LbmParams lp{};
     ForceTable force{};
     hipStream_t stream = nullptr;
+    bool own_stream = true;  // false while the caller's stream (lbmletkf_set_stream) is in use
     int device = 0;
     std::string last_error;
 

@@ -135,6 +135,7 @@ def _load():
         "lbmletkf_group_synchronize": ([P], I),
         "lbmletkf_group_get_stream": ([P], P),
         "lbmletkf_get_stream": ([P], P),
+        "lbmletkf_set_stream": ([P, P], I),
         "lbmletkf_synchronize": ([P], I),
     }
     for name, (args, res) in sig.items():
@@ -398,6 +399,11 @@ class Context:
     @property
     def stream(self) -> int:
         return lib.lbmletkf_get_stream(self.h) or 0
+
+    def set_stream(self, stream: int):
+        """Run on the caller's hipStream_t (an integer handle) from now on; 0 returns
+        to a stream the context owns."""
+        self._ck(lib.lbmletkf_set_stream(self.h, C.c_void_p(stream or None)))
 
     def synchronize(self):
         self._ck(lib.lbmletkf_synchronize(self.h))
