@@ -225,7 +225,7 @@ def _smag_loss(smp, cs, dx, w):
 
 def _best_cs2(smp, dx):
     a, b, t1, t2 = (smp[:, k].astype(np.float64) for k in (18, 19, 20, 21))
-    g = 2 * dx * dx * 2 * np.sqrt(a * a + b * b)  # nu = cs^2 g
+    g = dx * dx * 2 * np.sqrt(a * a + b * b)  # nu = cs^2 g, as in _smag_loss
     # minimise sum (2 cs^2 g a + t1)^2 + (2 cs^2 g b + t2)^2 over cs^2
     return float(-np.sum(2 * g * (a * t1 + b * t2)) / np.sum(4 * g * g * (a * a + b * b)))
 
@@ -313,3 +313,36 @@ def test_trained_closure_in_the_letkf_osse(lb):
         print(f"  {k:12s} analysis rmse " + " ".join(f"{x:.3e}" for x in r[k]), flush=True)
     assert np.isfinite(r["mlp"]).all() and np.isfinite(r["smagorinsky"]).all()
     assert r["mlp"][-1] < r["free"][-1] and r["smagorinsky"][-1] < r["free"][-1]
+
+
+def test_forced_flow_subgrid_stress_is_not_an_eddy_viscosity(lb):
+    """The forced ref flow (Watanabe-97 forcing, friction) as training ground: a
+    Cs = 0 run refined x4 (1024^2, tau - 1/2 = 2.4e-3) stays finite over 12,000
+    fine steps, and the box-filtered subgrid stress of its snapshots is no more
+    aligned with the filtered strain than the decaying flow's: the correlation of
+    -tau_dev with S_dev is ~0 and the best constant Cs is ~0, so Smagorinsky
+    (Cs = 0.2) has a larger a-priori error than no model."""
+    ct = _trainer_module()
+    coarse = lb.preset("ref")
+    coarse.variant = lb.VARIANT_MIXED
+    fine = ct.fine_config(coarse, 4)
+    print("case: forced ref flow, Cs = 0 at 1024^2, 2 members, 12,000 fine steps", flush=True)
+    with lb.Context(fine, 2) as ctx:
+        for k in range(2):
+            ctx.init_member(k, 1 + k)
+        ctx.forecast(12000)
+        _, u, v = ctx.get_macro(0)
+        assert np.isfinite(u).all() and np.isfinite(v).all()
+        with lb.Trainer(coarse, 2 * coarse.nx * coarse.ny) as tr:
+            for k in range(2):
+                tr.add_samples(ctx, k, 4)
+            smp = tr.get_samples()
+    a, b, t1, t2 = (smp[:, k].astype(np.float64) for k in (18, 19, 20, 21))
+    corr = -np.sum(a * t1 + b * t2) / np.sqrt(np.sum(a * a + b * b) * np.sum(t1 * t1 + t2 * t2))
+    w = 1.0 / float(np.mean(t1 * t1 + t2 * t2))
+    cs2 = _best_cs2(smp, coarse.dx)
+    l_smag = _smag_loss(smp, coarse.cs_smag, coarse.dx, w)
+    print(f"  u rms {np.sqrt(np.mean(u * u + v * v)):.3e}; corr(-tau, S) {corr:.4f}; best Cs^2 {cs2:.3e}; "
+          f"normalised stress error: none 1.000, Smagorinsky(Cs={coarse.cs_smag}) {l_smag:.3f}", flush=True)
+    assert abs(corr) < 0.05
+    assert l_smag > 1.0
