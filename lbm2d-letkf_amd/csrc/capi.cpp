@@ -1272,7 +1272,7 @@ int lbmletkf_set_stream(lbmletkf_ctx* ctx, void* stream) {
     Ctx* c = reinterpret_cast<Ctx*>(ctx);
     CHECK_CTX(c);
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    if (s == c->stream) return LBMLETKF_OK;
+    if (s == c->stream || (s == nullptr && c->own_stream)) return LBMLETKF_OK;
     // the work queued so far completes on the old stream; captured cycles are
     // dropped and re-captured on the new one at the next cycle call
     c->drop_graphs();
