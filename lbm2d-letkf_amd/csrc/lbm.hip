@@ -16,6 +16,7 @@
 //   * the plain form keeps the reference's three passes (stream+macro, les,
 //     collide) and is launched member by member, as one reference rank would.
 #include <map>
+#include <type_traits>
 #include <mutex>
 
 #include "context.hpp"
@@ -355,11 +356,13 @@ struct PopRing {
     float* q35;  // [2][2][nx]  q = 3, 5 of rows of parity 0 / 1
     float* q68;  // [3][2][nx]  q = 6, 8 of rows r mod 3
 };
-template <int H>
+// NX > 0: the row length as a compile-time constant (every LDS ring offset then
+// becomes an immediate of its instruction); NX = 0 takes p.nx.
+template <int H, int NX>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k_lbm_rows3(const float* __restrict__ fin, float* __restrict__ fout,
                                                   LbmParams p, int strips_per_member) {
     extern __shared__ float lds3[];  // velocity rings [3 stages][u, v][2][nx]; population rings [2][12][nx]
-    const int nx = p.nx, ny = p.ny;
+    const int nx = NX > 0 ? NX : p.nx, ny = p.ny;
     const int nodes = int(p.nodes);  // offsets within one member fit in 32 bits
     float* vel = lds3;
     PopRing r1{vel + 12 * nx, vel + 14 * nx, vel + 18 * nx};
@@ -368,23 +371,34 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
     const int bb = (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
     const int member = bb / strips_per_member;
     const int y0 = (bb - member * strips_per_member) * H;
-    const float* f = fin + (long long)member * p.member_stride;
-    float* fo = fout + (long long)member * p.member_stride;
+    // the member's populations through buffer resources: the byte offset of
+    // population q in row j, (q nodes + j nx) 4, is uniform and goes to the scalar
+    // offset; the only per-lane offset is the column, 4 i (no vector address
+    // arithmetic per load or store).  One member is 36 nodes bytes (< 2^31 for the
+    // grids this kernel takes).
+    const __amdgpu_buffer_rsrc_t fr = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<float*>(fin + (long long)member * p.member_stride), 0, Q * nodes * 4, 0x00020000);
+    const __amdgpu_buffer_rsrc_t fw =
+        __builtin_amdgcn_make_buffer_rsrc(fout + (long long)member * p.member_stride, 0, Q * nodes * 4, 0x00020000);
     const int i = threadIdx.x;
     const int im = i == 0 ? nx - 1 : i - 1, ip = i == nx - 1 ? 0 : i + 1;
-    const unsigned un = unsigned(nodes), ui = unsigned(i), uim = unsigned(im), uip = unsigned(ip);
+    const int bi = 4 * i, bim = 4 * im, bip = 4 * ip;
+    auto ld = [&](int voff, int soff) {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(fr, voff, soff, 0));
+    };
     auto pull_g = [&](int r, float (&fc)[Q]) {
         const int j = wrap(y0 + r, ny);
-        const unsigned rm = unsigned(wrap(j - 1, ny) * nx), r0 = unsigned(j * nx), rp = unsigned(wrap(j + 1, ny) * nx);
-        fc[0] = f[0 * un + rp + uip];
-        fc[1] = f[1 * un + rp + ui];
-        fc[2] = f[2 * un + rp + uim];
-        fc[3] = f[3 * un + r0 + uip];
-        fc[4] = f[4 * un + r0 + ui];
-        fc[5] = f[5 * un + r0 + uim];
-        fc[6] = f[6 * un + rm + uip];
-        fc[7] = f[7 * un + rm + ui];
-        fc[8] = f[8 * un + rm + uim];
+        const int rm = 4 * wrap(j - 1, ny) * nx, r0 = 4 * j * nx, rp = 4 * wrap(j + 1, ny) * nx;
+        const int qn = 4 * nodes;
+        fc[0] = ld(bip, 0 * qn + rp);
+        fc[1] = ld(bi, 1 * qn + rp);
+        fc[2] = ld(bim, 2 * qn + rp);
+        fc[3] = ld(bip, 3 * qn + r0);
+        fc[4] = ld(bi, 4 * qn + r0);
+        fc[5] = ld(bim, 5 * qn + r0);
+        fc[6] = ld(bip, 6 * qn + rm);
+        fc[7] = ld(bi, 7 * qn + rm);
+        fc[8] = ld(bim, 8 * qn + rm);
     };
     // LDS column offsets of this thread and its west and east neighbours, renewed
     // from an opaque copy at every iteration: formed once outside the loop, the
@@ -435,23 +449,29 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
     // e_x = 0 populations of the collided rows: q4 of the last row, q7 of the last two
     float a4m1 = 0.f, a7m1 = 0.f, a7m2 = 0.f;  // step one
     float b4m1 = 0.f, b7m1 = 0.f, b7m2 = 0.f;  // step two
-    float fP[Q];
+    // rows pulled ahead: two rows in flight, the even one in fP0, the odd one in fP1
+    float fP0[Q], fP1[Q];
     // One iteration; the stage pairs (A, B), (C, D), (E, F) alternate register sets
     // between consecutive iterations, so no row is copied from one set to the other.
     // A moments stage's set holds, on entry, the moments of its row two iterations
     // back: the south velocities of its collide stage.
-    auto iter = [&](int k, float (&fa)[Q], float (&ma)[3], float (&fb)[Q], float (&mb)[3], float (&fc)[Q],
-                    float (&mc)[3], float (&fd)[Q], float (&md)[3], float (&fe)[Q], float (&me)[3],
+    // The parity of k is a compile-time constant of each of the two calls per loop
+    // step (the loop starts at an odd k), so that every ring slot chosen by r & 1 is
+    // a constant offset.
+    auto iter = [&](auto parity, int k, float (&fa)[Q], float (&ma)[3], float (&fb)[Q], float (&mb)[3],
+                    float (&fc)[Q], float (&mc)[3], float (&fd)[Q], float (&md)[3], float (&fe)[Q], float (&me)[3],
                     float (&ff)[Q], float (&mf)[3]) {
+        k = (k & ~1) | decltype(parity)::value;
         ci = i;
         asm volatile("" : "+v"(ci));
         cm = ci == 0 ? nx - 1 : ci - 1;
         cp = ci == nx - 1 ? 0 : ci + 1;
-        // A: row k (the pull was issued one iteration earlier)
+        // A: row k (the pull was issued two iterations earlier)
         const float uSa = ma[1], vSa = ma[2];
+        float (&fP)[Q] = decltype(parity)::value ? fP1 : fP0;
 #pragma unroll
         for (int q = 0; q < Q; ++q) fa[q] = fP[q];
-        if (k < H + 4) pull_g(k + 1, fP);
+        if (k + 2 <= H + 4) pull_g(k + 2, fP);
         moments(fa, p.c, ma[0], ma[1], ma[2]);
         vel_put(0, k, ma[1], ma[2]);
         __builtin_amdgcn_sched_barrier(0);  // stages stay in order (register pressure)
@@ -492,9 +512,10 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
         // F: row k-5 -> populations at t+3, written out
         if (k >= 5) {
             collide_row(2, k - 5, me[1], me[2], uSe, vSe, ff, mf);
-            const unsigned node = unsigned((y0 + k - 5) * nx) + ui;
+            const int row = 4 * (y0 + k - 5) * nx;
 #pragma unroll
-            for (int q = 0; q < Q; ++q) fo[unsigned(q) * un + node] = ff[q];
+            for (int q = 0; q < Q; ++q)
+                __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, ff[q]), fw, bi, 4 * q * nodes + row, 0);
         }
     };
     float fX[Q], fY[Q], fZ[Q], fW[Q], fS[Q], fT[Q];
@@ -502,11 +523,12 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
           mS[3] = {0.f, 0.f, 0.f}, mT[3] = {0.f, 0.f, 0.f};
 #pragma unroll
     for (int q = 0; q < Q; ++q) fY[q] = fW[q] = fT[q] = 0.f;
-    pull_g(-5, fP);
+    pull_g(-5, fP1);
+    pull_g(-4, fP0);
     static_assert((H + 10) % 2 == 0, "iterations come in pairs");
-    for (int k = -5; k <= H + 4; k += 2) {
-        iter(k, fX, mX, fY, mY, fZ, mZ, fW, mW, fS, mS, fT, mT);
-        iter(k + 1, fY, mY, fX, mX, fW, mW, fZ, mZ, fT, mT, fS, mS);
+    for (int k = -5; k <= H + 4; k += 2) {  // k odd
+        iter(std::integral_constant<int, 1>{}, k, fX, mX, fY, mY, fZ, mZ, fW, mW, fS, mS, fT, mT);
+        iter(std::integral_constant<int, 0>{}, k + 1, fY, mY, fX, mX, fW, mW, fZ, mZ, fT, mT, fS, mS);
     }
 }
 
@@ -806,7 +828,9 @@ bool lbm_rows2_supported(int nx, int ny, int h) {
 // LDS of the three-step strips: 144 B per column (12 velocity rows + 2 x 12
 // population rows of 4 B), 72 KB at nx = 512
 bool lbm_rows3_supported(int nx, int ny, int h) {
-    return nx % 64 == 0 && nx <= 512 && (h == 16 || h == 32) && ny % h == 0 && ny >= 2;
+    // byte offsets within one member's populations are 32-bit signed (buffer addressing)
+    return nx % 64 == 0 && nx <= 512 && (h == 16 || h == 32) && ny % h == 0 && ny >= 2 &&
+           (long long)nx * ny * Q * 4 < (1LL << 31);
 }
 
 int lbm_launch_steps(const Ctx& c) {
@@ -839,16 +863,23 @@ static hipError_t launch_rows3(Ctx& c, const float* fin, float* fout, int h) {
     const int strips = c.ny / h;
     const unsigned nb = unsigned(strips * c.M);
     const size_t lds = sizeof(float) * 36 * size_t(c.nx);
-    const void* fn = h == 16 ? reinterpret_cast<const void*>(&k_lbm_rows3<16>)
-                             : reinterpret_cast<const void*>(&k_lbm_rows3<32>);
+    const bool n256 = c.nx == 256;
+    const void* fn = h == 16 ? (n256 ? reinterpret_cast<const void*>(&k_lbm_rows3<16, 256>)
+                                     : reinterpret_cast<const void*>(&k_lbm_rows3<16, 0>))
+                             : (n256 ? reinterpret_cast<const void*>(&k_lbm_rows3<32, 256>)
+                                     : reinterpret_cast<const void*>(&k_lbm_rows3<32, 0>));
     if (lds > 65536) {
         const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
         if (e != hipSuccess) return e;
     }
-    if (h == 16)
-        k_lbm_rows3<16><<<nb, c.nx, lds, c.stream>>>(fin, fout, c.lp, strips);
+    if (h == 16 && n256)
+        k_lbm_rows3<16, 256><<<nb, c.nx, lds, c.stream>>>(fin, fout, c.lp, strips);
+    else if (h == 16)
+        k_lbm_rows3<16, 0><<<nb, c.nx, lds, c.stream>>>(fin, fout, c.lp, strips);
+    else if (n256)
+        k_lbm_rows3<32, 256><<<nb, c.nx, lds, c.stream>>>(fin, fout, c.lp, strips);
     else
-        k_lbm_rows3<32><<<nb, c.nx, lds, c.stream>>>(fin, fout, c.lp, strips);
+        k_lbm_rows3<32, 0><<<nb, c.nx, lds, c.stream>>>(fin, fout, c.lp, strips);
     return hipGetLastError();
 }
 
