@@ -544,7 +544,8 @@ def main():
     mixed = head["variant"] == "mixed"
     traffic_ana = measured_traffic("k_mixed_analysis<64, 3>", solve_wg) if mixed else None
     if tune[0] <= -200:
-        lbm_kernel, lbm_wg = "k_lbm_rows3<%d>" % (-tune[0] - 200), (cfg.ny // (-tune[0] - 200)) * M
+        lbm_kernel = "k_lbm_rows3<%d, %d>" % (-tune[0] - 200, 256 if cfg.nx == 256 else 0)
+        lbm_wg = (cfg.ny // (-tune[0] - 200)) * M
     elif tune[0] <= -100:
         lbm_kernel, lbm_wg = "k_lbm_rows2<%d>" % (-tune[0] - 100), (cfg.ny // (-tune[0] - 100)) * M
     elif tune[0] < 0:

@@ -358,9 +358,17 @@ struct PopRing {
 };
 // NX > 0: the row length as a compile-time constant (every LDS ring offset then
 // becomes an immediate of its instruction); NX = 0 takes p.nx.
+// Odd strips are stepped as the lattice reflected in y: logical row r is physical
+// row y0 + H - 1 - r, and population q is the physical population with e_y
+// negated (0, 1, 2 <-> 6, 7, 8), so those strips walk their rows bottom-up.  The
+// per-node functions are symmetric under that reflection, so the bits are those
+// of the upward walk; the reflection changes only the uniform (scalar) row and
+// population offsets of the loads and stores.  Two neighbouring strips then reach
+// their common boundary rows together (at the start of both, or at the end of
+// both), and the second read of those halo rows finds them in L2.
 template <int H, int NX>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k_lbm_rows3(const float* __restrict__ fin, float* __restrict__ fout,
-                                                  LbmParams p, int strips_per_member) {
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k_lbm_rows3(
+    const float* __restrict__ fin, float* __restrict__ fout, LbmParams p, int strips_per_member) {
     extern __shared__ float lds3[];  // velocity rings [3 stages][u, v][2][nx]; population rings [2][12][nx]
     const int nx = NX > 0 ? NX : p.nx, ny = p.ny;
     const int nodes = int(p.nodes);  // offsets within one member fit in 32 bits
@@ -370,7 +378,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
     const int nb = gridDim.x, b = blockIdx.x;
     const int bb = (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
     const int member = bb / strips_per_member;
-    const int y0 = (bb - member * strips_per_member) * H;
+    const int strip = bb - member * strips_per_member;
+    const int y0 = strip * H;
+    const bool mir = (strip & 1) != 0;  // uniform
     // the member's populations through buffer resources: the byte offset of
     // population q in row j, (q nodes + j nx) 4, is uniform and goes to the scalar
     // offset; the only per-lane offset is the column, 4 i (no vector address
@@ -386,19 +396,23 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
     auto ld = [&](int voff, int soff) {
         return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(fr, voff, soff, 0));
     };
+    // pull of logical row r: fc[q] = f[q](i - ex_q, r - ey_q) of the (reflected) lattice
     auto pull_g = [&](int r, float (&fc)[Q]) {
-        const int j = wrap(y0 + r, ny);
+        const int j = wrap(mir ? y0 + H - 1 - r : y0 + r, ny);
         const int rm = 4 * wrap(j - 1, ny) * nx, r0 = 4 * j * nx, rp = 4 * wrap(j + 1, ny) * nx;
         const int qn = 4 * nodes;
-        fc[0] = ld(bip, 0 * qn + rp);
-        fc[1] = ld(bi, 1 * qn + rp);
-        fc[2] = ld(bim, 2 * qn + rp);
+        // physical offsets of logical population 0 (pulled from logical row r + 1) and
+        // logical population 6 (from row r - 1); 1, 2 and 7, 8 follow them
+        const int os = mir ? 6 * qn + rm : rp, on = mir ? rp : 6 * qn + rm;
+        fc[0] = ld(bip, os);
+        fc[1] = ld(bi, os + qn);
+        fc[2] = ld(bim, os + 2 * qn);
         fc[3] = ld(bip, 3 * qn + r0);
         fc[4] = ld(bi, 4 * qn + r0);
         fc[5] = ld(bim, 5 * qn + r0);
-        fc[6] = ld(bip, 6 * qn + rm);
-        fc[7] = ld(bi, 7 * qn + rm);
-        fc[8] = ld(bim, 8 * qn + rm);
+        fc[6] = ld(bip, on);
+        fc[7] = ld(bi, on + qn);
+        fc[8] = ld(bim, on + 2 * qn);
     };
     // LDS column offsets of this thread and its west and east neighbours, renewed
     // from an opaque copy at every iteration: formed once outside the loop, the
@@ -512,10 +526,14 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
         // F: row k-5 -> populations at t+3, written out
         if (k >= 5) {
             collide_row(2, k - 5, me[1], me[2], uSe, vSe, ff, mf);
-            const int row = 4 * (y0 + k - 5) * nx;
+            const int row = 4 * (mir ? y0 + H - 1 - (k - 5) : y0 + k - 5) * nx;
+            // physical offsets of logical populations 0 and 6 (the reflection swaps them)
+            const int o0 = (mir ? 24 : 0) * nodes + row, o6 = (mir ? 0 : 24) * nodes + row;
 #pragma unroll
-            for (int q = 0; q < Q; ++q)
-                __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, ff[q]), fw, bi, 4 * q * nodes + row, 0);
+            for (int q = 0; q < Q; ++q) {
+                const int off = q < 3 ? o0 + 4 * q * nodes : (q < 6 ? 4 * q * nodes + row : o6 + 4 * (q - 6) * nodes);
+                __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, ff[q]), fw, bi, off, 0);
+            }
         }
     };
     float fX[Q], fY[Q], fZ[Q], fW[Q], fS[Q], fT[Q];
@@ -531,6 +549,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k
         iter(std::integral_constant<int, 0>{}, k + 1, fY, mY, fX, mX, fW, mW, fZ, mZ, fT, mT, fS, mS);
     }
 }
+
+
 
 // ----------------------------------------------------------------------------
 // plain form: the reference's three passes for one member

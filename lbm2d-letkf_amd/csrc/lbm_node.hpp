@@ -34,13 +34,14 @@ __device__ __forceinline__ void pull9(const float* __restrict__ f, long long nod
 }
 
 // rho, u = c sum f ex / rho, v = c sum f ey / rho   (src/lbm.h:72-109), the
-// quotients through the hardware reciprocal of rho (1 ulp; rho ~ 1)
+// quotients through the hardware reciprocal of rho (1 ulp; rho ~ 1).  The sums
+// are grouped symmetrically under the reflection y -> -y (q = 0, 1, 2 <-> 6, 7, 8):
+// on the reflected populations they give the same rho and u and exactly -v, so a
+// kernel may step the reflected lattice and reflect back bit for bit.
 __device__ __forceinline__ void moments(const float fc[Q], float c, float& rho, float& u, float& v) {
     LBMNODE_CONTRACT_PER_STATEMENT
-    float r = fc[0];
-#pragma unroll
-    for (int q = 1; q < Q; ++q) r += fc[q];
-    const float jx = (fc[2] + fc[5] + fc[8]) - (fc[0] + fc[3] + fc[6]);
+    const float r = (((fc[0] + fc[6]) + (fc[1] + fc[7])) + ((fc[2] + fc[8]) + (fc[3] + fc[5]))) + fc[4];
+    const float jx = ((fc[2] + fc[8]) + fc[5]) - ((fc[0] + fc[6]) + fc[3]);
     const float jy = (fc[6] + fc[7] + fc[8]) - (fc[0] + fc[1] + fc[2]);
     const float ir = __builtin_amdgcn_rcpf(r);
     rho = r;
@@ -85,7 +86,8 @@ __device__ __forceinline__ void equilibrium9(float rho, float u, float v, const 
     feq[0] = rw2 * (ap - 3.f * upv);   // (-1, -1)
     feq[2] = rw2 * (am + 3.f * umv);   // (+1, -1)
     feq[6] = rw2 * (am - 3.f * umv);   // (-1, +1)
-    const float moving = ((feq[0] + feq[1]) + (feq[2] + feq[3])) + ((feq[5] + feq[6]) + (feq[7] + feq[8]));
+    // grouped symmetrically under y -> -y, as in moments()
+    const float moving = ((feq[0] + feq[6]) + (feq[1] + feq[7])) + ((feq[2] + feq[8]) + (feq[3] + feq[5]));
     feq[4] = rho - moving;
 }
 

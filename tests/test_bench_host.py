@@ -43,7 +43,7 @@ def test_traffic_summary_holds_the_benchmark_shapes():
     nodes, M = 256 * 256, 64
     algorithmic = b.LBM_BYTES_PER_NODE_UPDATE * nodes * M  # the state read and written once
     for kernel, wg in (("k_mixed_analysis<64, 3>", nodes // 16), ("k_mixed_analysis<64, 3>", nodes // 32),
-                       ("k_lbm_rows3<16>", (256 // 16) * M), ("k_lbm_rows2<16>", (256 // 16) * M)):
+                       ("k_lbm_rows3<16, 256>", (256 // 16) * M), ("k_lbm_rows2<16>", (256 // 16) * M)):
         t = b.measured_traffic(kernel, wg)
         assert t is not None, (kernel, wg)
         assert algorithmic <= t <= 2 * algorithmic, (kernel, wg, t)
