@@ -205,6 +205,16 @@ int lbmletkf_cycle(lbmletkf_ctx* ctx, const float* yo_device);
 int lbmletkf_ensemble_stats(lbmletkf_ctx* ctx, const float* truth_rho, const float* truth_u,
                             const float* truth_v, int truth_on_device, double out[4]);
 
+/* Per-node fields over the members ([ny][nx] each; a NULL output is skipped): the
+ * ensemble mean of u and v, the mean and standard deviation of the vorticity
+ * (central differences, src/io.cu:61-86), and the standard deviations of u and v;
+ * deviations biased (1/M).  The ENSEMBLE_STAT outputs of src/io.cu:90-145
+ * (ens_mean_u, ens_mean_v, ens_vor_mean, ens_vor_std) plus the spread fields,
+ * without gathering the members on the host (one reduction over the members on
+ * the device; the buffers are allocated at the first call). */
+int lbmletkf_ensemble_fields(lbmletkf_ctx* ctx, float* mean_u, float* mean_v, float* vor_mean, float* vor_std,
+                             float* sd_u, float* sd_v);
+
 /* lbmletkf_ensemble_stats with the truth on the device and no allocation: no
  * counterpart in the reference; for callers that cannot allocate device memory
  * (code under HIP graph capture, tight serving loops).  The per-block partial sums

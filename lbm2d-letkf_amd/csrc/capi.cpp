@@ -558,7 +558,7 @@ Ctx::~Ctx() {
     void* ptrs[] = {f[0], f[1], plain_ruv, plain_nus, force_xy, d_force, d_step, d_status,
                     d_site_node, d_loc_ptr, d_loc_site, d_loc_w, d_obs_coef, d_yo, d_hx, d_hx1, d_dy,
                     d_dyo, d_gram, d_bvec, d_mix_u, d_mix_b, d_dump, d_red, d_truth, d_sweeps,
-                    d_sweeps64, d_closure, d_site_map};
+                    d_sweeps64, d_closure, d_site_map, d_ens_uv, d_ens_out};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (const ProfRec& r : prof_pending) {
@@ -1054,6 +1054,21 @@ int lbmletkf_ensemble_stats(lbmletkf_ctx* ctx, const float* truth_rho, const flo
     out[1] = double(std::sqrt(s[1] / (2.0L * n)));
     out[2] = double(std::sqrt(s[2] / (long double)n));
     out[3] = double(std::sqrt(s[3] / (long double)n));
+    return LBMLETKF_OK;
+}
+
+int lbmletkf_ensemble_fields(lbmletkf_ctx* ctx, float* mean_u, float* mean_v, float* vor_mean, float* vor_std,
+                             float* sd_u, float* sd_v) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    CHECK_CTX(c);
+    const size_t n = size_t(c->nodes);
+    if (!c->d_ens_uv) HIP_TRY(c, dalloc(&c->d_ens_uv, size_t(2) * c->M * n));
+    if (!c->d_ens_out) HIP_TRY(c, dalloc(&c->d_ens_out, 6 * n));
+    HIP_TRY(c, launch_ensemble_fields(*c, c->d_ens_uv, c->d_ens_out));
+    if (int st = sync_and_check(c)) return st;
+    float* outs[6] = {mean_u, mean_v, vor_mean, vor_std, sd_u, sd_v};
+    for (int k = 0; k < 6; ++k)
+        if (outs[k]) HIP_TRY(c, hipMemcpy(outs[k], c->d_ens_out + k * n, sizeof(float) * n, hipMemcpyDeviceToHost));
     return LBMLETKF_OK;
 }
 

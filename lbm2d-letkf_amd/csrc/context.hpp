@@ -132,6 +132,9 @@ struct Ctx {
     //   [2M^2+2M, 3M^2+2M) V (PLAIN), row-major
     int dump_g0 = 0, dump_n = 0;
     double* d_dump = nullptr;     // [dump_n][dump_record(M)]
+    // lbmletkf_ensemble_fields: members' (u, v) and the six output fields (allocated on first use)
+    float* d_ens_uv = nullptr;
+    float* d_ens_out = nullptr;
 
     // scratch for reductions
     double* d_red = nullptr;      // [red_cap]
@@ -191,6 +194,9 @@ hipError_t launch_nudge(Ctx& c, const float* ro, const float* uo, const float* v
 hipError_t launch_perturb(Ctx& c, int member, const float* ur, const float* vr, float eps);
 hipError_t launch_set_feq(Ctx& c, int member, const float* d_rho, const float* d_u, const float* d_v);
 hipError_t launch_macro(Ctx& c, const float* f_member, float* rho, float* u, float* v);
+// uv: [M][2][nodes] scratch; out: [6][nodes] (mean u, mean v, mean and sd of the
+// vorticity, sd of u and of v)
+hipError_t launch_ensemble_fields(Ctx& c, float* uv, float* out);
 hipError_t launch_ensemble_stats(Ctx& c, const float* truth_rho, const float* truth_u,
                                  const float* truth_v, double* partials, int n_blocks);
 int ensemble_stats_blocks(const Ctx& c);

@@ -754,6 +754,64 @@ __global__ void k_macro(const float* __restrict__ f, float* __restrict__ rho,
     if (v) v[node] = vv;
 }
 
+// (u, v) of every member at every node: uv[member][0][node] = u, [1] = v
+__global__ void k_member_uv(const float* __restrict__ f, float* __restrict__ uv, LbmParams p) {
+    const long long node = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= p.nodes) return;
+    const int m = blockIdx.y;
+    const float* fm = f + (long long)m * p.member_stride;
+    float fc[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) fc[q] = fm[q * p.nodes + node];
+    float r, uu, vv;
+    moments(fc, p.c, r, uu, vv);
+    uv[(2LL * m) * p.nodes + node] = uu;
+    uv[(2LL * m + 1) * p.nodes + node] = vv;
+}
+
+// Per node over the members (src/io.cu:90-145 with ENSEMBLE_STAT, and the
+// vorticity of src/io.cu:61-86): out[0] = mean u, [1] = mean v, [2] = mean
+// vorticity, [3] = its standard deviation, [4], [5] = standard deviations of u
+// and v; deviations biased (1/M), from FP64 sums of values and squares.
+__global__ void k_ensemble_fields(const float* __restrict__ uv, float* __restrict__ out, int M, float dx,
+                                  LbmParams p) {
+    const long long node = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= p.nodes) return;
+    const int nx = p.nx, ny = p.ny;
+    const int i = int(node % nx), j = int(node / nx);
+    const long long e = (long long)j * nx + (i + 1 == nx ? 0 : i + 1);
+    const long long w = (long long)j * nx + (i == 0 ? nx - 1 : i - 1);
+    const long long n = (long long)(j + 1 == ny ? 0 : j + 1) * nx + i;
+    const long long s = (long long)(j == 0 ? ny - 1 : j - 1) * nx + i;
+    double su = 0, sv = 0, su2 = 0, sv2 = 0, sw = 0, sw2 = 0;
+    for (int m = 0; m < M; ++m) {
+        const float* u = uv + (2LL * m) * p.nodes;
+        const float* v = u + p.nodes;
+        const float uy = (u[n] - u[s]) / (2.f * dx);
+        const float vx = (v[e] - v[w]) / (2.f * dx);
+        const float vor = vx - uy;
+        const double a = u[node], b = v[node];
+        su += a;
+        sv += b;
+        su2 += a * a;
+        sv2 += b * b;
+        sw += vor;
+        sw2 += double(vor) * vor;
+    }
+    const double im = 1.0 / M;
+    auto sd = [](double ss, double mean) {
+        const double var = ss - mean * mean;
+        return var > 0 ? sqrt(var) : 0.0;
+    };
+    const double mu = su * im, mv = sv * im, mw = sw * im;
+    out[node] = float(mu);
+    out[p.nodes + node] = float(mv);
+    out[2 * p.nodes + node] = float(mw);
+    out[3 * p.nodes + node] = float(sd(sw2 * im, mw));
+    out[4 * p.nodes + node] = float(sd(su2 * im, mu));
+    out[5 * p.nodes + node] = float(sd(sv2 * im, mv));
+}
+
 // Ensemble mean / biased spread of (u,v) and rho against a truth, summed per
 // block in FP64 (ensemble_stat/main.cpp:108-177).  partials[block][4] =
 // { sum (ubar-ut)^2+(vbar-vt)^2, sum su^2+sv^2, sum (rbar-rt)^2, sum sr^2 }.
@@ -994,6 +1052,14 @@ hipError_t stats_scratch_address(double** out) {
     }
     *out = it->second;
     return hipSuccess;
+}
+
+hipError_t launch_ensemble_fields(Ctx& c, float* uv, float* out) {
+    const int nt = 256;
+    const dim3 grid_uv(node_blocks(c.nodes, nt), unsigned(c.M));
+    k_member_uv<<<grid_uv, nt, 0, c.stream>>>(c.f[c.cur], uv, c.lp);
+    k_ensemble_fields<<<node_blocks(c.nodes, nt), nt, 0, c.stream>>>(uv, out, c.M, float(c.cfg.dx), c.lp);
+    return hipGetLastError();
 }
 
 hipError_t launch_ensemble_stats(Ctx& c, const float* truth_rho, const float* truth_u,

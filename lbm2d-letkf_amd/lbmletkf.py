@@ -134,6 +134,7 @@ def _load():
         "lbmletkf_group_cycle": ([P, P], I),
         "lbmletkf_group_synchronize": ([P], I),
         "lbmletkf_group_get_stream": ([P], P),
+        "lbmletkf_ensemble_fields": ([P, P, P, P, P, P, P], I),
         "lbmletkf_get_stream": ([P], P),
         "lbmletkf_set_stream": ([P, P], I),
         "lbmletkf_synchronize": ([P], I),
@@ -308,6 +309,13 @@ class Context:
         ev = np.empty((count, M))
         self._ck(lib.lbmletkf_get_dump_evd(self.h, _ptr(Cm), _ptr(V), _ptr(ev)))
         return Cm, V, ev
+
+    def ensemble_fields(self):
+        """Per-node (mean_u, mean_v, vor_mean, vor_std, sd_u, sd_v) over the members,
+        each [ny, nx] (src/io.cu:90-145 ENSEMBLE_STAT and the spread fields)."""
+        out = [np.empty((self.ny, self.nx), np.float32) for _ in range(6)]
+        self._ck(lib.lbmletkf_ensemble_fields(self.h, *[_ptr(a) for a in out]))
+        return tuple(out)
 
     def get_obs_ensemble(self):
         """(dy [n_obs][M], dyo [n_obs]) of the last analysis."""

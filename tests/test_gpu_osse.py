@@ -185,3 +185,37 @@ def test_plain_cycle_matches_analysis_then_forecast(lb, hipbuf):
         out.append(ens.get_all_states())
         ens.close()
     np.testing.assert_array_equal(out[0], out[1])
+
+
+def test_ensemble_fields_match_numpy(lb):
+    """lbmletkf_ensemble_fields against the members' own macroscopic fields: the
+    mean of u and v, the mean and standard deviation of the vorticity (central
+    differences, src/io.cu:61-86) and the spread of u and v, per node."""
+    cfg = small_cfg(lb, "baseline", n=64)
+    cfg.nx, cfg.ny = 128, 64
+    M = 12
+    print("case: ensemble fields 128x64 M=12", flush=True)
+    with lb.Context(cfg, M) as ens:
+        for k in range(M):
+            ens.init_member(k, 500 + k)
+        ens.forecast(7)
+        mu, mv, wm, ws, su, sv = ens.ensemble_fields()
+        uvs = [ens.get_macro(k)[1:] for k in range(M)]
+    u = np.stack([a for a, _ in uvs]).astype(np.float64)
+    v = np.stack([b for _, b in uvs]).astype(np.float64)
+    dx = np.float32(cfg.dx)
+    vor = []
+    for a, b in uvs:  # float32 as the device forms it
+        uy = (np.roll(a, -1, axis=0) - np.roll(a, 1, axis=0)) / (np.float32(2) * dx)
+        vx = (np.roll(b, -1, axis=1) - np.roll(b, 1, axis=1)) / (np.float32(2) * dx)
+        vor.append((vx - uy).astype(np.float64))
+    vor = np.stack(vor)
+    scale_u = np.abs(u).max()
+    np.testing.assert_allclose(mu, u.mean(0), rtol=0, atol=1e-7 * scale_u)
+    np.testing.assert_allclose(mv, v.mean(0), rtol=0, atol=1e-7 * scale_u)
+    np.testing.assert_allclose(su, u.std(0), rtol=0, atol=1e-6 * scale_u)
+    np.testing.assert_allclose(sv, v.std(0), rtol=0, atol=1e-6 * scale_u)
+    scale_w = np.abs(vor).max()
+    np.testing.assert_allclose(wm, vor.mean(0), rtol=0, atol=1e-6 * scale_w)
+    np.testing.assert_allclose(ws, vor.std(0), rtol=0, atol=1e-5 * scale_w)
+    print(f"  spread of u {su.mean():.3e}, vorticity sd {ws.mean():.3e}", flush=True)
