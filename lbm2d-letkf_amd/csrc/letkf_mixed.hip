@@ -47,6 +47,8 @@ struct MixedParams {
     float diag;             // (M-1)/beta
     float coef[3];          // R^-1 per observed variable
     double sqm1;            // sqrt(M-1)
+    float check_after;      // FP32 phase: after a sweep whose largest cosine is below this, the
+    float check_tol;        //   columns' Gram (matrix cores) may stop it below check_tol
     float tol32;            // FP32 phase: stop after a sweep whose largest pair cosine is below
     double tol64;           // FP64 phase: the same test, then at most max64 sweeps
     int max32, max64;
@@ -532,8 +534,11 @@ __device__ inline void warm_start(Rows rows, float* cfull, int lane) {
 // return the columns are sc * u[] and the matrix U = uscale * sc * u (uscale a power
 // of two that keeps the squared column norms in range).
 template <int N>
+__device__ float gram_max_cos(const float (&u)[N], float* cv, int lane);
+
+template <int N>
 __device__ __forceinline__ float jacobi_fp32(float (&u)[N], float& sc, const float* cfull, int lane,
-                                             const MixedParams& mp) {
+                                             const MixedParams& mp, float* check_lds = nullptr) {
     const int l = lane & (N - 1);
     float a0 = 0.f;
 #pragma unroll
@@ -567,6 +572,9 @@ __device__ __forceinline__ float jacobi_fp32(float (&u)[N], float& sc, const flo
         mc = wave_max(mc);
         ++sw32;
         if (mc <= mp.tol32 || (mc <= 1e-4f && mc > 0.5f * mc_prev)) break;
+        // Near convergence the next sweep mostly confirms it: measure the columns'
+        // largest cosine directly and stop when it is already below check_tol.
+        if (check_lds && mc <= mp.check_after && gram_max_cos<N>(u, check_lds, lane) <= mp.check_tol) break;
         mc_prev = mc;
     }
     if (lane == 0 && mp.sweeps32) atomicAdd(mp.sweeps32, (unsigned long long)sw32);
@@ -661,6 +669,73 @@ struct FusedShared {
 template <int N>
 __device__ inline int vswz(int k) {
     return (4 * ((k >> 2) & 3) + 16 * ((k & 3) ^ ((k >> 2) & 3))) & (N - 1);
+}
+
+// Largest pair cosine |u_p . u_q| / (|u_p| |u_q|) of the columns (lane l holds
+// column l in u[0..N), up to a per-column scale, which cancels): the columns go to
+// cv (row k at cv[k N + (col ^ vswz(k))]) and the tiles (I, J), I <= J, of U^T U are
+// formed on the FP32 matrix cores (v_mfma_f32_16x16x4f32, exact products, FP32
+// sums: the cosine is resolved to ~1e-6).  Uses all of cv; wave-uniform result.
+template <int N>
+__device__ float gram_max_cos(const float (&u)[N], float* cv, int lane) {
+    constexpr int NT = N / 16;
+    constexpr int NTU = NT * (NT + 1) / 2;
+    int ln = lane;
+    asm volatile("" : "+v"(ln));  // addresses formed here, not hoisted into the sweep loop
+    if (ln < N) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) cv[k * N + (ln ^ vswz<N>(k))] = u[k];
+    }
+    __syncthreads();
+    const int c = ln & 15, g = ln >> 4;
+    f32x4 acc[NTU];
+#pragma unroll
+    for (int t = 0; t < NTU; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < N / 4; ++s) {
+        const int k = 4 * s + g;
+        float a[NT];
+#pragma unroll
+        for (int I = 0; I < NT; ++I) a[I] = cv[k * N + ((16 * I + c) ^ vswz<N>(k))];
+        int t = 0;
+#pragma unroll
+        for (int I = 0; I < NT; ++I)
+#pragma unroll
+            for (int J = I; J < NT; ++J, ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[I], a[J], acc[t], 0, 0, 0);
+    }
+    // tile (I, J): lane (c, g) holds G[16 I + 4 g + r][16 J + c]; the diagonal from the
+    // tiles (I, I) where 4 g + r == c
+    __syncthreads();  // every read of the columns is done
+    {
+        int t = 0;
+#pragma unroll
+        for (int I = 0; I < NT; ++I) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                if (4 * g + r == c) cv[16 * I + c] = acc[t][r];
+            t += NT - I;
+        }
+    }
+    __syncthreads();
+    float mx = 0.f;
+    int t = 0;
+#pragma unroll
+    for (int I = 0; I < NT; ++I)
+#pragma unroll
+        for (int J = I; J < NT; ++J, ++t) {
+            const float dq = cv[16 * J + c];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int p = 16 * I + 4 * g + r, q = 16 * J + c;
+                if (p < q) {
+                    const float d = cv[p] * dq;
+                    const float cs = d > 0.f ? fabsf(acc[t][r]) * __builtin_amdgcn_rsqf(d) : 0.f;
+                    mx = fmaxf(mx, cs);
+                }
+            }
+        }
+    __syncthreads();  // cv is free again
+    return wave_max(mx);
 }
 
 // C with its diagonal from the FULL Gram accumulators: (M-1)/beta on the members'
@@ -793,7 +868,7 @@ __device__ __forceinline__ void analysis_body(int bx, int nbx, const int* __rest
         __syncthreads();
         float u[N];
         float sc;
-        const float uscale = jacobi_fp32<N>(u, sc, sh.cv, lane, mp);
+        const float uscale = jacobi_fp32<N>(u, sc, sh.cv, lane, mp, sh.cv);
 
         // ---- eigenpairs: lambda_i = ||u_i|| in FP64, v_i = u_i / ||u_i|| ------------------
         double n4[4] = {0.0, 0.0, 0.0, 0.0};
@@ -1278,6 +1353,14 @@ MixedParams mixed_params(const Ctx& c, unsigned long long* sweeps32, unsigned lo
     // basis, and any non-orthogonality left there becomes an error of the next
     // gridpoint's eigenpairs that no later sweep removes.
     mp.tol32 = c.cfg.jacobi_tol32 > 0 ? float(c.cfg.jacobi_tol32) : 3e-6f;
+#ifndef LBMLETKF_CHECK_AFTER
+#define LBMLETKF_CHECK_AFTER 3e-3f
+#endif
+#ifndef LBMLETKF_CHECK_DIV
+#define LBMLETKF_CHECK_DIV 3.f
+#endif
+    mp.check_after = LBMLETKF_CHECK_AFTER;
+    mp.check_tol = mp.tol32 / LBMLETKF_CHECK_DIV;
     mp.tol64 = c.cfg.jacobi_tol;
     mp.max32 = c.cfg.jacobi_max_sweeps;
     mp.max64 = c.cfg.jacobi_fp64_sweeps;
