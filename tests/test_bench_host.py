@@ -37,13 +37,13 @@ def test_steps_per_launch_of_the_shape_codes():
 def test_traffic_summary_holds_the_benchmark_shapes():
     """The rows the roofline looks up at 256^2 x 64 for the shapes the autotuner
     picks -- the fused analysis at chains of 16 and 32 (4,096 and 2,048 workgroups),
-    three- and two-step strips of 16 rows (1,024 workgroups) -- are in the committed
+    three-step strips of 16 rows (1,024 workgroups) -- are in the committed
     summary, with byte counts no smaller than what the kernels must move."""
     b = _bench()
     nodes, M = 256 * 256, 64
     algorithmic = b.LBM_BYTES_PER_NODE_UPDATE * nodes * M  # the state read and written once
     for kernel, wg in (("k_mixed_analysis<64, 3>", nodes // 16), ("k_mixed_analysis<64, 3>", nodes // 32),
-                       ("k_lbm_rows3<16, 256>", (256 // 16) * M), ("k_lbm_rows2<16>", (256 // 16) * M)):
+                       ("k_lbm_rows3<16, 256>", (256 // 16) * M)):
         t = b.measured_traffic(kernel, wg)
         assert t is not None, (kernel, wg)
         assert algorithmic <= t <= 2 * algorithmic, (kernel, wg, t)
