@@ -108,6 +108,13 @@ __device__ inline int xcd_order(int b, int nb) { return (nb % 8 == 0) ? (b % 8) 
 [[maybe_unused]] __device__ inline double rot_sqrt(double x) { return sqrt(x); }
 [[maybe_unused]] __device__ inline float rot_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 [[maybe_unused]] __device__ inline double rot_rsq(double x) { return 1.0 / sqrt(x); }
+// a + b rounded once, never fused with the product that formed a: the two lanes of
+// a pair add the same two values in opposite order and must agree bit for bit
+__device__ inline float add_uncontracted(float a, float b) {
+#pragma clang fp contract(off)
+    return a + b;
+}
+
 
 // Cross-lane reads of the sweeps.  A lane reads the value of lane ^ m: through the
 // LDS crossbar (ds_bpermute, any m) or, for the masks the data-parallel primitives
@@ -127,6 +134,133 @@ struct XorDpp {
 };
 constexpr int DPP_XOR1 = 0xB1, DPP_XOR2 = 0x4E, DPP_XOR3 = 0x1B, DPP_XOR8 = 0x128;
 
+// In-place FMAs whose first factor comes from another lane of the same quad, as
+// the DPP source operand of v_fmac_f32: lane l reads lane l ^ D (D = 1, 2, 3), one
+// instruction per element and no register for the moved value (the compiler folds
+// DPP moves into untied VALU operations only, not into v_fmac).  Each block opens
+// with the two wait states the hardware requires between a VALU write of a
+// register and a DPP read of it; inside a block no instruction writes a register
+// that a later one reads through DPP.  All lanes of the wave must be active.  An
+// instruction reads all its operands before it writes, so d[i] += d[i](lane ^ D) b
+// takes the partner's value from before the update.
+#define LML_QP1 "quad_perm:[1,0,3,2]"
+#define LML_QP2 "quad_perm:[2,3,0,1]"
+#define LML_QP3 "quad_perm:[3,2,1,0]"
+#define LML_F(QP, d, s0, s1) \
+    "v_fmac_f32_dpp %[" #d "], %[" #s0 "], %[" #s1 "] " QP " row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+#define LML_V8(n, v, r, K)                                                                                  \
+    [n##0] r(v[K]), [n##1] r(v[K + 1]), [n##2] r(v[K + 2]), [n##3] r(v[K + 3]), [n##4] r(v[K + 4]),       \
+        [n##5] r(v[K + 5]), [n##6] r(v[K + 6]), [n##7] r(v[K + 7])
+#define LML_V16(n, v, r, K)                                                                                 \
+    LML_V8(n, v, r, K), [n##8] r(v[K + 8]), [n##9] r(v[K + 9]), [n##10] r(v[K + 10]), [n##11] r(v[K + 11]), \
+        [n##12] r(v[K + 12]), [n##13] r(v[K + 13]), [n##14] r(v[K + 14]), [n##15] r(v[K + 15])
+#define LML_S4 [s0] "+v"(s[0]), [s1] "+v"(s[1]), [s2] "+v"(s[2]), [s3] "+v"(s[3])
+#define LML_S4W [s0] "=v"(s[0]), [s1] "=v"(s[1]), [s2] "=v"(s[2]), [s3] "=v"(s[3])
+// the lines of a block: d[i] += a[i](lane ^ D) * b, d[i] += d[i](lane ^ D) * b,
+// s[i & 3] += a[i](lane ^ D) * c[i], s[i & 3] += a[i](lane ^ D) * a[i] (the sums s are
+// set by the first block, not accumulated onto)
+#define LML_AX(QP, i) LML_F(QP, d##i, a##i, b)
+#define LML_SAX(QP, i) LML_F(QP, d##i, d##i, b)
+#define LML_DOT(QP, i, j) LML_F(QP, s##j, a##i, c##i)
+#define LML_SDOT(QP, i, j) LML_F(QP, s##j, a##i, a##i)
+#define LML_M(QP, d, s0, s1) \
+    "v_mul_f32_dpp %[" #d "], %[" #s0 "], %[" #s1 "] " QP " row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+#define LML_DOT0(QP, i, j) LML_M(QP, s##j, a##i, c##i)
+#define LML_SDOT0(QP, i, j) LML_M(QP, s##j, a##i, a##i)
+#define LML_8(M, QP)                                                                                        \
+    M(QP, 0) M(QP, 1) M(QP, 2) M(QP, 3) M(QP, 4) M(QP, 5) M(QP, 6) M(QP, 7)
+#define LML_16(M, QP) LML_8(M, QP) M(QP, 8) M(QP, 9) M(QP, 10) M(QP, 11) M(QP, 12) M(QP, 13) M(QP, 14) M(QP, 15)
+#define LML_D8(M, M0, QP)                                                                                   \
+    M0(QP, 0, 0) M0(QP, 1, 1) M0(QP, 2, 2) M0(QP, 3, 3) M(QP, 4, 0) M(QP, 5, 1) M(QP, 6, 2) M(QP, 7, 3)
+#define LML_D16(M, M0, QP)                                                                                  \
+    LML_D8(M, M0, QP) M(QP, 8, 0) M(QP, 9, 1) M(QP, 10, 2) M(QP, 11, 3) M(QP, 12, 0) M(QP, 13, 1)           \
+        M(QP, 14, 2) M(QP, 15, 3)
+// blocks of W = 16 elements (8 for a last part of 8), each opening with the wait states
+#define LML_AXPY(W, QP) \
+    asm("s_nop 1\n\t" LML_##W(LML_AX, QP) : LML_V##W(d, d, "+v", K) : LML_V##W(a, a, "v", K), [b] "v"(b))
+#define LML_SAXPY(W, QP) asm("s_nop 1\n\t" LML_##W(LML_SAX, QP) : LML_V##W(d, d, "+v", K) : [b] "v"(b))
+// (the first block of a sum starts its four partial sums with products)
+#define LML_DOTB(W, QP)                                                                                     \
+    if constexpr (K == 0)                                                                                   \
+        asm("s_nop 1\n\t" LML_D##W(LML_DOT, LML_DOT0, QP)                                                    \
+            : LML_S4W                                                                                       \
+            : LML_V##W(a, a, "v", K), LML_V##W(c, c, "v", K));                                              \
+    else                                                                                                    \
+        asm("s_nop 1\n\t" LML_D##W(LML_DOT, LML_DOT, QP) : LML_S4 : LML_V##W(a, a, "v", K), LML_V##W(c, c, "v", K))
+#define LML_SDOTB(W, QP)                                                                                    \
+    if constexpr (K == 0)                                                                                   \
+        asm("s_nop 1\n\t" LML_D##W(LML_SDOT, LML_SDOT0, QP) : LML_S4W : LML_V##W(a, a, "v", K));            \
+    else                                                                                                    \
+        asm("s_nop 1\n\t" LML_D##W(LML_SDOT, LML_SDOT, QP) : LML_S4 : LML_V##W(a, a, "v", K))
+#define LML_BY_D(OP, W) \
+    if constexpr (D == 1) OP(W, LML_QP1); \
+    else if constexpr (D == 2) OP(W, LML_QP2); \
+    else OP(W, LML_QP3)
+#define LML_BY_W(OP)                            \
+    if constexpr (K + 16 <= H) {                \
+        LML_BY_D(OP, 16);                       \
+    } else {                                    \
+        LML_BY_D(OP, 8);                        \
+    }
+#define LML_NEXT (K + 16 <= H ? K + 16 : K + 8)
+
+template <int D, int H, int K = 0>
+__device__ __forceinline__ void dpp_axpy(float (&d)[H], const float (&a)[H], float b) {
+    static_assert(D >= 1 && D <= 3 && H % 8 == 0, "quad lane offset, blocks of 8 or 16");
+    LML_BY_W(LML_AXPY)
+    if constexpr (LML_NEXT < H) dpp_axpy<D, H, LML_NEXT>(d, a, b);
+}
+template <int D, int H, int K = 0>
+__device__ __forceinline__ void dpp_self_axpy(float (&d)[H], float b) {
+    static_assert(D >= 1 && D <= 3 && H % 8 == 0, "quad lane offset, blocks of 8 or 16");
+    LML_BY_W(LML_SAXPY)
+    if constexpr (LML_NEXT < H) dpp_self_axpy<D, H, LML_NEXT>(d, b);
+}
+template <int D, int H, int K = 0>
+__device__ __forceinline__ void dpp_dot(float (&s)[4], const float (&a)[H], const float (&c)[H]) {
+    static_assert(D >= 1 && D <= 3 && H % 8 == 0, "quad lane offset, blocks of 8 or 16");
+    LML_BY_W(LML_DOTB)
+    if constexpr (LML_NEXT < H) dpp_dot<D, H, LML_NEXT>(s, a, c);
+}
+template <int D, int H, int K = 0>
+__device__ __forceinline__ void dpp_self_dot(float (&s)[4], const float (&a)[H]) {
+    static_assert(D >= 1 && D <= 3 && H % 8 == 0, "quad lane offset, blocks of 8 or 16");
+    LML_BY_W(LML_SDOTB)
+    if constexpr (LML_NEXT < H) dpp_self_dot<D, H, LML_NEXT>(s, a);
+}
+#undef LML_NEXT
+#undef LML_BY_W
+#undef LML_BY_D
+#undef LML_SDOTB
+#undef LML_DOTB
+#undef LML_SAXPY
+#undef LML_AXPY
+#undef LML_D16
+#undef LML_D8
+#undef LML_16
+#undef LML_8
+#undef LML_SDOT0
+#undef LML_DOT0
+#undef LML_M
+#undef LML_SDOT
+#undef LML_DOT
+#undef LML_SAX
+#undef LML_AX
+#undef LML_S4W
+#undef LML_S4
+#undef LML_V16
+#undef LML_V8
+#undef LML_F
+#undef LML_QP3
+#undef LML_QP2
+#undef LML_QP1
+
+// the DPP control of the read from lane ^ D inside a quad
+template <int D>
+constexpr int dpp_xor_ctrl() {
+    return D == 1 ? DPP_XOR1 : D == 2 ? DPP_XOR2 : DPP_XOR3;
+}
+
 // Jacobi rotation of the 2x2 Gram matrix [[a_p, g], [g, a_q]] of a column pair,
 // u_p' = c (u_p - t u_q), u_q' = c (u_q + t u_p), with t = tan(theta) of the smaller
 // angle: for d = a_q - a_p and e = 2g (scaled by the column scales),
@@ -143,29 +277,24 @@ __device__ inline void jacobi_tc(float ap, float aq, float gam, float& t, float&
 }
 
 // One round of the one-sided (Hestenes) Jacobi method with whole columns: lane j
-// holds column j as sc * u[], its reciprocal scale isc and al = ||column||^2; the
-// pair of lane j is the lane that `xch` reads (lane ^ m).  The factor c of the
-// rotation is folded into the column scale (and 1/c = (1 + t^2) c into isc), so the
-// update is one FMA per element.  Both lanes of a pair form the rotation from the
+// holds column j as sc * u[], its reciprocal scale isc and al = ||column||^2; its
+// pair is lane j ^ D.  The rotation's factor c goes into the column scale (and
+// 1/c = (1 + t^2) c into isc), so the update is one FMA per element, reading the
+// partner's element through DPP.  Both lanes of a pair form the rotation from the
 // same operands in the same order, so they agree on it bit for bit.  Returns
 // |g| / sqrt(a_p a_q) before the rotation.
-template <int N, class X>
-__device__ inline float hestenes_round(float (&u)[N], float& sc, float& isc, float& al, bool low, X xch) {
-    float o[N];
-    float g4[4] = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-        o[i] = xch(u[i]);
-        g4[i & 3] = fmaf(u[i], o[i], g4[i & 3]);
-    }
+template <int N, int D>
+__device__ inline float hestenes_round(float (&u)[N], float& sc, float& isc, float& al, bool low) {
+    float g4[4];
+    dpp_self_dot<D, N>(g4, u);
+    const XorDpp<dpp_xor_ctrl<D>()> xch;
     const float sc_o = xch(sc), al_o = xch(al);
     const float gam = (sc * sc_o) * ((g4[0] + g4[1]) + (g4[2] + g4[3]));
     const float ap = low ? al : al_o, aq = low ? al_o : al;
     float t, c;
     jacobi_tc(ap, aq, gam, t, c);
     const float coef = (low ? -t : t) * (sc_o * isc);
-#pragma unroll
-    for (int i = 0; i < N; ++i) u[i] = fmaf(coef, o[i], u[i]);
+    dpp_self_axpy<D, N>(u, coef);
     sc *= c;
     isc *= fmaf(t, t, 1.f) * c;
     al = low ? ap - t * gam : aq + t * gam;
@@ -174,73 +303,81 @@ __device__ inline float hestenes_round(float (&u)[N], float& sc, float& isc, flo
 }
 
 // One round with split columns.  For the pair (P, Q) = (p, p ^ m), p with bit b
-// of m clear, the lane of P holds rows [0, N/2) of both columns and the lane of Q
-// rows [N/2, N): x[] the lane's own column, y[] its partner's ("guest" rows),
-// each with its scale, reciprocal scale and squared norm (sx, ix, ax; sy, iy, ay),
-// kept bit-identical in both lanes.  The pair inner product is the sum of the two
-// lanes' half sums (one exchange); each lane then updates its rows of both
-// columns.  No column moves.
+// of m clear, the lane of P holds rows [0, N/2) of column P in x[] and the lane of
+// Q rows [N/2, N) of column Q; the other half of each column (the lane's "guest"
+// rows: rows [0, N/2) of Q for the lane of P) lies in the y[] of lane ^ D, where
+// D = 0..3 is how far the walk over the masks has gone since the guest rows last
+// moved.  The lane keeps the scale, reciprocal scale and squared norm of its own
+// half (sx, ix, ax) and of its guest (sy, iy, ay).  The norms are bit-identical in
+// both lanes of a pair; the two halves of one column carry scales of their own.
+// The pair inner product is the sum of the two lanes' scaled half sums (one
+// exchange): both lanes must form it bit for bit alike, since they each derive
+// the rotation from it, so the sum is rounded once and not fused with the product
+// that scaled either half (a fused form rounds the partner's half only, and the
+// halves of a column then turn by angles that differ in the last bits: with nearly
+// equal eigenvalues the sweeps stop converging at cosines ~1e-5).
 // The rotation is written from the lane's own view, d' = a_guest - a_own: the
 // lane of Q sees d' = -d and so t' = -t exactly (sign flips are exact, and a tie
-// d = 0 takes t > 0 from P's view), which gives both lanes the coefficients
-//   own:   -t' s_guest / s_own,   guest:  t' s_own / s_guest
-// and the norms a_own - t' g, a_guest + t' g without selecting by role.
-template <int H>
-__device__ inline float split_round(float (&x)[H], float (&y)[H], float& sx, float& ix, float& ax, float& sy,
+// d = 0 takes t > 0 from P's view).  It is applied as two shears, so that both
+// updates are in place and each may read the other lane through DPP:
+//   x' = x - t' (sy/sx) y,  then  y'' = y + t' c^2 (sx/sy) x',  sx' = sx c,  sy' = sy q c,
+// with q = 1 + t'^2 (c^2 = 1/q), which gives sx' x' = c (X - t' Y) and
+// sy' y'' = c (Y + t' X) for X = sx x, Y = sy y (the second shear's coefficient is
+// that of the lane that owns the rows, lane ^ D).  The two halves of a column take
+// the shears in opposite order, so they agree to the rounding of c^2 q = 1 and of
+// the tracked reciprocal scales, a few ulp per rotation: measured on nearly scalar
+// and rank-deficient problems, the sweeps converge as far as with whole-column
+// rotations (and as with 1/q and the reciprocals refined by a Newton step).
+template <int H, int D>
+__device__ inline float shear_round(float (&x)[H], float (&y)[H], float& sx, float& ix, float& ax, float& sy,
                                     float& iy, float& ay, bool hi, XorBperm xch) {
-    float s4[4] = {0.f, 0.f, 0.f, 0.f};
+    float s4[4];
+    if constexpr (D == 0) {
 #pragma unroll
-    for (int i = 0; i < H; ++i) s4[i & 3] = fmaf(x[i], y[i], s4[i & 3]);
-    const float s = (s4[0] + s4[1]) + (s4[2] + s4[3]);
-    const float g = s + xch(s);      // the same sum in both lanes (addition commutes)
-    const float gam = (sx * sy) * g;  // (multiplication commutes)
-    const float dp = ay - ax, e = 2.f * gam;
+        for (int i = 0; i < 4; ++i) s4[i] = x[i] * y[i];
+#pragma unroll
+        for (int i = 4; i < H; ++i) s4[i & 3] = fmaf(x[i], y[i], s4[i & 3]);
+    } else {
+        dpp_dot<D, H>(s4, y, x);
+    }
+    const float ps = ((s4[0] + s4[1]) + (s4[2] + s4[3])) * (sx * sy);
+    const float g = add_uncontracted(ps, xch(ps));
+    const float dp = ay - ax, e = 2.f * g;
     const float den = fabsf(dp) + rot_sqrt(fmaf(dp, dp, e * e));
     const bool pos = (dp > 0.f) | ((dp == 0.f) & !hi);  // no short-circuit branch
     const float t = den > 0.f ? (pos ? e : -e) * rot_rcp(den) : 0.f;
-    const float c = rot_rsq(fmaf(t, t, 1.f));
-    const float co = -t * (sy * ix), cg = t * (sx * iy);
-    // the guest rows' new values go to fresh registers (the move that follows
-    // writes them back), the own rows are updated in place
+    const float q = fmaf(t, t, 1.f);
+    const float c = rot_rsq(q);
+    const float co = -t * (sy * ix), ka = t * (sx * iy) * (c * c);
+    if constexpr (D == 0) {
 #pragma unroll
-    for (int i = 0; i < H; ++i) {
-        const float yn = fmaf(cg, x[i], y[i]);
-        x[i] = fmaf(co, y[i], x[i]);
-        y[i] = yn;
+        for (int i = 0; i < H; ++i) x[i] = fmaf(co, y[i], x[i]);
+#pragma unroll
+        for (int i = 0; i < H; ++i) y[i] = fmaf(ka, x[i], y[i]);
+    } else {
+        dpp_axpy<D, H>(x, y, co);
+        dpp_axpy<D, H>(y, x, XorDpp<dpp_xor_ctrl<D>()>{}(ka));
     }
-    const float ic = fmaf(t, t, 1.f) * c;
+    const float ic = q * c;
     sx *= c;
-    sy *= c;
     ix *= ic;
-    iy *= ic;
+    sy *= ic;
+    iy *= c;
     // explicit FMAs: the lane of P forms a_Q as fma(t', g, a_Q) and the lane of Q
     // as fma(-t', g, a_Q) with its t' = -t, the same rounding
     const float den2 = ax * ay;
-    ax = fmaf(-t, gam, ax);
-    ay = fmaf(t, gam, ay);
-    return den2 > 0.f ? fabsf(gam) * rot_rsq(den2) : 0.f;
+    ax = fmaf(-t, g, ax);
+    ay = fmaf(t, g, ay);
+    return den2 > 0.f ? fabsf(g) * rot_rsq(den2) : 0.f;
 }
-// guest rows move to lane ^ (1 << j): y'[lane] = y[lane ^ (1 << j)]
-template <int H, class X>
-__device__ inline void move_guest(float (&y)[H], float& sy, float& iy, float& ay, X xch) {
-#pragma unroll
-    for (int i = 0; i < H; ++i) y[i] = xch(y[i]);
+
+// the guest's scales and norm follow the walk over the masks: from m to m ^ f the
+// lane's guest is the previous guest of lane ^ f
+template <class X>
+__device__ inline void move_guest_scalars(float& sy, float& iy, float& ay, X xch) {
     sy = xch(sy);
     iy = xch(iy);
     ay = xch(ay);
-}
-
-// guest rows move to lane ^ (1 << lb): a DPP move for lb = 0, 1, 3, else the crossbar
-template <int H>
-__device__ inline void move_guest_bit(float (&y)[H], float& sy, float& iy, float& ay, int lb, int lane) {
-    if (lb == 0)
-        move_guest<H>(y, sy, iy, ay, XorDpp<DPP_XOR1>{});
-    else if (lb == 1)
-        move_guest<H>(y, sy, iy, ay, XorDpp<DPP_XOR2>{});
-    else if (lb == 3)
-        move_guest<H>(y, sy, iy, ay, XorDpp<DPP_XOR8>{});
-    else
-        move_guest<H>(y, sy, iy, ay, XorBperm{(lane ^ (1 << lb)) << 2});
 }
 
 // One cyclic sweep (every column pair once) of the FP32 one-sided Jacobi method
@@ -249,11 +386,13 @@ __device__ inline void move_guest_bit(float (&y)[H], float& sy, float& iy, float
 // group b, lanes with bit b clear keep the first N/2 rows of their column and
 // receive the same rows of their partner's, the others the last N/2 rows (one
 // crossbar exchange of half a column), so each round costs half the inner-product
-// FMAs and no column exchange.  Inside a group the masks follow a Gray code: from
-// m to m ^ (1 << j) the guest rows move to lane ^ (1 << j), a quad permutation or a
-// row rotation for j = 0, 1, 3.  The group ends by returning the guest rows to their
-// lanes.  The rounds m = 1, 2, 3 pair lanes inside a quad and exchange whole
-// columns by quad permutations.
+// FMAs and no column exchange.  Inside a group the masks follow a Gray code, in
+// blocks of four rounds m0, m0 ^ 1, m0 ^ 3, m0 ^ 2: within a block the guest rows
+// stay where they are and are read through DPP (D = 0, 1, 3, 2), and between
+// blocks (a flip of bit j >= 2) they move once through the crossbar.  The group
+// ends by returning the guest rows to their lanes, scaled to the own half's scale.
+// The rounds m = 1, 2, 3 pair lanes inside a quad and read whole columns through
+// DPP.
 template <int N>
 __device__ inline float hestenes_sweep(float (&u)[N], float& sc, float& isc, float& al, int lane) {
     constexpr int H = N / 2;
@@ -276,37 +415,50 @@ __device__ inline float hestenes_sweep(float (&u)[N], float& sc, float& isc, flo
         float sy = XorBperm{(lane ^ gsize) << 2}(sc), iy = XorBperm{(lane ^ gsize) << 2}(isc),
               ay = XorBperm{(lane ^ gsize) << 2}(al);
         int m = gsize;
-        // rounds in twos (gsize is even): after round k the Gray code flips bit
-        // ctz(k + 1), which is bit 0 (a quad permutation) for every even k; the pairs
-        // also let the two rounds' updates alternate registers without copies.
-        // (Putting the frequent move on the crossbar instead, to offload the VALU,
-        // measured 5% slower: the crossbar is the scarcer of the two.)
-        for (int k = 0;; k += 2) {
-            mc = fmaxf(mc, split_round<H>(x, y, sx, ix, ax, sy, iy, ay, hi, XorBperm{(lane ^ m) << 2}));
+        for (int k = 0;; k += 4) {
+            mc = fmaxf(mc, shear_round<H, 0>(x, y, sx, ix, ax, sy, iy, ay, hi, XorBperm{(lane ^ m) << 2}));
             m ^= 1;
-            move_guest<H>(y, sy, iy, ay, XorDpp<DPP_XOR1>{});
-            mc = fmaxf(mc, split_round<H>(x, y, sx, ix, ax, sy, iy, ay, hi, XorBperm{(lane ^ m) << 2}));
-            if (k + 2 == gsize) break;
-            const int j = __builtin_ctz(k + 2);  // the Gray-code bit that flips (>= 1)
+            move_guest_scalars(sy, iy, ay, XorDpp<DPP_XOR1>{});
+            mc = fmaxf(mc, shear_round<H, 1>(x, y, sx, ix, ax, sy, iy, ay, hi, XorBperm{(lane ^ m) << 2}));
+            m ^= 2;
+            move_guest_scalars(sy, iy, ay, XorDpp<DPP_XOR2>{});
+            mc = fmaxf(mc, shear_round<H, 3>(x, y, sx, ix, ax, sy, iy, ay, hi, XorBperm{(lane ^ m) << 2}));
+            m ^= 1;
+            move_guest_scalars(sy, iy, ay, XorDpp<DPP_XOR1>{});
+            mc = fmaxf(mc, shear_round<H, 2>(x, y, sx, ix, ax, sy, iy, ay, hi, XorBperm{(lane ^ m) << 2}));
+            // the guest rows move once, on either path: after the group's last round
+            // the other half of the lane's column (the guest of lane ^ m, in lane
+            // ^ m ^ 2) comes home; otherwise, with the Gray-code bit j >= 2 flipping,
+            // the lane's new guest is the guest of lane ^ (1 << j), in lane ^ (1 << j) ^ 2
+            const bool last = k + 4 == gsize;
+            const int j = __builtin_ctz(k + 4);
+            const XorBperm mv{(lane ^ (last ? m : 1 << j) ^ 2) << 2};
+#pragma unroll
+            for (int i = 0; i < H; ++i) y[i] = mv(y[i]);
+            if (last) break;
             m ^= 1 << j;
-            move_guest_bit<H>(y, sy, iy, ay, j, lane);
+            if (j == 3)
+                move_guest_scalars(sy, iy, ay, XorDpp<DPP_XOR8>{});
+            else
+                move_guest_scalars(sy, iy, ay, XorBperm{(lane ^ (1 << j)) << 2});
         }
         {
-            const XorBperm back{(lane ^ m) << 2};
+            // bring the other half of the lane's column to this half's scale sx
+            const float r = XorBperm{(lane ^ m) << 2}(sy) / sx;
 #pragma unroll
             for (int i = 0; i < H; ++i) {
-                const float tv = back(y[i]);
+                const float tv = y[i] * r;
                 u[i] = hi ? tv : x[i];
                 u[H + i] = hi ? x[i] : tv;
             }
         }
         sc = sx;
-        isc = ix;
+        isc = 1.f / sx;
         al = ax;
     }
-    mc = fmaxf(mc, hestenes_round<N>(u, sc, isc, al, (lane & 1) == 0, XorDpp<DPP_XOR1>{}));
-    mc = fmaxf(mc, hestenes_round<N>(u, sc, isc, al, (lane & 2) == 0, XorDpp<DPP_XOR2>{}));
-    mc = fmaxf(mc, hestenes_round<N>(u, sc, isc, al, (lane & 2) == 0, XorDpp<DPP_XOR3>{}));
+    mc = fmaxf(mc, hestenes_round<N, 1>(u, sc, isc, al, (lane & 1) == 0));
+    mc = fmaxf(mc, hestenes_round<N, 2>(u, sc, isc, al, (lane & 2) == 0));
+    mc = fmaxf(mc, hestenes_round<N, 3>(u, sc, isc, al, (lane & 2) == 0));
     return mc;
 }
 
