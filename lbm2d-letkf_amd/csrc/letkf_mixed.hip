@@ -38,10 +38,13 @@ namespace lbmletkf {
 namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 struct MixedParams {
     int M, nodes, n_vars;
+    int n_obs;              // scalar observations (rows of dY)
     int nx, ny;             // grid (k_mixed_analysis walks its chains along columns)
     long long member_stride;
     float diag;             // (M-1)/beta
@@ -509,55 +512,67 @@ __device__ __forceinline__ void gram_acc(int g, int lane, const int* __restrict_
     const int kq = lane >> 4;    // which 8 of the 32 k-values this lane holds
     // Observation index j of the local set = entry e0 + j / nv, variable j % nv.  A
     // lane's 8 consecutive k-values step (e, va) without dividing; values beyond P
-    // and members beyond M read a valid element and contribute zero, so the loop has
-    // no branches and all its loads are in flight together.
+    // read a valid element with weight zero, so the loop has no branches and all its
+    // loads are in flight together.
     // the R^-1 weights as uniform values (a select between them, not an indexed load)
     const float cf0 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(mp.coef[0])));
     const float cf1 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(mp.coef[1])));
     const float cf2 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(mp.coef[2])));
+    // dY and the innovations through buffer loads: a 32-bit offset per observation,
+    // the member tiles 64 bytes apart as immediate offsets, and reads past the end of
+    // dY return 0.  Members 16 I + col >= M read the next row's elements (finite);
+    // their rows and columns of C and rows of b are cleared after the loop.
+    const __amdgpu_buffer_rsrc_t rdy =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(dy), 0, mp.n_obs * M * 4, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rdyo =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(dyo), 0, mp.n_obs * 4, 0x00020000);
+    const float cmask = col == 0 ? 1.f : 0.f;
     for (int base = 0; base < P; base += 32) {
-        float x[NT][8], inn[8];
         const int jj0 = base + 8 * kq;
         int e = e0 + jj0 / nv, va = jj0 % nv;
-        unsigned oo[8];
+        int oo[8];
         float sw[8];
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
             const bool ok = jj0 + t < P;
             const int ec = ok ? e : e0, vc = ok ? va : 0;
             const float cf = vc == 0 ? cf0 : (vc == 1 ? cf1 : cf2);
-            oo[t] = unsigned(loc_site[ec]) * unsigned(nv) + unsigned(vc);
+            oo[t] = loc_site[ec] * nv + vc;
             const float w = loc_w[ec];
             sw[t] = __builtin_amdgcn_sqrtf(w * cf) * (ok ? 1.f : 0.f);  // 1 ulp; 0 beyond P
             const bool wrapv = va + 1 == nv;
             va = wrapv ? 0 : va + 1;
             e += wrapv ? 1 : 0;
         }
-        // masks as factors, so the loads stay unconditional
-        float mmask[NT];
-#pragma unroll
-        for (int I = 0; I < NT; ++I) mmask[I] = (16 * I + col < M) ? 1.f : 0.f;
-        const float cmask = col == 0 ? 1.f : 0.f;
+        float x[NT + 1][8];
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
+            const int vo = (oo[t] * M + col) * 4;
 #pragma unroll
-            for (int I = 0; I < NT; ++I) {
-                const int mm = 16 * I + col;
-                const float d = dy[(size_t)oo[t] * M + (mm < M ? mm : M - 1)];
-                x[I][t] = d * (sw[t] * mmask[I]);
-            }
-            inn[t] = dyo[oo[t]] * (sw[t] * cmask);
+            for (int I = 0; I < NT; ++I)
+                x[I][t] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rdy, vo + 64 * I, 0, 0)) *
+                          sw[t];
+            x[NT][t] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rdyo, oo[t] * 4, 0, 0)) *
+                       (sw[t] * cmask);
         }
+        // split x = hi + lo, both bf16 (hi rounded to nearest), two values per conversion
         bf16x8 hi[NT + 1], lo[NT + 1];
 #pragma unroll
         for (int I = 0; I <= NT; ++I) {
+            bf16x2 h[4], l[4];
 #pragma unroll
-            for (int t = 0; t < 8; ++t) {
-                const float xv = (I < NT) ? x[I][t] : inn[t];
-                const __bf16 h = (__bf16)xv;
-                hi[I][t] = h;
-                lo[I][t] = (__bf16)(xv - (float)h);
+            for (int t = 0; t < 4; ++t) {
+                const float a = x[I][2 * t], b = x[I][2 * t + 1];
+                h[t] = __builtin_convertvector(f32x2{a, b}, bf16x2);
+                const unsigned bits = __builtin_bit_cast(unsigned, h[t]);
+                const float ha = __builtin_bit_cast(float, bits << 16);
+                const float hb = __builtin_bit_cast(float, bits & 0xffff0000u);
+                l[t] = __builtin_convertvector(f32x2{a - ha, b - hb}, bf16x2);
             }
+            hi[I] = __builtin_shufflevector(__builtin_shufflevector(h[0], h[1], 0, 1, 2, 3),
+                                            __builtin_shufflevector(h[2], h[3], 0, 1, 2, 3), 0, 1, 2, 3, 4, 5, 6, 7);
+            lo[I] = __builtin_shufflevector(__builtin_shufflevector(l[0], l[1], 0, 1, 2, 3),
+                                            __builtin_shufflevector(l[2], l[3], 0, 1, 2, 3), 0, 1, 2, 3, 4, 5, 6, 7);
         }
         int tix = 0;
 #pragma unroll
@@ -579,6 +594,35 @@ __device__ __forceinline__ void gram_acc(int g, int lane, const int* __restrict_
                 acc[NTILE + I] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(lo[I], hi[NT], acc[NTILE + I], 0, 0, 0);
             }
         }
+    }
+    if (M < N) {
+        // members >= M: rows and columns of C, rows of b (tile element r of lane (col,
+        // kq) is row 16 I + 4 kq + r, column 16 J + col), by factors formed here from an
+        // opaque copy of the lane index (hoisted out of the chain loop, the per-element
+        // conditions would be held across it)
+        int ln = lane;
+        asm volatile("" : "+v"(ln));
+        const int cl = ln & 15, kl = ln >> 4;
+        float rowm[NT][4], colm[NT];
+#pragma unroll
+        for (int I = 0; I < NT; ++I) {
+            colm[I] = 16 * I + cl < M ? 1.f : 0.f;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) rowm[I][r] = 16 * I + 4 * kl + r < M ? 1.f : 0.f;
+        }
+        int tz = 0;
+#pragma unroll
+        for (int I = 0; I < NT; ++I) {
+#pragma unroll
+            for (int J = FULL ? 0 : I; J < NT; ++J, ++tz) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) acc[tz][r] *= rowm[I][r] * colm[J];
+            }
+        }
+#pragma unroll
+        for (int I = 0; I < NT; ++I)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc[NTILE + I][r] *= rowm[I][r];
     }
 }
 
@@ -1496,6 +1540,7 @@ MixedParams mixed_params(const Ctx& c, unsigned long long* sweeps32, unsigned lo
     mp.nx = c.nx;
     mp.ny = c.ny;
     mp.n_vars = c.obs.n_vars;
+    mp.n_obs = c.obs.n_sites * c.obs.n_vars;
     mp.member_stride = c.lp.member_stride;
     mp.diag = float(double(c.M - 1) / c.cfg.inflation);
     for (int a = 0; a < 3; ++a) mp.coef[a] = c.obs.inv_sigma2[a];
