@@ -387,36 +387,39 @@ __device__ inline void move_guest_scalars(float& sy, float& iy, float& ay, X xch
 // on columns held whole (lane j = column j, N columns), returning the largest pair
 // cosine met.  The rounds m = 4..N-1 go in groups of equal top bit b of m: for
 // group b, lanes with bit b clear keep the first N/2 rows of their column and
-// receive the same rows of their partner's, the others the last N/2 rows (one
-// crossbar exchange of half a column), so each round costs half the inner-product
-// FMAs and no column exchange.  Inside a group the masks follow a Gray code, in
-// blocks of four rounds m0, m0 ^ 1, m0 ^ 3, m0 ^ 2: within a block the guest rows
-// stay where they are and are read through DPP (D = 0, 1, 3, 2), and between
-// blocks (a flip of bit j >= 2) they move once through the crossbar.  The group
-// ends by returning the guest rows to their lanes, scaled to the own half's scale.
-// The rounds m = 1, 2, 3 pair lanes inside a quad and read whole columns through
-// DPP.
+// receive the same rows of their partner's, the others the last N/2 rows, so each
+// round costs half the inner-product FMAs and no column exchange.  Inside a group
+// the masks follow a Gray code, in blocks of four rounds m0, m0 ^ 1, m0 ^ 3, m0 ^ 2:
+// within a block the guest rows stay where they are and are read through DPP
+// (D = 0, 1, 3, 2), and between blocks (a flip of bit j >= 2) they move once through
+// the crossbar.  A group ends with the other half of each lane's column back in the
+// lane; the next group keeps whichever half its split gives the lane and sends the
+// other half to its new partner (one crossbar exchange, the halves keeping scales of
+// their own), and only the last group rebuilds whole columns.  The rounds m = 1, 2, 3
+// pair lanes inside a quad and read whole columns through DPP.
 template <int N>
 __device__ inline float hestenes_sweep(float (&u)[N], float& sc, float& isc, float& al, int lane) {
     constexpr int H = N / 2;
     constexpr int LG = (N == 64) ? 6 : (N == 32) ? 5 : 4;
     static_assert(N == 16 || N == 32 || N == 64, "column count");
     float mc = 0.f;
-    for (int b = LG - 1; b >= 2; --b) {
-        const bool hi = ((lane >> b) & 1) != 0;
-        const int gsize = 1 << b;
-        float x[H], y[H];
-        {
-            const XorBperm to{(lane ^ gsize) << 2};
+    int b = LG - 1;
+    bool hi = ((lane >> b) & 1) != 0;
+    int gsize = 1 << b;
+    float x[H], y[H];
+    {
+        const XorBperm to{(lane ^ gsize) << 2};
 #pragma unroll
-            for (int i = 0; i < H; ++i) {
-                x[i] = hi ? u[H + i] : u[i];
-                y[i] = to(hi ? u[i] : u[H + i]);
-            }
+        for (int i = 0; i < H; ++i) {
+            x[i] = hi ? u[H + i] : u[i];
+            y[i] = to(hi ? u[i] : u[H + i]);
         }
-        float sx = sc, ix = isc, ax = al;
-        float sy = XorBperm{(lane ^ gsize) << 2}(sc), iy = XorBperm{(lane ^ gsize) << 2}(isc),
-              ay = XorBperm{(lane ^ gsize) << 2}(al);
+    }
+    float sx = sc, ix = isc, ax = al;
+    float sy = XorBperm{(lane ^ gsize) << 2}(sc), iy = XorBperm{(lane ^ gsize) << 2}(isc),
+          ay = XorBperm{(lane ^ gsize) << 2}(al);
+    float so, io;  // scale and reciprocal scale of the other half of the lane's column
+    for (;;) {
         int m = gsize;
         for (int k = 0;; k += 4) {
             mc = fmaxf(mc, shear_round<H, 0>(x, y, sx, ix, ax, sy, iy, ay, hi, XorBperm{(lane ^ m) << 2}));
@@ -446,19 +449,46 @@ __device__ inline float hestenes_sweep(float (&u)[N], float& sc, float& isc, flo
                 move_guest_scalars(sy, iy, ay, XorBperm{(lane ^ (1 << j)) << 2});
         }
         {
-            // bring the other half of the lane's column to this half's scale sx
-            const float r = XorBperm{(lane ^ m) << 2}(sy) / sx;
-#pragma unroll
-            for (int i = 0; i < H; ++i) {
-                const float tv = y[i] * r;
-                u[i] = hi ? tv : x[i];
-                u[H + i] = hi ? x[i] : tv;
-            }
+            const XorBperm om{(lane ^ m) << 2};
+            so = om(sy);
+            io = om(iy);
         }
-        sc = sx;
-        isc = 1.f / sx;
-        al = ax;
+        if (b == 2) break;
+        // into group b - 1: keep the half of the column that its split gives the lane,
+        // send the other half (and its scales) to lane ^ 2^(b-1), receive that lane's
+        const bool nhi = ((lane >> (b - 1)) & 1) != 0;
+        const bool same = hi == nhi;
+        gsize = 1 << (b - 1);
+        const XorBperm to{(lane ^ gsize) << 2};
+        if (!same) {
+            // the lanes whose half changes swap their two halves in place (v_swap_b32
+            // under the lane mask: one instruction per element, no third register)
+#pragma unroll
+            for (int i = 0; i < H; ++i) asm volatile("v_swap_b32 %0, %1" : "+v"(x[i]), "+v"(y[i]));
+        }
+#pragma unroll
+        for (int i = 0; i < H; ++i) y[i] = to(y[i]);
+        sy = to(same ? so : sx);
+        iy = to(same ? io : ix);
+        ay = to(ax);
+        sx = same ? sx : so;
+        ix = same ? ix : io;
+        hi = nhi;
+        --b;
     }
+    {
+        // whole columns again, the other half brought to this half's scale sx
+        const float r = so / sx;
+#pragma unroll
+        for (int i = 0; i < H; ++i) {
+            const float tv = y[i] * r;
+            u[i] = hi ? tv : x[i];
+            u[H + i] = hi ? x[i] : tv;
+        }
+    }
+    sc = sx;
+    isc = 1.f / sx;
+    al = ax;
     mc = fmaxf(mc, hestenes_round<N, 1>(u, sc, isc, al, (lane & 1) == 0));
     mc = fmaxf(mc, hestenes_round<N, 2>(u, sc, isc, al, (lane & 2) == 0));
     mc = fmaxf(mc, hestenes_round<N, 3>(u, sc, isc, al, (lane & 2) == 0));
