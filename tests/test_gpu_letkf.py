@@ -123,8 +123,11 @@ def test_analysis_matches_oracle(lb, orc, case, form):
     var_ids = [v for v, bit in ((0, 1), (1, 2), (2, 4)) if cfg.obs_vars & bit]
     # V's orthogonality: FP64 rotations (plain) keep it to rounding; the FP32 phase
     # of the mixed solver accumulates ~sweeps * M rotations of FP32 rounding each
-    # (measured 1.3e-5 at M = 32 with the near-singular rloc problem).
-    tol_orth = 1e-12 if form == "plain" else 4e-5
+    # (measured <= 2.3e-6 over these cases, the near-singular rloc problem at M = 32
+    # the largest).  The bound also catches rotations that the two lanes holding the
+    # halves of a column apply differently (1.8e-4 on rloc when their inner products
+    # differed in the last bit).
+    tol_orth = 1e-12 if form == "plain" else 2e-5
     errC = errEVD = errOrth = 0.0
     for i in range(nd):
         sites, wl = orc.local_obs(cfg, g0 + i)
