@@ -89,6 +89,9 @@ CASES = {
     "ref_gc_p4_M8": dict(preset="ref", n=64, M=8, obs_stride=4),
     "ref_pointwise_M4": dict(preset="ref", n=64, M=4, obs_stride=1),
     "ref_rloc_M32": dict(preset="ref", n=64, M=32, obs_stride=4, loc_rloc_len=1, inflation=1.05),
+    # member counts that fill the last tile of 16 only in part (N = 32, 64)
+    "baseline_gauss_M20": dict(preset="baseline", n=64, M=20, obs_stride=8, loc_scale=6.0, loc_cutoff=20.0),
+    "baseline_gauss_M40": dict(preset="baseline", n=64, M=40, obs_stride=4, loc_scale=6.0, loc_cutoff=14.0),
     # C scaled by 1e10 (R^-1 and (M-1)/beta alike; same eigenvectors and condition):
     # squared column norms ~1e22 take the solvers' power-of-two prescaling (above 1e18)
     "baseline_gauss_M16_scaled": dict(preset="baseline", n=64, M=16, obs_stride=8, loc_scale=6.0, loc_cutoff=20.0,
