@@ -476,7 +476,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this build targets one MI355X; "
-              "the multi-GPU layout is described in DESIGN.md", file=sys.stderr)
+              "each GPU would step its own members and analyse its own strip of grid columns", file=sys.stderr)
         sys.exit(2)
     if args.copy_probe:
         hip = Hip()

@@ -1,4 +1,4 @@
-"""The learned closure inside the LETKF OSSE, judged against Smagorinsky.
+"""The learned closure inside the LETKF OSSE, compared with Smagorinsky.
 
 The nature run is a fine run of the same LBM (grid refined by ``factor``, same c,
 Smagorinsky off), box-filtered to the coarse grid at every observation time:
@@ -6,7 +6,7 @@ the filtered high-resolution truth.  Observations of (u, v) are taken from that
 filtered truth on the coarse configuration's network with its error sigma.  Two
 coarse ensembles, identical apart from their eddy-viscosity closure (Smagorinsky
 or the trained network), assimilate the same observations; the analysis RMSE of
-each against the filtered truth is the judge.
+each against the filtered truth decides between them.
 """
 from __future__ import annotations
 

@@ -709,7 +709,7 @@ void orc_ensemble_stats(const orc_config* cfg, int M, const double* f, const dou
 }  // extern "C"
 
 // ---------------------------------------------------------------------------
-// learned eddy-viscosity closure (a north-star extension, no reference code)
+// learned eddy-viscosity closure (an extension beyond the reference: no reference code)
 // ---------------------------------------------------------------------------
 namespace {
 

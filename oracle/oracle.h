@@ -103,7 +103,7 @@ int  orc_letkf_analysis_strip(const orc_config* cfg, int M, double* f, const dou
 void orc_ensemble_stats(const orc_config* cfg, int M, const double* f, const double* trho,
                         const double* tu, const double* tv, double out[4]);
 
-/* ---- learned eddy-viscosity closure (a north-star extension; no reference code) ----
+/* ---- learned eddy-viscosity closure (an extension beyond the reference; no reference code) ----
  * features at node n: for the 3x3 stencil s = (dj+1)*3 + (di+1), the deviatoric
  * strain a = (ux - vy)/2 and shear b = (uy + vx)/2 at node (i+di, j+dj) from the
  * central differences of src/lbm.cu:44-47, divided by strain_scale:
