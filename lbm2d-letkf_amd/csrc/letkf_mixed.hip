@@ -50,8 +50,8 @@ struct MixedParams {
     float diag;             // (M-1)/beta
     float coef[3];          // R^-1 per observed variable
     double sqm1;            // sqrt(M-1)
-    float check_after;      // FP32 phase: after a sweep whose largest cosine is below this, the
-    float check_tol;        //   columns' Gram (matrix cores) may stop it below check_tol
+    float check_tol;        // fused FP32 phase: stop when the columns' Gram (matrix cores) has
+                            //   no pair cosine above this
     float tol32;            // FP32 phase: stop after a sweep whose largest pair cosine is below
     double tol64;           // FP64 phase: the same test, then at most max64 sweeps
     int max32, max64;
@@ -285,8 +285,8 @@ __device__ inline void jacobi_tc(float ap, float aq, float gam, float& t, float&
 // 1/c = (1 + t^2) c into isc), so the update is one FMA per element, reading the
 // partner's element through DPP.  Both lanes of a pair form the rotation from the
 // same operands in the same order, so they agree on it bit for bit.  Returns
-// |g| / sqrt(a_p a_q) before the rotation.
-template <int N, int D>
+// |g| / sqrt(a_p a_q) before the rotation (COS), or 0.
+template <int N, int D, bool COS>
 __device__ inline float hestenes_round(float (&u)[N], float& sc, float& isc, float& al, bool low) {
     float g4[4];
     dpp_self_dot<D, N>(g4, u);
@@ -301,6 +301,7 @@ __device__ inline float hestenes_round(float (&u)[N], float& sc, float& isc, flo
     sc *= c;
     isc *= fmaf(t, t, 1.f) * c;
     al = low ? ap - t * gam : aq + t * gam;
+    if constexpr (!COS) return 0.f;
     const float den = ap * aq;
     return den > 0.f ? fabsf(gam) * rot_rsq(den) : 0.f;
 }
@@ -331,7 +332,7 @@ __device__ inline float hestenes_round(float (&u)[N], float& sc, float& isc, flo
 // the tracked reciprocal scales, a few ulp per rotation: measured on nearly scalar
 // and rank-deficient problems, the sweeps converge as far as with whole-column
 // rotations (and as with 1/q and the reciprocals refined by a Newton step).
-template <int H, int D>
+template <int H, int D, bool COS>
 __device__ inline float shear_round(float (&x)[H], float (&y)[H], float& sx, float& ix, float& ax, float& sy,
                                     float& iy, float& ay, bool hi, XorBperm xch) {
     float s4[4];
@@ -371,6 +372,7 @@ __device__ inline float shear_round(float (&x)[H], float (&y)[H], float& sx, flo
     const float den2 = ax * ay;
     ax = fmaf(-t, g, ax);
     ay = fmaf(t, g, ay);
+    if constexpr (!COS) return 0.f;
     return den2 > 0.f ? fabsf(g) * rot_rsq(den2) : 0.f;
 }
 
@@ -396,8 +398,9 @@ __device__ inline void move_guest_scalars(float& sy, float& iy, float& ay, X xch
 // lane; the next group keeps whichever half its split gives the lane and sends the
 // other half to its new partner (one crossbar exchange, the halves keeping scales of
 // their own), and only the last group rebuilds whole columns.  The rounds m = 1, 2, 3
-// pair lanes inside a quad and read whole columns through DPP.
-template <int N>
+// pair lanes inside a quad and read whole columns through DPP.  With COS the sweep
+// returns the largest pair cosine it met (before each rotation), else 0.
+template <int N, bool COS>
 __device__ inline float hestenes_sweep(float (&u)[N], float& sc, float& isc, float& al, int lane) {
     constexpr int H = N / 2;
     constexpr int LG = (N == 64) ? 6 : (N == 32) ? 5 : 4;
@@ -422,16 +425,16 @@ __device__ inline float hestenes_sweep(float (&u)[N], float& sc, float& isc, flo
     for (;;) {
         int m = gsize;
         for (int k = 0;; k += 4) {
-            mc = fmaxf(mc, shear_round<H, 0>(x, y, sx, ix, ax, sy, iy, ay, hi, XorBperm{(lane ^ m) << 2}));
+            mc = fmaxf(mc, shear_round<H, 0, COS>(x, y, sx, ix, ax, sy, iy, ay, hi, XorBperm{(lane ^ m) << 2}));
             m ^= 1;
             move_guest_scalars(sy, iy, ay, XorDpp<DPP_XOR1>{});
-            mc = fmaxf(mc, shear_round<H, 1>(x, y, sx, ix, ax, sy, iy, ay, hi, XorBperm{(lane ^ m) << 2}));
+            mc = fmaxf(mc, shear_round<H, 1, COS>(x, y, sx, ix, ax, sy, iy, ay, hi, XorBperm{(lane ^ m) << 2}));
             m ^= 2;
             move_guest_scalars(sy, iy, ay, XorDpp<DPP_XOR2>{});
-            mc = fmaxf(mc, shear_round<H, 3>(x, y, sx, ix, ax, sy, iy, ay, hi, XorBperm{(lane ^ m) << 2}));
+            mc = fmaxf(mc, shear_round<H, 3, COS>(x, y, sx, ix, ax, sy, iy, ay, hi, XorBperm{(lane ^ m) << 2}));
             m ^= 1;
             move_guest_scalars(sy, iy, ay, XorDpp<DPP_XOR1>{});
-            mc = fmaxf(mc, shear_round<H, 2>(x, y, sx, ix, ax, sy, iy, ay, hi, XorBperm{(lane ^ m) << 2}));
+            mc = fmaxf(mc, shear_round<H, 2, COS>(x, y, sx, ix, ax, sy, iy, ay, hi, XorBperm{(lane ^ m) << 2}));
             // the guest rows move once, on either path: after the group's last round
             // the other half of the lane's column (the guest of lane ^ m, in lane
             // ^ m ^ 2) comes home; otherwise, with the Gray-code bit j >= 2 flipping,
@@ -489,9 +492,9 @@ __device__ inline float hestenes_sweep(float (&u)[N], float& sc, float& isc, flo
     sc = sx;
     isc = 1.f / sx;
     al = ax;
-    mc = fmaxf(mc, hestenes_round<N, 1>(u, sc, isc, al, (lane & 1) == 0));
-    mc = fmaxf(mc, hestenes_round<N, 2>(u, sc, isc, al, (lane & 2) == 0));
-    mc = fmaxf(mc, hestenes_round<N, 3>(u, sc, isc, al, (lane & 2) == 0));
+    mc = fmaxf(mc, hestenes_round<N, 1, COS>(u, sc, isc, al, (lane & 1) == 0));
+    mc = fmaxf(mc, hestenes_round<N, 2, COS>(u, sc, isc, al, (lane & 2) == 0));
+    mc = fmaxf(mc, hestenes_round<N, 3, COS>(u, sc, isc, al, (lane & 2) == 0));
     return mc;
 }
 
@@ -762,9 +765,12 @@ __device__ inline void warm_start(Rows rows, float* cfull, int lane) {
 template <int N>
 __device__ float gram_max_cos(const float (&u)[N], float* cv, int lane);
 
-template <int N>
+// With CHECK, the columns' largest pair cosine is measured on the matrix cores
+// (check_lds: scratch for gram_max_cos) after each sweep from sweep `first` on, and
+// that measurement alone decides the stop; `sweeps` returns the sweep count.
+template <int N, bool CHECK>
 __device__ __forceinline__ float jacobi_fp32(float (&u)[N], float& sc, const float* cfull, int lane,
-                                             const MixedParams& mp, float* check_lds = nullptr) {
+                                             const MixedParams& mp, float* check_lds, int first, int& sweeps) {
     const int l = lane & (N - 1);
     float a0 = 0.f;
 #pragma unroll
@@ -788,22 +794,44 @@ __device__ __forceinline__ float jacobi_fp32(float (&u)[N], float& sc, const flo
     // halves from one sweep to the next: that is the FP32 rounding floor of
     // this matrix (convergence, even slowed by clustered eigenvalues, keeps
     // dividing it by more than 2).
-    float al = 0.f, mc_prev = 1.f;
+    float al = 0.f;
     sc = 1.f;
     int sw32 = 0;
-    while (sw32 < mp.max32) {
-        hestenes_renorm<N, float>(u, sc, al);
-        float isc = 1.f;
-        float mc = hestenes_sweep<N>(u, sc, isc, al, lane);
-        mc = wave_max(mc);
-        ++sw32;
-        if (mc <= mp.tol32 || (mc <= 1e-4f && mc > 0.5f * mc_prev)) break;
-        // Near convergence the next sweep mostly confirms it: measure the columns'
-        // largest cosine directly and stop when it is already below check_tol.
-        if (check_lds && mc <= mp.check_after && gram_max_cos<N>(u, check_lds, lane) <= mp.check_tol) break;
-        mc_prev = mc;
+    if constexpr (CHECK) {
+        // The sweeps form no cosine of their own (a product, a reciprocal square root
+        // and a maximum per round).  The cosine measured after sweep k is, to first
+        // order, the largest one that sweep k + 1 meets before its rotations, so when
+        // the rate test on it fires, one more sweep runs, as the same test on the
+        // cosines met by the sweeps would have it: on clustered eigenvalues that sweep
+        // still halves the columns' departure from orthogonality.
+        float gm_prev = 1.f;
+        bool last = false;
+        while (sw32 < mp.max32) {
+            hestenes_renorm<N, float>(u, sc, al);
+            float isc = 1.f;
+            hestenes_sweep<N, false>(u, sc, isc, al, lane);
+            ++sw32;
+            if (last) break;
+            if (sw32 < first) continue;
+            const float gm = gram_max_cos<N>(u, check_lds, lane);
+            if (gm <= mp.check_tol) break;
+            last = gm <= 1e-4f && gm > 0.5f * gm_prev;
+            gm_prev = gm;
+        }
+    } else {
+        float mc_prev = 1.f;
+        while (sw32 < mp.max32) {
+            hestenes_renorm<N, float>(u, sc, al);
+            float isc = 1.f;
+            float mc = hestenes_sweep<N, true>(u, sc, isc, al, lane);
+            mc = wave_max(mc);
+            ++sw32;
+            if (mc <= mp.tol32 || (mc <= 1e-4f && mc > 0.5f * mc_prev)) break;
+            mc_prev = mc;
+        }
     }
     if (lane == 0 && mp.sweeps32) atomicAdd(mp.sweeps32, (unsigned long long)sw32);
+    sweeps = sw32;
     return uscale;
 }
 
@@ -842,7 +870,8 @@ __device__ __forceinline__ void solve_body(int bx, int nbx, const int* __restric
             __syncthreads();
         }
         float sc;
-        const float uscale = jacobi_fp32<N>(u, sc, cfull, lane, mp);
+        int sweeps;
+        const float uscale = jacobi_fp32<N, false>(u, sc, cfull, lane, mp, nullptr, 0, sweeps);
         if (lane < N) {
             float* ug = ws_u + (long long)g * N * N;
             const float so = sc * uscale;
@@ -1072,6 +1101,7 @@ __device__ __forceinline__ void analysis_body(int bx, int nbx, const int* __rest
     const int gy0 = (blk / mp.nx) * mp.chain;
     const int gy1 = min(gy0 + mp.chain, mp.ny);
     bool bad = false;
+    int sweeps = 2;
     for (int gy = gy0; gy < gy1; ++gy) {
         const int g = gy * mp.nx + gx;
         f32x4 acc[GRAM_FULL<N>];
@@ -1094,7 +1124,13 @@ __device__ __forceinline__ void analysis_body(int bx, int nbx, const int* __rest
         __syncthreads();
         float u[N];
         float sc;
-        const float uscale = jacobi_fp32<N>(u, sc, sh.cv, lane, mp, sh.cv);
+        // The first measurement follows the second sweep, or for a warm start the
+        // sweep count of the previous gridpoint up to 3: the cosines of a warm start
+        // fall ~0.1 -> 1e-3 -> floor, and a measurement that fails costs about a
+        // seventh of a sweep (0.22 ms per analysis at 256^2 x 64 when made after
+        // every warm start's second sweep).
+        const int first = gy == gy0 ? 2 : min(max(sweeps, 2), 3);
+        const float uscale = jacobi_fp32<N, true>(u, sc, sh.cv, lane, mp, sh.cv, first, sweeps);
 
         // ---- eigenpairs: lambda_i = ||u_i|| in FP64, v_i = u_i / ||u_i|| ------------------
         double n4[4] = {0.0, 0.0, 0.0, 0.0};
@@ -1580,13 +1616,9 @@ MixedParams mixed_params(const Ctx& c, unsigned long long* sweeps32, unsigned lo
     // basis, and any non-orthogonality left there becomes an error of the next
     // gridpoint's eigenpairs that no later sweep removes.
     mp.tol32 = c.cfg.jacobi_tol32 > 0 ? float(c.cfg.jacobi_tol32) : 3e-6f;
-#ifndef LBMLETKF_CHECK_AFTER
-#define LBMLETKF_CHECK_AFTER 3e-3f
-#endif
 #ifndef LBMLETKF_CHECK_DIV
 #define LBMLETKF_CHECK_DIV 3.f
 #endif
-    mp.check_after = LBMLETKF_CHECK_AFTER;
     mp.check_tol = mp.tol32 / LBMLETKF_CHECK_DIV;
     mp.tol64 = c.cfg.jacobi_tol;
     mp.max32 = c.cfg.jacobi_max_sweeps;
