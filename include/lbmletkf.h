@@ -100,8 +100,11 @@ typedef struct lbmletkf_config {
     double closure_strain_scale; /* MLP inputs are strain components / this scale            */
     double closure_nu_scale;     /* MLP output nu_t = this scale * softplus(z)               */
     double closure_nu_max;       /* nu_t above this (or not finite) falls back to Smagorinsky */
-    /* MIXED: the FP32 phase stops after the sweep whose largest pair cosine is at
-     * most this (0: 3e-6), or once that cosine, below 1e-4, no longer halves per sweep */
+    /* MIXED: FP32 stop tolerance (0: 3e-6).  Without FP64 sweeps the phase stops
+     * once the columns' largest pair cosine, measured after a sweep, is at most a
+     * third of this, or one sweep after that measurement, below 1e-4, stops halving;
+     * with FP64 sweeps it stops after the sweep whose largest pair cosine met is at
+     * most this, or once that cosine, below 1e-4, no longer halves per sweep */
     double jacobi_tol32;
 } lbmletkf_config;
 
