@@ -348,8 +348,12 @@ __device__ inline float shear_round(float (&x)[H], float (&y)[H], float& sx, flo
     const float g = add_uncontracted(ps, xch(ps));
     const float dp = ay - ax, e = 2.f * g;
     const float den = fabsf(dp) + rot_sqrt(fmaf(dp, dp, e * e));
-    const bool pos = (dp > 0.f) | ((dp == 0.f) & !hi);  // no short-circuit branch
-    const float t = den > 0.f ? (pos ? e : -e) * rot_rcp(den) : 0.f;
+    // t > 0 for d' > 0, and for a tie d' = 0 in the lane of P: one integer compare
+    // of the bits of d' with 0 in the lane of P and 1 in the lane of Q (+0 and the
+    // positive floats have bits >= 0, the positive floats >= 1, the negative < 0)
+    const bool pos = __float_as_int(dp) >= int(hi);
+    // den = 0 only where e = 0, and then any finite 1/den gives t = 0
+    const float t = (pos ? e : -e) * rot_rcp(fmaxf(den, 0x1p-126f));
     const float q = fmaf(t, t, 1.f);
     const float c = rot_rsq(q);
     const float co = -t * (sy * ix), ka = t * (sx * iy) * (c * c);
