@@ -312,7 +312,11 @@ __device__ inline float hestenes_round(float (&u)[N], float& sc, float& isc, flo
 // rows: rows [0, N/2) of Q for the lane of P) lies in the y[] of lane ^ D, where
 // D = 0..3 is how far the walk over the masks has gone since the guest rows last
 // moved.  The lane keeps the scale, reciprocal scale and squared norm of its own
-// half (sx, ix, ax) and of its guest (sy, iy, ay).  The norms are bit-identical in
+// half (sx, ix, ax), and those of the rows it holds in y[] (sy, iy, ay: the guest
+// of lane ^ D), which it reads of lane ^ D through DPP operands and updates with
+// the rotation of lane ^ D.  A column's norm, kept by the lane of its first half
+// and by the holder of its other half, is updated alike in both (a_own - t' g and
+// a_held + t g with the same rounded product), so the norms are bit-identical in
 // both lanes of a pair; the two halves of one column carry scales of their own.
 // The pair inner product is the sum of the two lanes' scaled half sums (one
 // exchange): both lanes must form it bit for bit alike, since they each derive
@@ -332,6 +336,15 @@ __device__ inline float hestenes_round(float (&u)[N], float& sc, float& isc, flo
 // the tracked reciprocal scales, a few ulp per rotation: measured on nearly scalar
 // and rank-deficient problems, the sweeps converge as far as with whole-column
 // rotations (and as with 1/q and the reciprocals refined by a Newton step).
+// a value of lane ^ D (D = 0: the lane's own)
+template <int D>
+__device__ __forceinline__ float from_quad(float v) {
+    if constexpr (D == 0)
+        return v;
+    else
+        return XorDpp<dpp_xor_ctrl<D>()>{}(v);
+}
+
 template <int H, int D, bool COS>
 __device__ inline float shear_round(float (&x)[H], float (&y)[H], float& sx, float& ix, float& ax, float& sy,
                                     float& iy, float& ay, bool hi, XorBperm xch) {
@@ -344,9 +357,10 @@ __device__ inline float shear_round(float (&x)[H], float (&y)[H], float& sx, flo
     } else {
         dpp_dot<D, H>(s4, y, x);
     }
-    const float ps = ((s4[0] + s4[1]) + (s4[2] + s4[3])) * (sx * sy);
+    const float gsy = from_quad<D>(sy), gay = from_quad<D>(ay);  // the guest's, held by lane ^ D
+    const float ps = ((s4[0] + s4[1]) + (s4[2] + s4[3])) * (sx * gsy);
     const float g = add_uncontracted(ps, xch(ps));
-    const float dp = ay - ax, e = 2.f * g;
+    const float dp = gay - ax, e = 2.f * g;
     const float den = fabsf(dp) + rot_sqrt(fmaf(dp, dp, e * e));
     // t > 0 for d' > 0, and for a tie d' = 0 in the lane of P: one integer compare
     // of the bits of d' with 0 in the lane of P and 1 in the lane of Q (+0 and the
@@ -356,37 +370,30 @@ __device__ inline float shear_round(float (&x)[H], float (&y)[H], float& sx, flo
     const float t = (pos ? e : -e) * rot_rcp(fmaxf(den, 0x1p-126f));
     const float q = fmaf(t, t, 1.f);
     const float c = rot_rsq(q);
-    const float co = -t * (sy * ix), ka = t * (sx * iy) * (c * c);
+    const float co = -t * (gsy * ix);
+    // the second shear's coefficient without the held rows' reciprocal scale
+    const float w = (t * sx) * (c * c);
+    const float ic = q * c, tg = t * g;
     if constexpr (D == 0) {
 #pragma unroll
         for (int i = 0; i < H; ++i) x[i] = fmaf(co, y[i], x[i]);
+        const float ka = w * iy;
 #pragma unroll
         for (int i = 0; i < H; ++i) y[i] = fmaf(ka, x[i], y[i]);
     } else {
         dpp_axpy<D, H>(x, y, co);
-        dpp_axpy<D, H>(y, x, XorDpp<dpp_xor_ctrl<D>()>{}(ka));
+        dpp_axpy<D, H>(y, x, from_quad<D>(w) * iy);
     }
-    const float ic = q * c;
+    const float den2 = ax * gay;
     sx *= c;
     ix *= ic;
-    sy *= ic;
-    iy *= c;
-    // explicit FMAs: the lane of P forms a_Q as fma(t', g, a_Q) and the lane of Q
-    // as fma(-t', g, a_Q) with its t' = -t, the same rounding
-    const float den2 = ax * ay;
-    ax = fmaf(-t, g, ax);
-    ay = fmaf(t, g, ay);
+    ax = add_uncontracted(ax, -tg);
+    // the held rows take the rotation of lane ^ D, whose guest they are
+    sy *= from_quad<D>(ic);
+    iy *= from_quad<D>(c);
+    ay = add_uncontracted(ay, from_quad<D>(tg));
     if constexpr (!COS) return 0.f;
     return den2 > 0.f ? fabsf(g) * rot_rsq(den2) : 0.f;
-}
-
-// the guest's scales and norm follow the walk over the masks: from m to m ^ f the
-// lane's guest is the previous guest of lane ^ f
-template <class X>
-__device__ inline void move_guest_scalars(float& sy, float& iy, float& ay, X xch) {
-    sy = xch(sy);
-    iy = xch(iy);
-    ay = xch(ay);
 }
 
 // One cyclic sweep (every column pair once) of the FP32 one-sided Jacobi method
@@ -431,13 +438,10 @@ __device__ inline float hestenes_sweep(float (&u)[N], float& sc, float& isc, flo
         for (int k = 0;; k += 4) {
             mc = fmaxf(mc, shear_round<H, 0, COS>(x, y, sx, ix, ax, sy, iy, ay, hi, XorBperm{(lane ^ m) << 2}));
             m ^= 1;
-            move_guest_scalars(sy, iy, ay, XorDpp<DPP_XOR1>{});
             mc = fmaxf(mc, shear_round<H, 1, COS>(x, y, sx, ix, ax, sy, iy, ay, hi, XorBperm{(lane ^ m) << 2}));
             m ^= 2;
-            move_guest_scalars(sy, iy, ay, XorDpp<DPP_XOR2>{});
             mc = fmaxf(mc, shear_round<H, 3, COS>(x, y, sx, ix, ax, sy, iy, ay, hi, XorBperm{(lane ^ m) << 2}));
             m ^= 1;
-            move_guest_scalars(sy, iy, ay, XorDpp<DPP_XOR1>{});
             mc = fmaxf(mc, shear_round<H, 2, COS>(x, y, sx, ix, ax, sy, iy, ay, hi, XorBperm{(lane ^ m) << 2}));
             // the guest rows move once, on either path: after the group's last round
             // the other half of the lane's column (the guest of lane ^ m, in lane
@@ -448,18 +452,15 @@ __device__ inline float hestenes_sweep(float (&u)[N], float& sc, float& isc, flo
             const XorBperm mv{(lane ^ (last ? m : 1 << j) ^ 2) << 2};
 #pragma unroll
             for (int i = 0; i < H; ++i) y[i] = mv(y[i]);
+            // their scales and norm go with them
+            sy = mv(sy);
+            iy = mv(iy);
             if (last) break;
+            ay = mv(ay);
             m ^= 1 << j;
-            if (j == 3)
-                move_guest_scalars(sy, iy, ay, XorDpp<DPP_XOR8>{});
-            else
-                move_guest_scalars(sy, iy, ay, XorBperm{(lane ^ (1 << j)) << 2});
         }
-        {
-            const XorBperm om{(lane ^ m) << 2};
-            so = om(sy);
-            io = om(iy);
-        }
+        so = sy;
+        io = iy;
         if (b == 2) break;
         // into group b - 1: keep the half of the column that its split gives the lane,
         // send the other half (and its scales) to lane ^ 2^(b-1), receive that lane's
