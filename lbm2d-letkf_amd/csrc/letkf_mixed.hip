@@ -388,10 +388,21 @@ __device__ inline float shear_round(float (&x)[H], float (&y)[H], float& sx, flo
     sx *= c;
     ix *= ic;
     ax = add_uncontracted(ax, -tg);
-    // the held rows take the rotation of lane ^ D, whose guest they are
-    sy *= from_quad<D>(ic);
-    iy *= from_quad<D>(c);
-    ay = add_uncontracted(ay, from_quad<D>(tg));
+    // the held rows take the rotation of lane ^ D, whose guest they are (in the
+    // block's last round as DPP source operands written out: the compiler leaves
+    // the moves there unfolded)
+    if constexpr (D == 2) {
+        asm("s_nop 1\n\t"
+            "v_mul_f32_dpp %0, %3, %0 quad_perm:[2,3,0,1] row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+            "v_mul_f32_dpp %1, %4, %1 quad_perm:[2,3,0,1] row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+            "v_add_f32_dpp %2, %5, %2 quad_perm:[2,3,0,1] row_mask:0xf bank_mask:0xf bound_ctrl:1"
+            : "+v"(sy), "+v"(iy), "+v"(ay)
+            : "v"(ic), "v"(c), "v"(tg));
+    } else {
+        sy *= from_quad<D>(ic);
+        iy *= from_quad<D>(c);
+        ay = add_uncontracted(ay, from_quad<D>(tg));
+    }
     if constexpr (!COS) return 0.f;
     return den2 > 0.f ? fabsf(g) * rot_rsq(den2) : 0.f;
 }
