@@ -306,6 +306,15 @@ __device__ inline float hestenes_round(float (&u)[N], float& sc, float& isc, flo
     return den > 0.f ? fabsf(gam) * rot_rsq(den) : 0.f;
 }
 
+// a value of lane ^ D (D = 0: the lane's own)
+template <int D>
+__device__ __forceinline__ float from_quad(float v) {
+    if constexpr (D == 0)
+        return v;
+    else
+        return XorDpp<dpp_xor_ctrl<D>()>{}(v);
+}
+
 // One round with split columns.  For the pair (P, Q) = (p, p ^ m), p with bit b
 // of m clear, the lane of P holds rows [0, N/2) of column P in x[] and the lane of
 // Q rows [N/2, N) of column Q; the other half of each column (the lane's "guest"
@@ -330,21 +339,13 @@ __device__ inline float hestenes_round(float (&u)[N], float& sc, float& isc, flo
 // updates are in place and each may read the other lane through DPP:
 //   x' = x - t' (sy/sx) y,  then  y'' = y + t' c^2 (sx/sy) x',  sx' = sx c,  sy' = sy q c,
 // with q = 1 + t'^2 (c^2 = 1/q), which gives sx' x' = c (X - t' Y) and
-// sy' y'' = c (Y + t' X) for X = sx x, Y = sy y (the second shear's coefficient is
-// that of the lane that owns the rows, lane ^ D).  The two halves of a column take
+// sy' y'' = c (Y + t' X) for X = sx x, Y = sy y (the holder of the guest rows
+// applies the second shear with t' c^2 sx of lane ^ D, whose guest they are, and
+// its own 1/sy).  The two halves of a column take
 // the shears in opposite order, so they agree to the rounding of c^2 q = 1 and of
 // the tracked reciprocal scales, a few ulp per rotation: measured on nearly scalar
 // and rank-deficient problems, the sweeps converge as far as with whole-column
 // rotations (and as with 1/q and the reciprocals refined by a Newton step).
-// a value of lane ^ D (D = 0: the lane's own)
-template <int D>
-__device__ __forceinline__ float from_quad(float v) {
-    if constexpr (D == 0)
-        return v;
-    else
-        return XorDpp<dpp_xor_ctrl<D>()>{}(v);
-}
-
 template <int H, int D, bool COS>
 __device__ inline float shear_round(float (&x)[H], float (&y)[H], float& sx, float& ix, float& ax, float& sy,
                                     float& iy, float& ay, bool hi, XorBperm xch) {
@@ -389,8 +390,8 @@ __device__ inline float shear_round(float (&x)[H], float (&y)[H], float& sx, flo
     ix *= ic;
     ax = add_uncontracted(ax, -tg);
     // the held rows take the rotation of lane ^ D, whose guest they are (in the
-    // block's last round as DPP source operands written out: the compiler leaves
-    // the moves there unfolded)
+    // block's last round as DPP source operands written out, after the two wait
+    // states a DPP read needs: the compiler leaves the moves there unfolded)
     if constexpr (D == 2) {
         asm("s_nop 1\n\t"
             "v_mul_f32_dpp %0, %3, %0 quad_perm:[2,3,0,1] row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
